@@ -1,0 +1,131 @@
+/*
+ * tal_bsw.h -- C ABI of the MI355X banded Smith-Waterman extension engine.
+ *
+ * Drop-in for TAL's BandedPairWiseSW (src/alignment/BSW/bandedSWA.h:113-336):
+ * the same 56-byte SeqPair records, the same packed reference/query byte
+ * buffers addressed by per-pair offsets, the same scoring parameters, and the
+ * same six outputs written in place, bit-identical to the AVX-512BW build of
+ * getScores16 / getScores8.  Plain C types only; every call returns a status
+ * code and never exits the process.
+ *
+ * Group semantics.  Like the reference, getScores16 processes pairs in
+ * consecutive groups of 32 and getScores8 in groups of 64; pairs of one group
+ * share the row loop, the column range and the band narrowing, so a pair's
+ * outputs may depend on the pairs next to it.  The engine reproduces those
+ * groups exactly.  The last group of a call is padded with pairs of
+ * len1 = 0, h0 = 0, idr = idq = 0 and len2 = 0 (16-bit) or the last pair's
+ * len2 (8-bit); the reference pads the same lengths but reads h0/idq from
+ * whatever lies past the caller's array (bandedSWA.cpp:2710-2717,
+ * :2013-2018), and unlike the reference nothing is written past pairs[n-1].
+ *
+ * Persistent scratch.  The reference object keeps its 16-bit F scratch
+ * across groups and calls, and its tail scan can read column F[ncol] before
+ * the current group stores it (bandedSWA.cpp:3034-3035, :3267, :3293-3302).
+ * A handle keeps the same state on the device, starting from zero like a
+ * freshly constructed object, so a sequence of calls on one handle matches
+ * the same sequence on one reference object.
+ */
+#ifndef TAL_BSW_H
+#define TAL_BSW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. */
+#define TAL_OK 0
+#define TAL_EINVAL 1  /* bad argument (null pointer, n < 0, bad parameter) */
+#define TAL_ENOMEM 2  /* device or pinned-host allocation failed */
+#define TAL_EDEVICE 3 /* HIP runtime error; see tal_bsw_last_error */
+#define TAL_ERANGE 4  /* a pair exceeds the path's limits (see below) */
+
+/* SeqPair, bandedSWA.h:90-99.  idr/idq: byte offsets of the reference
+ * (rows, len1) and query (columns, len2) in their buffers; h0: initial score;
+ * seqid/regid: passed through.  Outputs: score, tle, gtle, qle, gscore,
+ * max_off. */
+typedef struct tal_seqpair {
+    int32_t idr, idq, id;
+    int32_t len1, len2;
+    int32_t h0;
+    int32_t seqid, regid;
+    int32_t score, tle, gtle, qle;
+    int32_t gscore, max_off;
+} tal_seqpair;
+
+/* Constructor arguments of BandedPairWiseSW (bandedSWA.cpp:50-94).
+ * w_mismatch is the positive penalty, as the constructor takes it; the
+ * ambiguity score is fixed at -1 (DEFAULT_AMBIG, bandedSWA.h:54).  The
+ * reference's mat/numThreads arguments only feed its scalar path and its
+ * per-thread scratch sizing, so they have no counterpart here. */
+typedef struct tal_bsw_params {
+    int32_t o_del, e_del, o_ins, e_ins;
+    int32_t zdrop, end_bonus;
+    int32_t w_match, w_mismatch;
+} tal_bsw_params;
+
+typedef struct tal_bsw tal_bsw;
+
+/* Replaces BandedPairWiseSW::BandedPairWiseSW (bandedSWA.cpp:50-94).
+ * Binds the handle to HIP device `device`.  e_del and e_ins must be nonzero
+ * (the reference divides by them, :2887-2903); w_match and w_mismatch must
+ * fit int8_t as in the reference signature. */
+int tal_bsw_create(const tal_bsw_params *params, int device, tal_bsw **out);
+
+/* Replaces BandedPairWiseSW::~BandedPairWiseSW (bandedSWA.cpp:97-100). */
+void tal_bsw_destroy(tal_bsw *h);
+
+/* Human-readable description of the last failing call on this handle. */
+const char *tal_bsw_last_error(const tal_bsw *h);
+
+/* Replaces BandedPairWiseSW::getScores16 (bandedSWA.cpp:2664-2688).
+ * pairs/ref/qer are host memory; ref_len/qer_len are the buffer extents
+ * (every idr+len1 <= ref_len, idq+len2 <= qer_len).  Limits: 0 <= len1,len2
+ * <= 32767 (MAX_SEQ_LEN16 - 1), w >= 0.  Blocking. */
+int tal_bsw_get_scores16(tal_bsw *h, tal_seqpair *pairs, const uint8_t *ref,
+                         size_t ref_len, const uint8_t *qer, size_t qer_len,
+                         int32_t n, int32_t w);
+
+/* Replaces BandedPairWiseSW::getScores8 (bandedSWA.cpp:1970-1995).  Limits:
+ * 0 <= len1,len2 <= 127 (MAX_SEQ_LEN8 - 1), w >= 0.  Blocking. */
+int tal_bsw_get_scores8(tal_bsw *h, tal_seqpair *pairs, const uint8_t *ref,
+                        size_t ref_len, const uint8_t *qer, size_t qer_len,
+                        int32_t n, int32_t w);
+
+/* 8/16-bit dispatcher.  The reference leaves this split to its caller
+ * (bench-bsw always calls getScores16, benchmarks/bench-bsw.cpp:253).  A pair
+ * goes to the 8-bit path iff len1 <= 127, len2 <= 127 and
+ * h0 + max(w_match,0) * min(len1,len2) <= 127, so no 8-bit score can wrap;
+ * the rest go to the 16-bit path.  Results equal getScores8 on the 8-bit pairs
+ * (in input order) followed by getScores16 on the others, on this handle. */
+int tal_bsw_get_scores_auto(tal_bsw *h, tal_seqpair *pairs, const uint8_t *ref,
+                            size_t ref_len, const uint8_t *qer, size_t qer_len,
+                            int32_t n, int32_t w);
+
+/* Device-resident variants: pairs/ref/qer are device pointers on the handle's
+ * device; outputs are written in place into the device pairs.  `stream` is a
+ * hipStream_t (NULL = the handle's own stream); the call returns once the
+ * work is queued, except that the 16-bit path may wait for a small status
+ * word when a group's stale-column read needs a rerun.  No host copies of the
+ * inputs are made. */
+int tal_bsw_get_scores16_device(tal_bsw *h, tal_seqpair *pairs,
+                                const uint8_t *ref, const uint8_t *qer,
+                                int32_t n, int32_t w, void *stream);
+int tal_bsw_get_scores8_device(tal_bsw *h, tal_seqpair *pairs,
+                               const uint8_t *ref, const uint8_t *qer,
+                               int32_t n, int32_t w, void *stream);
+int tal_bsw_get_scores_auto_device(tal_bsw *h, tal_seqpair *pairs,
+                                   const uint8_t *ref, const uint8_t *qer,
+                                   int32_t n, int32_t w, void *stream);
+
+/* Number of pairs of pairs[0..n) that the dispatcher sends to the 8-bit path
+ * (host memory). */
+int32_t tal_bsw_count_8bit(const tal_bsw *h, const tal_seqpair *pairs,
+                           int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

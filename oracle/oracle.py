@@ -1,0 +1,153 @@
+"""ctypes binding of the oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product library.  The oracle is a
+CPU restatement of TAL's BandedPairWiseSW (src/alignment/BSW/bandedSWA.cpp);
+see bsw_oracle.h for what each entry point restates.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+# SeqPair, bandedSWA.h:90-99 (14 x int32 = 56 bytes)
+SEQPAIR_FIELDS = ["idr", "idq", "id", "len1", "len2", "h0", "seqid", "regid",
+                  "score", "tle", "gtle", "qle", "gscore", "max_off"]
+SEQPAIR_DTYPE = np.dtype([(f, "<i4") for f in SEQPAIR_FIELDS])
+OUT_FIELDS = ["score", "tle", "gtle", "qle", "gscore", "max_off"]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        i32, i8 = ctypes.c_int32, ctypes.c_int8
+        L.bsw_oracle_create.restype = P
+        L.bsw_oracle_create.argtypes = [ctypes.c_int] * 6 + [i8, i8]
+        L.bsw_oracle_destroy.argtypes = [P]
+        for name in ("bsw_oracle_get_scores16", "bsw_oracle_get_scores8",
+                     "bsw_oracle_scalar"):
+            getattr(L, name).argtypes = [P, P, P, P, i32, i32]
+        L.bsw_oracle_band_dp.restype = ctypes.c_int
+        L.bsw_oracle_band_dp.argtypes = [P, ctypes.c_int, P, ctypes.c_int, P,
+                                         i32, ctypes.c_int] + [P] * 5
+        L.bsw_oracle_set_stale_mode.argtypes = [P, ctypes.c_int]
+        L.bsw_oracle_set_mutation.argtypes = [P, ctypes.c_int]
+        L.bsw_oracle_stats.argtypes = [P, P]
+        L.bsw_oracle_f16_nonzero_mask.restype = ctypes.c_uint32
+        L.bsw_oracle_f16_nonzero_mask.argtypes = [P, ctypes.c_int]
+        L.bsw_oracle_band_cells.restype = ctypes.c_uint64
+        L.bsw_oracle_band_cells.argtypes = [P, ctypes.c_int, ctypes.c_int, i32]
+        L.bsw_oracle_run_mt.restype = ctypes.c_int
+        L.bsw_oracle_run_mt.argtypes = [ctypes.c_int] * 6 + [i8, i8, ctypes.c_int,
+                                        P, P, P, i32, i32, i32, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Oracle:
+    """One restated BandedPairWiseSW object (its scratch persists across calls,
+    as the reference object's does)."""
+
+    def __init__(self, o_del=6, e_del=1, o_ins=6, e_ins=1, zdrop=100,
+                 end_bonus=5, w_match=1, w_mismatch=4):
+        self.params = (o_del, e_del, o_ins, e_ins, zdrop, end_bonus, w_match, w_mismatch)
+        self._h = lib().bsw_oracle_create(*self.params)
+        if not self._h:
+            raise MemoryError("bsw_oracle_create failed")
+
+    def close(self):
+        if self._h:
+            lib().bsw_oracle_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _run(self, fn, pairs, ref, qer, w):
+        assert pairs.dtype == SEQPAIR_DTYPE and pairs.flags.c_contiguous
+        ref = np.ascontiguousarray(ref, dtype=np.uint8)
+        qer = np.ascontiguousarray(qer, dtype=np.uint8)
+        fn(self._h, _ptr(pairs), _ptr(ref), _ptr(qer), len(pairs), int(w))
+        return pairs
+
+    def scores16(self, pairs, ref, qer, w):
+        return self._run(lib().bsw_oracle_get_scores16, pairs, ref, qer, w)
+
+    def scores8(self, pairs, ref, qer, w):
+        return self._run(lib().bsw_oracle_get_scores8, pairs, ref, qer, w)
+
+    def scalar(self, pairs, ref, qer, w):
+        return self._run(lib().bsw_oracle_scalar, pairs, ref, qer, w)
+
+    def band_dp(self, query, target, w, h0):
+        q = np.ascontiguousarray(query, dtype=np.uint8)
+        t = np.ascontiguousarray(target, dtype=np.uint8)
+        outs = [ctypes.c_int() for _ in range(5)]
+        s = lib().bsw_oracle_band_dp(self._h, len(q), _ptr(q), len(t), _ptr(t),
+                                     int(w), int(h0), *[ctypes.byref(x) for x in outs])
+        qle, tle, gtle, gscore, max_off = (x.value for x in outs)
+        return dict(score=s, qle=qle, tle=tle, gtle=gtle, gscore=gscore, max_off=max_off)
+
+    def set_stale_mode(self, mode):
+        lib().bsw_oracle_set_stale_mode(self._h, int(mode))
+
+    def set_mutation(self, m):
+        """Plants a known mistake (tests of the pins only)."""
+        lib().bsw_oracle_set_mutation(self._h, int(m))
+
+    def stats(self):
+        out = np.zeros(3, dtype=np.uint64)
+        lib().bsw_oracle_stats(self._h, _ptr(out))
+        return dict(groups=int(out[0]), stale_decisive=int(out[1]), stale_nonzero=int(out[2]))
+
+    def f16_nonzero_mask(self, col):
+        return int(lib().bsw_oracle_f16_nonzero_mask(self._h, int(col)))
+
+    def band_cells(self, len1, len2, w):
+        return int(lib().bsw_oracle_band_cells(self._h, int(len1), int(len2), int(w)))
+
+
+def run_mt(pairs, ref, qer, w, bits=16, nthreads=1, chunk=20000, params=None):
+    """Each thread owns one oracle object and takes contiguous chunks."""
+    p = params or (6, 1, 6, 1, 100, 5, 1, 4)
+    ref = np.ascontiguousarray(ref, dtype=np.uint8)
+    qer = np.ascontiguousarray(qer, dtype=np.uint8)
+    rc = lib().bsw_oracle_run_mt(*p, int(bits), _ptr(pairs), _ptr(ref), _ptr(qer),
+                                 len(pairs), int(w), int(chunk), int(nthreads))
+    if rc != 0:
+        raise MemoryError("bsw_oracle_run_mt failed")
+    return pairs
+
+
+def make_pairs(refs, queries, h0s):
+    """Pack python lists of code arrays into (pairs, ref_buf, qer_buf)."""
+    n = len(refs)
+    pairs = np.zeros(n, dtype=SEQPAIR_DTYPE)
+    rlen = np.array([len(r) for r in refs], dtype=np.int64)
+    qlen = np.array([len(q) for q in queries], dtype=np.int64)
+    roff = np.concatenate([[0], np.cumsum(rlen)[:-1]]) if n else np.zeros(0, np.int64)
+    qoff = np.concatenate([[0], np.cumsum(qlen)[:-1]]) if n else np.zeros(0, np.int64)
+    ref = np.zeros(max(1, int(rlen.sum())), dtype=np.uint8)
+    qer = np.zeros(max(1, int(qlen.sum())), dtype=np.uint8)
+    for k in range(n):
+        ref[roff[k]:roff[k] + rlen[k]] = refs[k]
+        qer[qoff[k]:qoff[k] + qlen[k]] = queries[k]
+    pairs["idr"], pairs["idq"], pairs["id"] = roff, qoff, np.arange(n)
+    pairs["len1"], pairs["len2"], pairs["h0"] = rlen, qlen, h0s
+    pairs["seqid"] = pairs["regid"] = -1
+    for f in OUT_FIELDS:
+        pairs[f] = -1
+    return pairs, ref, qer

@@ -1,0 +1,72 @@
+"""C ABI checks that need no GPU: the library loads, exports every entry
+point include/tal_bsw.h declares, the SeqPair layout is the reference's, and
+argument errors are reported without touching a device."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import tal_bsw as T
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "tal_bsw.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(tal_bsw_\w+)\s*\(", text)))
+
+
+def test_header_declares_binding_exports():
+    assert _declared() == sorted(T.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(T.LIB_PATH)
+    for name in _declared():
+        assert hasattr(lib, name), name
+
+
+def test_seqpair_layout(tmp_path):
+    src = tmp_path / "layout.c"
+    src.write_text("""
+#include <stddef.h>
+#include "tal_bsw.h"
+_Static_assert(sizeof(tal_seqpair) == 56, "size");
+_Static_assert(offsetof(tal_seqpair, h0) == 20, "h0");
+_Static_assert(offsetof(tal_seqpair, score) == 32, "score");
+_Static_assert(offsetof(tal_seqpair, max_off) == 52, "max_off");
+_Static_assert(sizeof(tal_bsw_params) == 32, "params");
+int main(void) { return 0; }
+""")
+    subprocess.check_call(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), "-c", str(src),
+                           "-o", str(tmp_path / "layout.o")])
+    assert T.SEQPAIR_DTYPE.itemsize == 56
+
+
+def test_cpp_shim_compiles(tmp_path):
+    src = tmp_path / "shim.cpp"
+    src.write_text('#include "bandedSWA_tal.hpp"\nint main(){return 0;}\n')
+    subprocess.check_call(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                           str(src)])
+
+
+def test_create_rejects_bad_params():
+    L = T.load()
+    h = ctypes.c_void_p()
+    assert L.tal_bsw_create(None, 0, ctypes.byref(h)) == T.TAL_EINVAL
+    p = T.Params(6, 0, 6, 1, 100, 5, 1, 4)  # e_del = 0 would divide by zero
+    assert L.tal_bsw_create(ctypes.byref(p), 0, ctypes.byref(h)) == T.TAL_EINVAL
+    p = T.Params(6, 1, 6, 1, 100, 5, 300, 4)  # w_match must fit int8
+    assert L.tal_bsw_create(ctypes.byref(p), 0, ctypes.byref(h)) == T.TAL_EINVAL
+    assert not h.value
+
+
+def test_null_handle_calls():
+    L = T.load()
+    assert L.tal_bsw_count_8bit(None, None, 0) == -1
+    assert L.tal_bsw_get_scores16(None, None, None, 0, None, 0, 0, 100) == T.TAL_EINVAL
+    assert L.tal_bsw_last_error(None) == b"null handle"

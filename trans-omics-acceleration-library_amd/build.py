@@ -1,0 +1,59 @@
+"""Builds the native libraries of the package in-tree.
+
+libtal_bsw.so   : HIP kernels + C ABI (include/tal_bsw.h), gfx950 only.
+libtal_synth.so : deterministic synthetic workloads (csrc/bsw_synth.c).
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = os.environ.get("TAL_BSW_ARCH", "gfx950")
+
+HIP_SOURCES = ["bsw_kernels.hip", "tal_bsw.hip"]
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+    f"-I{os.path.join(ROOT, 'include')}",
+]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
+def build_hip(out=None, extra=()):
+    out = out or os.path.join(HERE, "libtal_bsw.so")
+    objs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(CSRC, src.replace(".hip", ".o"))
+        _run([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", out])
+    for o in objs:
+        os.remove(o)
+    return out
+
+
+def build_synth(out=None):
+    out = out or os.path.join(HERE, "libtal_synth.so")
+    _run(["gcc", "-O3", "-fPIC", "-fopenmp", "-shared", "-std=gnu11",
+          os.path.join(CSRC, "bsw_synth.c"), "-o", out, "-lm"])
+    return out
+
+
+def build_all():
+    build_synth()
+    build_hip()
+
+
+if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "synth":
+        build_synth()
+    else:
+        build_all()

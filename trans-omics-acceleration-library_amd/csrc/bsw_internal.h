@@ -1,0 +1,60 @@
+/*
+ * bsw_internal.h -- device-side data structures shared by the kernels
+ * (bsw_kernels.hip) and the host runtime (tal_bsw.hip).
+ */
+#ifndef TAL_BSW_INTERNAL_H
+#define TAL_BSW_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/tal_bsw.h"
+
+namespace talbsw {
+
+/* Largest query length the 16-bit path accepts (MAX_SEQ_LEN16 - 1); the
+ * per-handle stale-column image covers columns 0..MAX_COL16. */
+constexpr int MAX_LEN16 = 32767;
+constexpr int MAX_LEN8 = 127;
+constexpr int IMAGE_COLS = MAX_LEN16 + 1;
+
+/* Scoring and band parameters, pre-scaled for the lane width: 16-bit lanes
+ * hold the value itself, 8-bit lanes hold value << 8 in an int16 so that
+ * 16-bit add/sub/max/min wrap and compare exactly as epi8 does. */
+struct KParams {
+    int32_t n;        /* pairs in this call */
+    int32_t ngroups;  /* ceil(n / group lanes) */
+    int32_t ntasks;   /* wave tasks in this launch */
+    int32_t w;        /* band width argument */
+    int32_t o_del, e_del, o_ins, e_ins, end_bonus;
+    int32_t smax;      /* max(0, w_match, -w_mismatch, ambig) */
+    int32_t last_len2; /* 8-bit padding lanes take the last pair's len2 */
+    int32_t state_cols; /* columns per global state slab (0 = LDS state) */
+    int16_t match, mism, ambig;
+    int16_t e_del_s, oe_del_s, e_ins_s, oe_ins_s, zdrop_s;
+};
+
+struct KBuffers {
+    tal_seqpair *pairs;
+    const uint8_t *ref;
+    const uint8_t *qer;
+    const int32_t *task_groups; /* ntasks * groups-per-wave ids, -1 = none;
+                                   null = identity order */
+    int32_t *task_counter;      /* persistent-grid work counter */
+    uint32_t *slabs;            /* global state slabs when not in LDS */
+    /* 16-bit stale-column bookkeeping (all null for the 8-bit path) */
+    const uint32_t *stale_in; /* per group: lanes whose F[ncol] is nonzero */
+    uint32_t *dep_out;        /* per group: lanes whose tail depended on it */
+    uint8_t *wrote_out;       /* per group: F[ncol] stored during the group */
+    uint32_t *fmask;          /* per group final F nonzero masks, c < ncol */
+    const int64_t *fmask_off; /* per group offset into fmask */
+};
+
+/* Per-group shape: rows (max len1) and columns (max len2) of the group. */
+struct GroupShape {
+    int32_t nrow, ncol;
+};
+
+} // namespace talbsw
+
+#endif

@@ -1,0 +1,606 @@
+/*
+ * bsw_kernels.hip -- gfx950 kernels of the banded Smith-Waterman extension.
+ *
+ * Mapping.  The reference vectorises across pairs: one AVX-512 lane per pair,
+ * 32 pairs (16-bit) or 64 pairs (8-bit) per group sharing one row loop
+ * (src/alignment/BSW/bandedSWA.cpp:2962-3356 and :2263-2661).  Here one
+ * wavefront lane is one pair; a 64-lane wave runs two 16-bit groups (one per
+ * half-wave) or one 8-bit group.  Group-wide decisions of the reference
+ * (column range, band narrowing, all-lanes-zero exit, head/tail scans) become
+ * ballots over the group's lanes.  Control flow is uniform inside a group, so
+ * the two groups of a 16-bit wave diverge only as whole half-waves.
+ *
+ * State.  The reference's per-group H_h and F rows ([col][lane] int16/int8)
+ * live in one dword per (column, lane): low half H_h, high half F.  They sit
+ * in LDS when the wave's widest group fits, else in a per-block slab of
+ * global memory.  H_v (the first-column boundary) is recomputed from its
+ * closed form instead of being stored.
+ *
+ * Widths.  16-bit lanes use int16 arithmetic, which wraps exactly like the
+ * epi16 instructions.  8-bit lanes keep value << 8 in an int16: add, sub,
+ * signed max/compare and unsigned min then behave exactly like epi8 on the
+ * high byte, so one code path serves both widths.
+ */
+#include "bsw_internal.h"
+
+namespace talbsw {
+
+template <int BITS> struct Width {
+    static constexpr int GL = BITS == 16 ? 32 : 64; /* lanes per group */
+    static constexpr int GPW = 64 / GL;             /* groups per wave */
+    static constexpr int SH = BITS == 16 ? 0 : 8;   /* value scale shift */
+    /* Wrap an integer to the lane width and scale it. */
+    static __device__ __forceinline__ int16_t v(int x)
+    {
+        return (int16_t)(uint16_t)((uint32_t)x << SH);
+    }
+    static __device__ __forceinline__ int out(int16_t s) { return (int)s >> SH; }
+};
+
+__device__ __forceinline__ int16_t smax16(int16_t a, int16_t b) { return a > b ? a : b; }
+__device__ __forceinline__ int16_t smin16(int16_t a, int16_t b) { return a < b ? a : b; }
+__device__ __forceinline__ int16_t umin16(int16_t a, int16_t b)
+{
+    return (uint16_t)a < (uint16_t)b ? a : b;
+}
+__device__ __forceinline__ int16_t add16(int16_t a, int16_t b) { return (int16_t)(a + b); }
+__device__ __forceinline__ int16_t sub16(int16_t a, int16_t b) { return (int16_t)(a - b); }
+__device__ __forceinline__ int16_t abs16(int16_t a) { return a < 0 ? (int16_t)(-(int)a) : a; }
+
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+
+/* Reduction over the lanes of one group (xor offsets stay inside it). */
+template <int GL> __device__ __forceinline__ int group_max(int x)
+{
+#pragma unroll
+    for (int o = GL / 2; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, 64));
+    return x;
+}
+
+/* Score of ref code s1 against query code s2 (raw bytes; pads are 99/100).
+ * 16-bit: code 4 maps to 0xFFFF, so only code 4 has the top bit.
+ * 8-bit : code 4 maps to 0xFF, and any byte >= 128 also has the top bit. */
+template <int BITS>
+__device__ __forceinline__ int16_t score_of(int s1, int s2, const KParams &K)
+{
+    bool amb;
+    if (BITS == 16)
+        amb = (s1 == 4) | (s2 == 4);
+    else
+        amb = (s1 == 4) | (s2 == 4) | (s1 >= 128) | (s2 >= 128);
+    int16_t s = (s1 == s2) ? K.match : K.mism;
+    return amb ? K.ambig : s;
+}
+
+/* One dword of state per (column, lane). */
+struct StateRef {
+    uint32_t *base; /* column 0 of this lane: base[c * 64] */
+    __device__ __forceinline__ uint32_t ld(int c) const { return base[(size_t)c * 64]; }
+    __device__ __forceinline__ void st(int c, uint32_t v) const { base[(size_t)c * 64] = v; }
+};
+
+__device__ __forceinline__ uint32_t pack(int16_t h, int16_t f)
+{
+    return (uint32_t)(uint16_t)h | ((uint32_t)(uint16_t)f << 16);
+}
+
+/*
+ * Runs group g (or nothing when g < 0) on this half-wave / wave.  The body is
+ * the reference's smithWaterman512_{16,8} row loop with its batch wrapper's
+ * per-group setup folded in.
+ */
+template <int BITS>
+__device__ void run_group(const KParams &K, const KBuffers &B, int g, StateRef S)
+{
+    typedef Width<BITS> Wd;
+    constexpr int GL = Wd::GL;
+    const int lane = threadIdx.x;
+    const int half = lane / GL;
+    const int gl = lane % GL;
+    const uint64_t gmask = GL == 64 ? ~0ull : (0xFFFFFFFFull << (32 * half));
+    const int16_t ONE = Wd::v(1), TWO = Wd::v(2);
+
+    /* ---- per-lane pair (wrapper :2787-2826 / :2092-2120) ---- */
+    const bool gvalid = g >= 0;
+    const int p = gvalid ? g * GL + gl : 0;
+    const bool real = gvalid && p < K.n;
+    int len1 = 0, len2 = 0, h0 = 0, idr = 0, idq = 0;
+    if (real) {
+        const tal_seqpair *sp = B.pairs + p;
+        idr = sp->idr;
+        idq = sp->idq;
+        len1 = sp->len1;
+        len2 = sp->len2;
+        h0 = sp->h0;
+    } else if (gvalid && BITS == 8) {
+        len2 = K.last_len2; /* :2017 */
+    }
+    const int nrow = group_max<GL>(len1);
+    const int ncol = group_max<GL>(len2);
+
+    /* per-lane band (:2882-2904): integer division then +1, floor 1, cap w */
+    int qlm = BITS == 16 ? (int)(uint16_t)(len2 * K.smax) : (int)(uint8_t)(len2 * K.smax);
+    int tb = BITS == 16 ? (int)(uint16_t)(qlm + K.end_bonus - K.o_ins)
+                        : (int)(uint8_t)(qlm + K.end_bonus - K.o_ins);
+    int b1 = tb / K.e_ins + 1;
+    b1 = b1 > 1 ? b1 : 1;
+    int band = K.w < b1 ? K.w : b1;
+    band = BITS == 16 ? (int)(uint16_t)band : (int)(uint8_t)band;
+    tb = BITS == 16 ? (int)(uint16_t)(qlm + K.end_bonus - K.o_del)
+                    : (int)(uint8_t)(qlm + K.end_bonus - K.o_del);
+    int b2 = tb / K.e_del + 1;
+    b2 = b2 > 1 ? b2 : 1;
+    band = band < b2 ? band : b2;
+
+    const int16_t h0w = Wd::v(h0);
+    const int16_t tlen = Wd::v(len1), qlen = Wd::v(len2), bandv = Wd::v(band);
+    int16_t head = 0, tail = qlen;
+    const int16_t mlen = umin16(add16(qlen, bandv), tlen);
+    int16_t maxScore = h0w, x = 0, y = 0, gscore = Wd::v(-1), max_off = 0, max_ie = 0;
+    bool live = true; /* exit0 */
+
+    /* ---- H_h row -1 and F (:2867-2879, :3034) ---- */
+    const bool stale = BITS == 16 && B.stale_in && gvalid && ((B.stale_in[g] >> gl) & 1u);
+    {
+        int16_t hv = h0w;
+        for (int c = 0; c <= ncol; c++) {
+            if (c == 1)
+                hv = h0w > K.oe_ins_s ? sub16(h0w, K.oe_ins_s) : (int16_t)0;
+            else if (c >= 2)
+                hv = (c < ncol) ? smax16(sub16(hv, K.e_ins_s), 0) : (int16_t)0;
+            const int16_t f = (c < ncol) ? (int16_t)0 : (int16_t)(stale ? 1 : 0);
+            S.st(c, pack(hv, f));
+        }
+    }
+
+    int beg = 0, end = ncol, nbeg = 0, nend = ncol;
+    bool gdone = !gvalid;
+    bool fwritten = false;
+    uint64_t dep = 0;
+    const uint8_t *rseq = B.ref + (int64_t)idr;
+    const uint8_t *qseq = B.qer + (int64_t)idq;
+
+    for (int i = 0;; i++) {
+        const bool ract = !gdone && i < nrow;
+        if (ballot(ract) == 0) break;
+        if (!ract) {
+            gdone = true;
+            continue;
+        }
+        /* column range (:3059-3063) */
+        beg = nbeg;
+        end = nend;
+        if (beg < i - K.w) beg = i - K.w;
+        if (end > i + K.w + 1) end = i + K.w + 1;
+        if (end > ncol) end = ncol;
+        const int16_t i1 = Wd::v(i + 1);
+        int16_t e11 = 0, maxRS = 0, y1 = 0;
+        int16_t h10 = 0;
+        if (beg == 0) { /* H_v[i+1] (:2828-2837, :3065-3067) */
+            if (i + 1 < nrow)
+                h10 = smax16(Wd::v(h0 - K.o_del - (i + 1) * K.e_del), 0);
+            else
+                h10 = Wd::v(99); /* DUMMY1 left at H_v[maxLen1] (:2821-2825) */
+        }
+        /* per-lane band head/tail (:3080-3088) */
+        {
+            const int16_t ph = head, pt = tail;
+            head = smax16(head, sub16(Wd::v(i), bandv));
+            tail = umin16(tail, add16(i1, bandv));
+            tail = umin16(tail, qlen);
+            const bool same = (ballot(head == ph && tail == pt) & gmask) == gmask;
+            if (!same) { /* trim (:3092-3112) */
+                for (int c = beg; c < end; c++) {
+                    const bool below = head > Wd::v(c);
+                    if ((ballot(below) & gmask) == 0) break;
+                    if (below || Wd::v(c + 1) > tail) S.st(c, 0u);
+                }
+            }
+        }
+        if (i1 > mlen || tail == head || head > tail) live = false; /* :3118-3129 */
+
+        /* the column loop (:3134-3196) */
+        const int s1 = i < len1 ? (int)rseq[i] : 99;
+        int16_t jj = Wd::v(beg);
+        int j = beg;
+        for (; j < end; j++) {
+            const int16_t pj = jj;
+            jj = add16(jj, ONE);
+            const uint32_t sv = S.ld(j);
+            const int16_t h00 = (int16_t)(sv & 0xFFFFu), f11 = (int16_t)(sv >> 16);
+            const int s2 = j < len2 ? (int)qseq[j] : 100;
+            const int16_t sbt = score_of<BITS>(s1, s2, K);
+            const int16_t m = h00 == 0 ? (int16_t)0 : add16(h00, sbt);
+            const int16_t h11 = smax16(smax16(m, e11), f11);
+            e11 = smax16(smax16(sub16(m, K.oe_ins_s), 0), sub16(e11, K.e_ins_s));
+            int16_t f21 = smax16(smax16(sub16(m, K.oe_del_s), 0), sub16(f11, K.e_del_s));
+            const bool left = head > pj;
+            const bool out1 = (pj > tail) || left;
+            const int16_t hst = out1 ? (int16_t)0 : h10;
+            if (out1) f21 = 0;
+            const bool out2 = (jj > tail) || left;
+            if (!out2) {
+                if (h11 >= maxRS) y1 = jj;
+                maxRS = smax16(maxRS, h11);
+            }
+            S.st(j, pack(hst, f21));
+            h10 = h11;
+            if (jj == qlen && live && !(jj > tail)) { /* gscore (:3176-3195) */
+                if (!(gscore > h11)) max_ie = i1;
+                gscore = smax16(gscore, h11);
+            }
+        }
+        { /* boundary store at j (end, or beg when the loop was empty) */
+            const bool out1 = (jj > tail) || (head > jj);
+            S.st(j, pack(out1 ? (int16_t)0 : h10, 0));
+            if (j == ncol) fwritten = true;
+        }
+
+        /* all lanes zero in this row: the group stops (:3205-3209) */
+        if ((ballot(maxRS == 0) & gmask) == gmask) {
+            gdone = true;
+            continue;
+        }
+        if (maxRS == 0) live = false;
+        {
+            const int16_t before = maxScore;
+            if (live) maxScore = smax16(maxScore, maxRS);
+            if (maxScore > before) { /* :3217-3226 */
+                y = y1;
+                x = i1;
+                max_off = smax16(max_off, abs16(sub16(y1, i1)));
+            }
+            /* z-drop (ZSCORE, :1868-1881) */
+            const int16_t di = sub16(i1, x), dj = sub16(y1, y);
+            const int16_t off = di > dj ? sub16(di, dj) : sub16(dj, di);
+            const int16_t drop = sub16(sub16(maxScore, maxRS), off);
+            if (drop > K.zdrop_s) live = false;
+        }
+
+        /* narrowing of the group's range (:3239-3260) */
+        int c;
+        for (c = beg; c < end; c++) {
+            if ((ballot(S.ld(c) == 0u) & gmask) == gmask)
+                nbeg = c;
+            else
+                break;
+        }
+        for (c = end; c >= beg; c--)
+            if ((ballot(S.ld(c) == 0u) & gmask) != gmask) break;
+        nend = (c + 2 < ncol) ? c + 2 : ncol;
+        if (BITS == 16) { /* :3267, 16-bit AVX-512 only */
+            beg = nbeg;
+            end = nend;
+        }
+
+        /* head scan (:3269-3291) */
+        uint64_t still = gmask;
+        int16_t lc = Wd::v(beg);
+        for (c = beg; c < end; c++) {
+            uint64_t z = ballot(S.ld(c) == 0u && live) & gmask;
+            if (z == 0) break;
+            z &= still;
+            lc = add16(lc, ONE);
+            if ((z >> lane) & 1ull) head = lc;
+            still = z;
+        }
+
+        /* 16-bit: is the tail scan about to read a stale F[ncol]?  A live
+         * lane with H_h[ncol] == 0 then depends on it iff "nonzero at ncol"
+         * (tail + 2) and "zero at ncol" (walk down) give different tails. */
+        if (BITS == 16 && end == ncol && !fwritten && end >= beg) {
+            bool dec = false;
+            if (live && (S.ld(ncol) & 0xFFFFu) == 0u) {
+                const int16_t tnz = smin16(add16(tail, TWO), qlen);
+                int16_t ix = Wd::v(ncol - 1);
+                for (int cc = ncol - 1; cc >= beg; cc--) {
+                    if (S.ld(cc) != 0u) break;
+                    ix = Wd::v(cc - 1);
+                }
+                const int16_t tz = smin16(add16(ix, TWO), qlen);
+                dec = tnz != tz;
+            }
+            dep |= ballot(dec) & gmask;
+        }
+
+        /* tail scan (:3293-3316) */
+        int16_t idx = tail;
+        still = gmask;
+        lc = Wd::v(end);
+        for (c = end; c >= beg; c--) {
+            uint64_t z = ballot(S.ld(c) == 0u && live) & gmask;
+            if (z == 0) break;
+            z &= still;
+            lc = sub16(lc, ONE);
+            if ((z >> lane) & 1ull) idx = lc;
+            still = z;
+        }
+        tail = smin16(add16(idx, TWO), qlen);
+    }
+
+    if (real) { /* outputs (:3327-3353) */
+        tal_seqpair *sp = B.pairs + p;
+        sp->score = Wd::out(maxScore);
+        sp->tle = Wd::out(x);
+        sp->qle = Wd::out(y);
+        sp->max_off = Wd::out(max_off);
+        sp->gscore = Wd::out(gscore);
+        sp->gtle = Wd::out(max_ie);
+    }
+    if (BITS == 16 && gvalid && B.dep_out) {
+        const uint32_t d = (uint32_t)(dep >> (32 * half));
+        const bool wr = fwritten;
+        if (gl == 0) {
+            B.dep_out[g] = d;
+            B.wrote_out[g] = wr ? 1 : 0;
+        }
+        uint32_t *fm = B.fmask + B.fmask_off[g];
+        for (int c = 0; c < ncol; c++) {
+            const uint64_t nz = ballot((S.ld(c) >> 16) != 0u);
+            if (gl == 0) fm[c] = (uint32_t)(nz >> (32 * half));
+        }
+    }
+}
+
+/* Persistent grid: each 64-thread block is one wave that pulls tasks. */
+template <int BITS, bool LDS>
+__global__ __launch_bounds__(64) void bsw_main(KParams K, KBuffers B)
+{
+    extern __shared__ uint32_t lds_state[];
+    typedef Width<BITS> Wd;
+    const int lane = threadIdx.x;
+    const int half = lane / Wd::GL;
+    uint32_t *base = LDS ? lds_state : B.slabs + (size_t)blockIdx.x * K.state_cols * 64;
+    StateRef S{base + lane};
+    for (;;) {
+        int t = 0;
+        if (lane == 0) t = atomicAdd(B.task_counter, 1);
+        t = __shfl(t, 0, 64);
+        if (t >= K.ntasks) break;
+        int g;
+        if (B.task_groups)
+            g = B.task_groups[t * Wd::GPW + half];
+        else {
+            g = t * Wd::GPW + half;
+            if (g >= K.ngroups) g = -1;
+        }
+        run_group<BITS>(K, B, g, S);
+    }
+}
+
+/* Group shapes: rows/columns of each group and the 8-bit padding length. */
+template <int GL>
+__global__ void bsw_shapes(const tal_seqpair *pairs, int32_t n, int32_t ngroups,
+                           int32_t pad_len2, GroupShape *out, int64_t *ncol_plus1)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    int nr = 0, nc = 0;
+    for (int l = 0; l < GL; l++) {
+        const int p = g * GL + l;
+        int l1 = 0, l2 = pad_len2;
+        if (p < n) {
+            l1 = pairs[p].len1;
+            l2 = pairs[p].len2;
+            /* a negative length is reported as an oversized group */
+            if (l1 < 0 || l2 < 0) l1 = l2 = 0x7FFFFFFF;
+        }
+        nr = max(nr, l1);
+        nc = max(nc, l2);
+    }
+    out[g].nrow = nr;
+    out[g].ncol = nc;
+    if (ncol_plus1) ncol_plus1[g] = nc + 1;
+}
+
+/* Stale-column resolution for the 16-bit path.  For each group that depended
+ * on F[ncol], find the last earlier group that stored that column (a group h
+ * stores columns [0, ncol_h) at its start and column ncol_h iff wrote_h), and
+ * compare its final F zero-mask there with the mask the group assumed. */
+__global__ void bsw_resolve(int32_t ngroups, const GroupShape *shape, const uint32_t *dep,
+                            const uint8_t *wrote, const uint32_t *fmask,
+                            const int64_t *fmask_off, const uint32_t *image,
+                            uint32_t *stale, int32_t *rerun_list, int32_t *rerun_count)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    const uint32_t d = dep[g];
+    if (d == 0u) return;
+    const int c = shape[g].ncol;
+    uint32_t truth = image[c];
+    for (int h = g - 1; h >= 0; h--) {
+        const int nh = shape[h].ncol;
+        if (nh > c) {
+            truth = fmask[fmask_off[h] + c];
+            break;
+        }
+        if (nh == c && wrote[h]) {
+            truth = 0u;
+            break;
+        }
+    }
+    if ((truth & d) != (stale[g] & d)) {
+        stale[g] = truth;
+        const int k = atomicAdd(rerun_count, 1);
+        rerun_list[k] = g;
+    }
+}
+
+/* Highest column each group stores: ncol (if wrote) or ncol-1. */
+__global__ void bsw_last_writer_scatter(int32_t ngroups, const GroupShape *shape,
+                                        const uint8_t *wrote, int32_t *last)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    const int top = wrote[g] ? shape[g].ncol : shape[g].ncol - 1;
+    if (top >= 0) atomicMax(last + top, g);
+}
+
+/* One block: suffix max of `last` over columns, then fold each column's last
+ * writer into the handle's image of the F scratch zero-masks. */
+__global__ __launch_bounds__(1024) void bsw_image_update(int32_t ncols, const GroupShape *shape,
+                                                         const int32_t *last, const uint32_t *fmask,
+                                                         const int64_t *fmask_off, uint32_t *image)
+{
+    __shared__ int32_t carry;
+    if (threadIdx.x == 0) carry = -1;
+    __syncthreads();
+    /* process columns from the top down in chunks of blockDim */
+    for (int top = ncols - 1; top >= 0; top -= blockDim.x) {
+        const int c = top - (int)threadIdx.x;
+        int v = c >= 0 ? last[c] : -1;
+        /* inclusive suffix max within the chunk via shared memory scan */
+        __shared__ int32_t buf[1024];
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+            int t = threadIdx.x >= (unsigned)o ? buf[threadIdx.x - o] : -1;
+            __syncthreads();
+            buf[threadIdx.x] = max(buf[threadIdx.x], t);
+            __syncthreads();
+        }
+        const int lw = max(buf[threadIdx.x], carry);
+        if (c >= 0 && lw >= 0) {
+            const int nh = shape[lw].ncol;
+            image[c] = nh > c ? fmask[fmask_off[lw] + c] : 0u;
+        }
+        __syncthreads();
+        if (threadIdx.x == blockDim.x - 1) carry = max(carry, buf[threadIdx.x]);
+        __syncthreads();
+    }
+}
+
+/* 8/16-bit split of the dispatcher: flag[k] = 1 for 8-bit pairs and
+ * flag[n+k] = 1 for 16-bit pairs. */
+__global__ void bsw_flag8(const tal_seqpair *pairs, int32_t n, int32_t match_pos, uint8_t *flag)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const tal_seqpair sp = pairs[k];
+    const int mn = sp.len1 < sp.len2 ? sp.len1 : sp.len2;
+    const long long bound = (long long)sp.h0 + (long long)match_pos * mn;
+    const uint8_t f = (sp.len1 >= 0 && sp.len2 >= 0 && sp.len1 <= MAX_LEN8 &&
+                       sp.len2 <= MAX_LEN8 && sp.h0 >= 0 && bound <= 127)
+                          ? 1
+                          : 0;
+    flag[k] = f;         /* 8-bit path */
+    flag[n + k] = 1 - f; /* 16-bit path */
+}
+
+/* Gather pairs by index list, and scatter outputs back. */
+__global__ void bsw_gather(const tal_seqpair *src, const int32_t *idx, int32_t m, tal_seqpair *dst)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m) dst[k] = src[idx[k]];
+}
+
+__global__ void bsw_scatter_out(const tal_seqpair *src, const int32_t *idx, int32_t m,
+                                tal_seqpair *dst)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= m) return;
+    const tal_seqpair s = src[k];
+    tal_seqpair *d = dst + idx[k];
+    d->score = s.score;
+    d->tle = s.tle;
+    d->gtle = s.gtle;
+    d->qle = s.qle;
+    d->gscore = s.gscore;
+    d->max_off = s.max_off;
+}
+
+/* ---------------------------------------------------------------- launchers */
+
+hipError_t launch_main(int bits, bool lds, int grid, size_t lds_bytes, const KParams &K,
+                       const KBuffers &B, hipStream_t s)
+{
+    if (bits == 16) {
+        if (lds)
+            hipLaunchKernelGGL((bsw_main<16, true>), dim3(grid), dim3(64), lds_bytes, s, K, B);
+        else
+            hipLaunchKernelGGL((bsw_main<16, false>), dim3(grid), dim3(64), 0, s, K, B);
+    } else {
+        if (lds)
+            hipLaunchKernelGGL((bsw_main<8, true>), dim3(grid), dim3(64), lds_bytes, s, K, B);
+        else
+            hipLaunchKernelGGL((bsw_main<8, false>), dim3(grid), dim3(64), 0, s, K, B);
+    }
+    return hipGetLastError();
+}
+
+hipError_t set_main_lds_limit(size_t bytes)
+{
+    hipError_t e = hipFuncSetAttribute((const void *)bsw_main<16, true>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    if (e != hipSuccess) return e;
+    return hipFuncSetAttribute((const void *)bsw_main<8, true>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+hipError_t launch_shapes(int bits, const tal_seqpair *pairs, int32_t n, int32_t ngroups,
+                         int32_t pad_len2, GroupShape *out, int64_t *ncol_plus1, hipStream_t s)
+{
+    const int tpb = 256, grid = (ngroups + tpb - 1) / tpb;
+    if (grid == 0) return hipSuccess;
+    if (bits == 16)
+        hipLaunchKernelGGL(bsw_shapes<32>, dim3(grid), dim3(tpb), 0, s, pairs, n, ngroups, 0, out,
+                           ncol_plus1);
+    else
+        hipLaunchKernelGGL(bsw_shapes<64>, dim3(grid), dim3(tpb), 0, s, pairs, n, ngroups,
+                           pad_len2, out, ncol_plus1);
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve(int32_t ngroups, const GroupShape *shape, const uint32_t *dep,
+                          const uint8_t *wrote, const uint32_t *fmask, const int64_t *fmask_off,
+                          const uint32_t *image, uint32_t *stale, int32_t *rerun_list,
+                          int32_t *rerun_count, hipStream_t s)
+{
+    const int tpb = 256, grid = (ngroups + tpb - 1) / tpb;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(bsw_resolve, dim3(grid), dim3(tpb), 0, s, ngroups, shape, dep, wrote,
+                       fmask, fmask_off, image, stale, rerun_list, rerun_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_image_update(int32_t ngroups, int32_t ncols, const GroupShape *shape,
+                               const uint8_t *wrote, int32_t *last, const uint32_t *fmask,
+                               const int64_t *fmask_off, uint32_t *image, hipStream_t s)
+{
+    const int tpb = 256, grid = (ngroups + tpb - 1) / tpb;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(bsw_last_writer_scatter, dim3(grid), dim3(tpb), 0, s, ngroups, shape,
+                       wrote, last);
+    hipLaunchKernelGGL(bsw_image_update, dim3(1), dim3(1024), 0, s, ncols, shape, last, fmask,
+                       fmask_off, image);
+    return hipGetLastError();
+}
+
+hipError_t launch_flag8(const tal_seqpair *pairs, int32_t n, int32_t match_pos, uint8_t *flag,
+                        hipStream_t s)
+{
+    const int tpb = 256, grid = (n + tpb - 1) / tpb;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(bsw_flag8, dim3(grid), dim3(tpb), 0, s, pairs, n, match_pos, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather(const tal_seqpair *src, const int32_t *idx, int32_t m,
+                         tal_seqpair *dst, hipStream_t s)
+{
+    const int tpb = 256, grid = (m + tpb - 1) / tpb;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(bsw_gather, dim3(grid), dim3(tpb), 0, s, src, idx, m, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_out(const tal_seqpair *src, const int32_t *idx, int32_t m,
+                              tal_seqpair *dst, hipStream_t s)
+{
+    const int tpb = 256, grid = (m + tpb - 1) / tpb;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(bsw_scatter_out, dim3(grid), dim3(tpb), 0, s, src, idx, m, dst);
+    return hipGetLastError();
+}
+
+} // namespace talbsw

@@ -1,0 +1,607 @@
+/*
+ * tal_bsw.hip -- host runtime behind include/tal_bsw.h.
+ *
+ * A handle owns one HIP stream, the device buffers a call needs (grown on
+ * demand, never shrunk), and the device image of the reference object's
+ * persistent 16-bit F scratch (one zero-mask per column).  Every entry point
+ * returns a status code; errors are recorded in the handle and never abort.
+ */
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "bsw_internal.h"
+
+namespace talbsw {
+hipError_t launch_main(int bits, bool lds, int grid, size_t lds_bytes, const KParams &K,
+                       const KBuffers &B, hipStream_t s);
+hipError_t set_main_lds_limit(size_t bytes);
+hipError_t launch_shapes(int bits, const tal_seqpair *pairs, int32_t n, int32_t ngroups,
+                         int32_t pad_len2, GroupShape *out, int64_t *ncol_plus1, hipStream_t s);
+hipError_t launch_resolve(int32_t ngroups, const GroupShape *shape, const uint32_t *dep,
+                          const uint8_t *wrote, const uint32_t *fmask, const int64_t *fmask_off,
+                          const uint32_t *image, uint32_t *stale, int32_t *rerun_list,
+                          int32_t *rerun_count, hipStream_t s);
+hipError_t launch_image_update(int32_t ngroups, int32_t ncols, const GroupShape *shape,
+                               const uint8_t *wrote, int32_t *last, const uint32_t *fmask,
+                               const int64_t *fmask_off, uint32_t *image, hipStream_t s);
+hipError_t launch_flag8(const tal_seqpair *pairs, int32_t n, int32_t match_pos, uint8_t *flag,
+                        hipStream_t s);
+hipError_t launch_gather(const tal_seqpair *src, const int32_t *idx, int32_t m,
+                         tal_seqpair *dst, hipStream_t s);
+hipError_t launch_scatter_out(const tal_seqpair *src, const int32_t *idx, int32_t m,
+                              tal_seqpair *dst, hipStream_t s);
+} // namespace talbsw
+
+using namespace talbsw;
+
+namespace {
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t bytes)
+    {
+        if (bytes <= cap) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            p = nullptr;
+            cap = 0;
+            if (e != hipSuccess) return e;
+        }
+        size_t want = std::max(bytes, cap * 3 / 2);
+        want = (want + 4095) & ~(size_t)4095;
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            p = nullptr;
+            return e;
+        }
+        cap = want;
+        return hipSuccess;
+    }
+    template <class T> T *as() const { return (T *)p; }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+/* Per-width scratch of one call. */
+struct PathBufs {
+    DevBuf shapes, ncp1, foff, dep, wrote, stale, fmask, rerun, tasks, counter, slabs, last,
+        cubtmp, summary;
+};
+
+} // namespace
+
+struct tal_bsw {
+    tal_bsw_params prm{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    size_t lds_per_block = 0;
+    size_t lds_per_cu = 0;
+    DevBuf image; /* IMAGE_COLS u32: F[c] nonzero lanes of the 16-bit scratch */
+    PathBufs b16, b8;
+    /* dispatcher scratch */
+    DevBuf flag, idx_all, idx_sel, nsel, pairs8, pairs16, cubtmp;
+    /* host-API staging */
+    DevBuf d_pairs, d_ref, d_qer;
+    std::string err;
+};
+
+namespace {
+
+int fail(tal_bsw *h, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (h) h->err = buf;
+    return code;
+}
+
+int hip_fail(tal_bsw *h, hipError_t e, const char *what)
+{
+    const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? TAL_ENOMEM
+                                                                                 : TAL_EDEVICE;
+    return fail(h, code, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIPCHK(h, expr, what)                                                                      \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess) return hip_fail((h), e_, (what));                                    \
+    } while (0)
+
+int smax_of(const tal_bsw_params &p)
+{
+    int s = 0;
+    const int mm = -(int)(int8_t)p.w_mismatch; /* stored negated, :65 */
+    s = std::max(s, (int)(int8_t)p.w_match);
+    s = std::max(s, (int)(int8_t)mm);
+    s = std::max(s, -1);
+    return s;
+}
+
+KParams base_params(const tal_bsw *h, int bits, int32_t n, int32_t w)
+{
+    KParams K{};
+    const tal_bsw_params &p = h->prm;
+    const int sh = bits == 16 ? 0 : 8;
+    auto sc = [&](int v) -> int16_t {
+        return (int16_t)(uint16_t)((bits == 16 ? (uint32_t)(uint16_t)v : (uint32_t)(uint8_t)v) << sh);
+    };
+    K.n = n;
+    K.w = w;
+    K.o_del = p.o_del;
+    K.e_del = p.e_del;
+    K.o_ins = p.o_ins;
+    K.e_ins = p.e_ins;
+    K.end_bonus = p.end_bonus;
+    K.smax = smax_of(p);
+    K.match = sc((int8_t)p.w_match);
+    K.mism = sc((int8_t)(-(int)(int8_t)p.w_mismatch));
+    K.ambig = sc(-1);
+    K.e_del_s = sc(p.e_del);
+    K.oe_del_s = sc(p.o_del + p.e_del);
+    K.e_ins_s = sc(p.e_ins);
+    K.oe_ins_s = sc(p.o_ins + p.e_ins);
+    K.zdrop_s = sc(p.zdrop);
+    return K;
+}
+
+/* Summary of the group shapes, computed on the device and read back. */
+struct ShapeSummary {
+    int32_t max_ncol;
+    int32_t max_nrow;
+    int64_t total_cols; /* sum over groups of ncol + 1 (16-bit only) */
+};
+
+__global__ void summarize(const GroupShape *shapes, const int64_t *foff_incl, int32_t ngroups,
+                          ShapeSummary *out)
+{
+    __shared__ int32_t mc[256], mr[256];
+    int32_t c = 0, r = 0;
+    for (int g = threadIdx.x; g < ngroups; g += blockDim.x) {
+        c = max(c, shapes[g].ncol);
+        r = max(r, shapes[g].nrow);
+    }
+    mc[threadIdx.x] = c;
+    mr[threadIdx.x] = r;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            mc[threadIdx.x] = max(mc[threadIdx.x], mc[threadIdx.x + o]);
+            mr[threadIdx.x] = max(mr[threadIdx.x], mr[threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out->max_ncol = mc[0];
+        out->max_nrow = mr[0];
+        out->total_cols = (foff_incl && ngroups > 0) ? foff_incl[ngroups - 1] : 0;
+    }
+}
+
+/* Inclusive prefix sums -> exclusive offsets in place (shift by one). */
+__global__ void to_exclusive(const int64_t *incl, int64_t *excl, int32_t n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) excl[k] = k == 0 ? 0 : incl[k - 1];
+}
+
+__global__ void iota32(int32_t *x, int32_t n)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) x[k] = k;
+}
+
+/* Pairs up a rerun list into 16-bit wave tasks (two groups per task). */
+__global__ void make_tasks(const int32_t *list, const int32_t *count, int32_t *tasks,
+                           int32_t cap_tasks, int gpw)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cap_tasks) return;
+    const int c = *count;
+    for (int k = 0; k < gpw; k++) {
+        const int i = t * gpw + k;
+        tasks[i] = i < c ? list[i] : -1;
+    }
+}
+
+int grid_for(const tal_bsw *h, bool lds, size_t lds_bytes, int ntasks)
+{
+    int per_cu = 8; /* waves per CU with state in global memory */
+    if (lds) {
+        per_cu = (int)(h->lds_per_cu / std::max<size_t>(lds_bytes, 1));
+        per_cu = std::max(1, std::min(per_cu, 16));
+    }
+    long long g = (long long)h->cu_count * per_cu;
+    return (int)std::max<long long>(1, std::min<long long>(g, ntasks));
+}
+
+/*
+ * One width-specific batch on device buffers.  16-bit runs the stale-column
+ * resolution loop and updates the handle's image afterwards.
+ */
+int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const uint8_t *qer,
+             int32_t n, int32_t w, int32_t last_len2, hipStream_t s)
+{
+    if (n == 0) return TAL_OK;
+    PathBufs &pb = bits == 16 ? h->b16 : h->b8;
+    const int GL = bits == 16 ? 32 : 64, GPW = 64 / GL;
+    const int32_t G = (n + GL - 1) / GL;
+    KParams K = base_params(h, bits, n, w);
+    K.ngroups = G;
+    K.last_len2 = last_len2;
+
+    HIPCHK(h, pb.shapes.ensure(sizeof(GroupShape) * G), "alloc shapes");
+    HIPCHK(h, pb.summary.ensure(sizeof(ShapeSummary)), "alloc summary");
+    GroupShape *shapes = pb.shapes.as<GroupShape>();
+    if (bits == 16) {
+        HIPCHK(h, pb.ncp1.ensure(sizeof(int64_t) * G), "alloc ncol");
+        HIPCHK(h, pb.foff.ensure(sizeof(int64_t) * G), "alloc offsets");
+        HIPCHK(h, launch_shapes(16, pairs, n, G, 0, shapes, pb.ncp1.as<int64_t>(), s), "shapes");
+        size_t tmp = 0;
+        HIPCHK(h, hipcub::DeviceScan::InclusiveSum(nullptr, tmp, pb.ncp1.as<int64_t>(),
+                                                   pb.ncp1.as<int64_t>(), G, s),
+               "scan size");
+        HIPCHK(h, pb.cubtmp.ensure(tmp + 16), "alloc scan tmp");
+        HIPCHK(h, hipcub::DeviceScan::InclusiveSum(pb.cubtmp.p, tmp, pb.ncp1.as<int64_t>(),
+                                                   pb.ncp1.as<int64_t>(), G, s),
+               "scan");
+        hipLaunchKernelGGL(summarize, dim3(1), dim3(256), 0, s, shapes, pb.ncp1.as<int64_t>(), G,
+                           pb.summary.as<ShapeSummary>());
+        hipLaunchKernelGGL(to_exclusive, dim3((G + 255) / 256), dim3(256), 0, s,
+                           pb.ncp1.as<int64_t>(), pb.foff.as<int64_t>(), G);
+    } else {
+        HIPCHK(h, launch_shapes(8, pairs, n, G, last_len2, shapes, nullptr, s), "shapes");
+        hipLaunchKernelGGL(summarize, dim3(1), dim3(256), 0, s, shapes, (const int64_t *)nullptr,
+                           G, pb.summary.as<ShapeSummary>());
+    }
+    HIPCHK(h, hipGetLastError(), "summary");
+    ShapeSummary sum{};
+    HIPCHK(h, hipMemcpyAsync(&sum, pb.summary.p, sizeof sum, hipMemcpyDeviceToHost, s),
+           "read summary");
+    HIPCHK(h, hipStreamSynchronize(s), "sync summary");
+    const int limit = bits == 16 ? MAX_LEN16 : MAX_LEN8;
+    if (sum.max_ncol > limit || sum.max_nrow > limit || last_len2 < 0 || last_len2 > limit)
+        return fail(h, TAL_ERANGE, "a pair length is negative or exceeds %d", limit);
+    const int32_t max_ncol = bits == 16 ? sum.max_ncol : MAX_LEN8;
+    const int64_t total_cols = sum.total_cols;
+
+    /* state placement: LDS if the widest group fits, else global slabs */
+    const int state_cols = max_ncol + 2;
+    const size_t lds_bytes = (size_t)state_cols * 64 * sizeof(uint32_t);
+    const bool lds = lds_bytes <= h->lds_per_block;
+    const int ntasks = (G + GPW - 1) / GPW;
+    int grid = grid_for(h, lds, lds_bytes, ntasks);
+    K.ntasks = ntasks;
+    K.state_cols = lds ? 0 : state_cols;
+
+    KBuffers B{};
+    B.pairs = pairs;
+    B.ref = ref;
+    B.qer = qer;
+    HIPCHK(h, pb.counter.ensure(sizeof(int32_t) * 4), "alloc counter");
+    B.task_counter = pb.counter.as<int32_t>();
+    if (!lds) {
+        HIPCHK(h, pb.slabs.ensure((size_t)grid * state_cols * 64 * sizeof(uint32_t)),
+               "alloc state slabs");
+        B.slabs = pb.slabs.as<uint32_t>();
+    }
+    if (bits == 16) {
+        HIPCHK(h, pb.dep.ensure(sizeof(uint32_t) * G), "alloc dep");
+        HIPCHK(h, pb.wrote.ensure(G), "alloc wrote");
+        HIPCHK(h, pb.stale.ensure(sizeof(uint32_t) * G), "alloc stale");
+        HIPCHK(h, pb.fmask.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(total_cols, 1)),
+               "alloc fmask");
+        HIPCHK(h, pb.rerun.ensure(sizeof(int32_t) * (G + 1)), "alloc rerun");
+        HIPCHK(h, pb.tasks.ensure(sizeof(int32_t) * (size_t)(G + 2) * 2), "alloc tasks");
+        HIPCHK(h, hipMemsetAsync(pb.stale.p, 0, sizeof(uint32_t) * G, s), "clear stale");
+        B.stale_in = pb.stale.as<uint32_t>();
+        B.dep_out = pb.dep.as<uint32_t>();
+        B.wrote_out = pb.wrote.as<uint8_t>();
+        B.fmask = pb.fmask.as<uint32_t>();
+        B.fmask_off = pb.foff.as<int64_t>();
+    }
+    HIPCHK(h, hipMemsetAsync(B.task_counter, 0, sizeof(int32_t), s), "clear counter");
+    HIPCHK(h, launch_main(bits, lds, grid, lds ? lds_bytes : 0, K, B, s), "main kernel");
+
+    if (bits == 16) {
+        int32_t *cnt = pb.counter.as<int32_t>() + 1;
+        int32_t *list = pb.rerun.as<int32_t>();
+        for (int iter = 0;; iter++) {
+            HIPCHK(h, hipMemsetAsync(cnt, 0, sizeof(int32_t), s), "clear rerun count");
+            HIPCHK(h, launch_resolve(G, shapes, B.dep_out, B.wrote_out, B.fmask, B.fmask_off,
+                                     h->image.as<uint32_t>(), pb.stale.as<uint32_t>(), list, cnt,
+                                     s),
+                   "resolve");
+            int32_t c = 0;
+            HIPCHK(h, hipMemcpyAsync(&c, cnt, sizeof c, hipMemcpyDeviceToHost, s), "read reruns");
+            HIPCHK(h, hipStreamSynchronize(s), "sync reruns");
+            if (c == 0) break;
+            if (iter > G + 1) return fail(h, TAL_EDEVICE, "stale-column resolution did not converge");
+            /* rerun the listed groups with their corrected F[ncol] masks */
+            const int rt = (c + GPW - 1) / GPW;
+            hipLaunchKernelGGL(make_tasks, dim3((rt + 255) / 256), dim3(256), 0, s, list, cnt,
+                               pb.tasks.as<int32_t>(), rt, GPW);
+            HIPCHK(h, hipGetLastError(), "make tasks");
+            KParams K2 = K;
+            K2.ntasks = rt;
+            KBuffers B2 = B;
+            B2.task_groups = pb.tasks.as<int32_t>();
+            HIPCHK(h, hipMemsetAsync(B.task_counter, 0, sizeof(int32_t), s), "clear counter");
+            const int g2 = grid_for(h, lds, lds_bytes, rt);
+            HIPCHK(h, launch_main(16, lds, std::min(g2, grid), lds ? lds_bytes : 0, K2, B2, s),
+                   "rerun kernel");
+        }
+        HIPCHK(h, pb.last.ensure(sizeof(int32_t) * IMAGE_COLS), "alloc last");
+        HIPCHK(h, hipMemsetAsync(pb.last.p, 0xFF, sizeof(int32_t) * IMAGE_COLS, s), "clear last");
+        HIPCHK(h, launch_image_update(G, max_ncol + 1, shapes, B.wrote_out, pb.last.as<int32_t>(),
+                                      B.fmask, B.fmask_off, h->image.as<uint32_t>(), s),
+               "image update");
+    }
+    return TAL_OK;
+}
+
+int check_common(tal_bsw *h, tal_seqpair *pairs, int32_t n, int32_t w)
+{
+    if (!h) return TAL_EINVAL;
+    if (n < 0) return fail(h, TAL_EINVAL, "n < 0");
+    if (w < 0) return fail(h, TAL_EINVAL, "w < 0");
+    if (n > 0 && !pairs) return fail(h, TAL_EINVAL, "null pairs");
+    return TAL_OK;
+}
+
+/* Host-side validation of the pairs of a host-buffer call. */
+int check_pairs(tal_bsw *h, const tal_seqpair *pairs, int32_t n, size_t ref_len, size_t qer_len,
+                int maxlen)
+{
+    for (int32_t k = 0; k < n; k++) {
+        const tal_seqpair &p = pairs[k];
+        if (p.len1 < 0 || p.len2 < 0 || p.idr < 0 || p.idq < 0)
+            return fail(h, TAL_EINVAL, "pair %d: negative length or offset", k);
+        if (p.len1 > maxlen || p.len2 > maxlen)
+            return fail(h, TAL_ERANGE, "pair %d: length %d/%d exceeds %d", k, p.len1, p.len2,
+                        maxlen);
+        if ((size_t)p.idr + (size_t)p.len1 > ref_len || (size_t)p.idq + (size_t)p.len2 > qer_len)
+            return fail(h, TAL_EINVAL, "pair %d: sequence outside its buffer", k);
+    }
+    return TAL_OK;
+}
+
+int host_call(tal_bsw *h, int mode, tal_seqpair *pairs, const uint8_t *ref, size_t ref_len,
+              const uint8_t *qer, size_t qer_len, int32_t n, int32_t w)
+{
+    int rc = check_common(h, pairs, n, w);
+    if (rc) return rc;
+    if (n > 0 && ((!ref && ref_len) || (!qer && qer_len)))
+        return fail(h, TAL_EINVAL, "null sequence buffer");
+    rc = check_pairs(h, pairs, n, ref_len, qer_len, mode == 8 ? MAX_LEN8 : MAX_LEN16);
+    if (rc) return rc;
+    if (n == 0) return TAL_OK;
+    HIPCHK(h, hipSetDevice(h->device), "set device");
+    hipStream_t s = h->stream;
+    HIPCHK(h, h->d_pairs.ensure(sizeof(tal_seqpair) * n), "alloc pairs");
+    HIPCHK(h, h->d_ref.ensure(std::max<size_t>(ref_len, 1)), "alloc ref");
+    HIPCHK(h, h->d_qer.ensure(std::max<size_t>(qer_len, 1)), "alloc qer");
+    HIPCHK(h, hipMemcpyAsync(h->d_pairs.p, pairs, sizeof(tal_seqpair) * n, hipMemcpyHostToDevice, s),
+           "copy pairs");
+    if (ref_len)
+        HIPCHK(h, hipMemcpyAsync(h->d_ref.p, ref, ref_len, hipMemcpyHostToDevice, s), "copy ref");
+    if (qer_len)
+        HIPCHK(h, hipMemcpyAsync(h->d_qer.p, qer, qer_len, hipMemcpyHostToDevice, s), "copy qer");
+    tal_seqpair *dp = h->d_pairs.as<tal_seqpair>();
+    if (mode == 16)
+        rc = tal_bsw_get_scores16_device(h, dp, h->d_ref.as<uint8_t>(), h->d_qer.as<uint8_t>(), n,
+                                         w, s);
+    else if (mode == 8)
+        rc = tal_bsw_get_scores8_device(h, dp, h->d_ref.as<uint8_t>(), h->d_qer.as<uint8_t>(), n,
+                                        w, s);
+    else
+        rc = tal_bsw_get_scores_auto_device(h, dp, h->d_ref.as<uint8_t>(),
+                                            h->d_qer.as<uint8_t>(), n, w, s);
+    if (rc) return rc;
+    HIPCHK(h, hipMemcpyAsync(pairs, dp, sizeof(tal_seqpair) * n, hipMemcpyDeviceToHost, s),
+           "copy results");
+    HIPCHK(h, hipStreamSynchronize(s), "sync results");
+    return TAL_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int tal_bsw_create(const tal_bsw_params *params, int device, tal_bsw **out)
+{
+    if (!params || !out) return TAL_EINVAL;
+    *out = nullptr;
+    if (params->e_del == 0 || params->e_ins == 0) return TAL_EINVAL;
+    if (params->w_match < -128 || params->w_match > 127 || params->w_mismatch < -128 ||
+        params->w_mismatch > 127)
+        return TAL_EINVAL;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return TAL_EDEVICE;
+    if (device < 0 || device >= ndev) return TAL_EINVAL;
+    tal_bsw *h = new (std::nothrow) tal_bsw();
+    if (!h) return TAL_ENOMEM;
+    h->prm = *params;
+    h->device = device;
+    int rc = TAL_OK;
+    do {
+        if (hipSetDevice(device) != hipSuccess) { rc = TAL_EDEVICE; break; }
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, device) != hipSuccess) { rc = TAL_EDEVICE; break; }
+        h->cu_count = prop.multiProcessorCount;
+        h->lds_per_block = prop.sharedMemPerBlock;
+        h->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
+                                                              : prop.sharedMemPerBlock;
+        if (set_main_lds_limit(h->lds_per_block) != hipSuccess) { rc = TAL_EDEVICE; break; }
+        if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { rc = TAL_EDEVICE; break; }
+        hipError_t e = h->image.ensure(sizeof(uint32_t) * IMAGE_COLS);
+        if (e != hipSuccess) { rc = TAL_ENOMEM; break; }
+        if (hipMemset(h->image.p, 0, sizeof(uint32_t) * IMAGE_COLS) != hipSuccess) { rc = TAL_EDEVICE; break; }
+    } while (0);
+    if (rc) {
+        tal_bsw_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return TAL_OK;
+}
+
+void tal_bsw_destroy(tal_bsw *h)
+{
+    if (!h) return;
+    (void)hipSetDevice(h->device);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    DevBuf *all[] = {&h->image,   &h->flag,    &h->idx_all, &h->idx_sel, &h->nsel,
+                     &h->pairs8,  &h->pairs16, &h->cubtmp,  &h->d_pairs, &h->d_ref,
+                     &h->d_qer};
+    for (DevBuf *b : all) b->release();
+    for (PathBufs *pb : {&h->b16, &h->b8}) {
+        DevBuf *bs[] = {&pb->shapes, &pb->ncp1, &pb->foff,    &pb->dep,   &pb->wrote,
+                        &pb->stale,  &pb->fmask, &pb->rerun,  &pb->tasks, &pb->counter,
+                        &pb->slabs,  &pb->last,  &pb->cubtmp, &pb->summary};
+        for (DevBuf *b : bs) b->release();
+    }
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+const char *tal_bsw_last_error(const tal_bsw *h) { return h ? h->err.c_str() : "null handle"; }
+
+int tal_bsw_get_scores16(tal_bsw *h, tal_seqpair *pairs, const uint8_t *ref, size_t ref_len,
+                         const uint8_t *qer, size_t qer_len, int32_t n, int32_t w)
+{
+    return host_call(h, 16, pairs, ref, ref_len, qer, qer_len, n, w);
+}
+
+int tal_bsw_get_scores8(tal_bsw *h, tal_seqpair *pairs, const uint8_t *ref, size_t ref_len,
+                        const uint8_t *qer, size_t qer_len, int32_t n, int32_t w)
+{
+    return host_call(h, 8, pairs, ref, ref_len, qer, qer_len, n, w);
+}
+
+int tal_bsw_get_scores_auto(tal_bsw *h, tal_seqpair *pairs, const uint8_t *ref, size_t ref_len,
+                            const uint8_t *qer, size_t qer_len, int32_t n, int32_t w)
+{
+    return host_call(h, 0, pairs, ref, ref_len, qer, qer_len, n, w);
+}
+
+int tal_bsw_get_scores16_device(tal_bsw *h, tal_seqpair *pairs, const uint8_t *ref,
+                                const uint8_t *qer, int32_t n, int32_t w, void *stream)
+{
+    int rc = check_common(h, pairs, n, w);
+    if (rc) return rc;
+    HIPCHK(h, hipSetDevice(h->device), "set device");
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    return run_path(h, 16, pairs, ref, qer, n, w, 0, s);
+}
+
+int tal_bsw_get_scores8_device(tal_bsw *h, tal_seqpair *pairs, const uint8_t *ref,
+                               const uint8_t *qer, int32_t n, int32_t w, void *stream)
+{
+    int rc = check_common(h, pairs, n, w);
+    if (rc) return rc;
+    if (n == 0) return TAL_OK;
+    HIPCHK(h, hipSetDevice(h->device), "set device");
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    int32_t last_len2 = 0;
+    HIPCHK(h, hipMemcpyAsync(&last_len2, &pairs[n - 1].len2, sizeof last_len2,
+                             hipMemcpyDeviceToHost, s),
+           "read last len2");
+    HIPCHK(h, hipStreamSynchronize(s), "sync last len2");
+    return run_path(h, 8, pairs, ref, qer, n, w, last_len2, s);
+}
+
+int tal_bsw_get_scores_auto_device(tal_bsw *h, tal_seqpair *pairs, const uint8_t *ref,
+                                   const uint8_t *qer, int32_t n, int32_t w, void *stream)
+{
+    int rc = check_common(h, pairs, n, w);
+    if (rc) return rc;
+    if (n == 0) return TAL_OK;
+    HIPCHK(h, hipSetDevice(h->device), "set device");
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    HIPCHK(h, h->flag.ensure(2 * (size_t)n), "alloc flags");
+    HIPCHK(h, h->idx_all.ensure(sizeof(int32_t) * n), "alloc index");
+    HIPCHK(h, h->idx_sel.ensure(sizeof(int32_t) * n), "alloc index");
+    HIPCHK(h, h->nsel.ensure(sizeof(int32_t) * 2), "alloc count");
+    HIPCHK(h, launch_flag8(pairs, n, smax_of(h->prm), h->flag.as<uint8_t>(), s), "flag8");
+    hipLaunchKernelGGL(iota32, dim3((n + 255) / 256), dim3(256), 0, s, h->idx_all.as<int32_t>(), n);
+    HIPCHK(h, hipGetLastError(), "iota");
+    /* two stable selections: 8-bit pair indices, then 16-bit ones, each in
+     * input order */
+    int32_t *sel = h->idx_sel.as<int32_t>();
+    int32_t *cnt = h->nsel.as<int32_t>();
+    size_t tmp = 0;
+    HIPCHK(h, hipcub::DeviceSelect::Flagged(nullptr, tmp, h->idx_all.as<int32_t>(),
+                                            h->flag.as<uint8_t>(), sel, cnt, n, s),
+           "select size");
+    HIPCHK(h, h->cubtmp.ensure(tmp + 16), "alloc select tmp");
+    HIPCHK(h, hipcub::DeviceSelect::Flagged(h->cubtmp.p, tmp, h->idx_all.as<int32_t>(),
+                                            h->flag.as<uint8_t>(), sel, cnt, n, s),
+           "select 8-bit");
+    int32_t n8 = 0;
+    HIPCHK(h, hipMemcpyAsync(&n8, cnt, sizeof n8, hipMemcpyDeviceToHost, s), "read n8");
+    HIPCHK(h, hipStreamSynchronize(s), "sync n8");
+    const int32_t n16 = n - n8;
+    if (n16) {
+        size_t tmp2 = 0;
+        HIPCHK(h, hipcub::DeviceSelect::Flagged(nullptr, tmp2, h->idx_all.as<int32_t>(),
+                                                h->flag.as<uint8_t>() + n, sel + n8, cnt + 1, n,
+                                                s),
+               "select size");
+        HIPCHK(h, h->cubtmp.ensure(tmp2 + 16), "alloc select tmp");
+        HIPCHK(h, hipcub::DeviceSelect::Flagged(h->cubtmp.p, tmp2, h->idx_all.as<int32_t>(),
+                                                h->flag.as<uint8_t>() + n, sel + n8, cnt + 1, n,
+                                                s),
+               "select 16-bit");
+    }
+    HIPCHK(h, h->pairs8.ensure(sizeof(tal_seqpair) * std::max(n8, 1)), "alloc pairs8");
+    HIPCHK(h, h->pairs16.ensure(sizeof(tal_seqpair) * std::max(n16, 1)), "alloc pairs16");
+    if (n8) {
+        HIPCHK(h, launch_gather(pairs, sel, n8, h->pairs8.as<tal_seqpair>(), s), "gather8");
+        rc = tal_bsw_get_scores8_device(h, h->pairs8.as<tal_seqpair>(), ref, qer, n8, w, s);
+        if (rc) return rc;
+        HIPCHK(h, launch_scatter_out(h->pairs8.as<tal_seqpair>(), sel, n8, pairs, s), "scatter8");
+    }
+    if (n16) {
+        HIPCHK(h, launch_gather(pairs, sel + n8, n16, h->pairs16.as<tal_seqpair>(), s),
+               "gather16");
+        rc = tal_bsw_get_scores16_device(h, h->pairs16.as<tal_seqpair>(), ref, qer, n16, w, s);
+        if (rc) return rc;
+        HIPCHK(h, launch_scatter_out(h->pairs16.as<tal_seqpair>(), sel + n8, n16, pairs, s),
+               "scatter16");
+    }
+    return TAL_OK;
+}
+
+int32_t tal_bsw_count_8bit(const tal_bsw *h, const tal_seqpair *pairs, int32_t n)
+{
+    if (!h || (!pairs && n > 0)) return -1;
+    const int sm = smax_of(h->prm);
+    int32_t c = 0;
+    for (int32_t k = 0; k < n; k++) {
+        const tal_seqpair &p = pairs[k];
+        const int mn = std::min(p.len1, p.len2);
+        if (p.len1 >= 0 && p.len2 >= 0 && p.len1 <= MAX_LEN8 && p.len2 <= MAX_LEN8 && p.h0 >= 0 &&
+            (long long)p.h0 + (long long)sm * mn <= 127)
+            c++;
+    }
+    return c;
+}
+
+} /* extern "C" */

@@ -1,0 +1,128 @@
+"""Python binding of libtal_bsw.so (the C ABI in include/tal_bsw.h).
+
+Mirrors the reference's BandedPairWiseSW interface
+(src/alignment/BSW/bandedSWA.h:113-336): construct with the scoring
+parameters, then call get_scores16 / get_scores8 on a SeqPair array and the
+packed reference/query byte buffers; outputs are written in place.  The GPU
+library must be built (build.py); there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtal_bsw.so")
+
+SEQPAIR_FIELDS = ["idr", "idq", "id", "len1", "len2", "h0", "seqid", "regid",
+                  "score", "tle", "gtle", "qle", "gscore", "max_off"]
+SEQPAIR_DTYPE = np.dtype([(f, "<i4") for f in SEQPAIR_FIELDS])
+OUT_FIELDS = ["score", "tle", "gtle", "qle", "gscore", "max_off"]
+
+TAL_OK, TAL_EINVAL, TAL_ENOMEM, TAL_EDEVICE, TAL_ERANGE = 0, 1, 2, 3, 4
+_STATUS = {1: "TAL_EINVAL", 2: "TAL_ENOMEM", 3: "TAL_EDEVICE", 4: "TAL_ERANGE"}
+
+# every C-ABI entry point of include/tal_bsw.h
+EXPORTS = ["tal_bsw_create", "tal_bsw_destroy", "tal_bsw_last_error",
+           "tal_bsw_get_scores16", "tal_bsw_get_scores8", "tal_bsw_get_scores_auto",
+           "tal_bsw_get_scores16_device", "tal_bsw_get_scores8_device",
+           "tal_bsw_get_scores_auto_device", "tal_bsw_count_8bit"]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in
+                ("o_del", "e_del", "o_ins", "e_ins", "zdrop", "end_bonus", "w_match", "w_mismatch")]
+
+
+class TalError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def load():
+    """Loads libtal_bsw.so; raises if it is not built (no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        P, i32, sz = ctypes.c_void_p, ctypes.c_int32, ctypes.c_size_t
+        L.tal_bsw_create.restype = ctypes.c_int
+        L.tal_bsw_create.argtypes = [ctypes.POINTER(Params), ctypes.c_int, ctypes.POINTER(P)]
+        L.tal_bsw_destroy.argtypes = [P]
+        L.tal_bsw_last_error.restype = ctypes.c_char_p
+        L.tal_bsw_last_error.argtypes = [P]
+        for name in ("tal_bsw_get_scores16", "tal_bsw_get_scores8", "tal_bsw_get_scores_auto"):
+            f = getattr(L, name)
+            f.restype = ctypes.c_int
+            f.argtypes = [P, P, P, sz, P, sz, i32, i32]
+        for name in ("tal_bsw_get_scores16_device", "tal_bsw_get_scores8_device",
+                     "tal_bsw_get_scores_auto_device"):
+            f = getattr(L, name)
+            f.restype = ctypes.c_int
+            f.argtypes = [P, P, P, P, i32, i32, P]
+        L.tal_bsw_count_8bit.restype = i32
+        L.tal_bsw_count_8bit.argtypes = [P, P, i32]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class BandedPairWiseSW:
+    """One engine handle on one GPU (the reference object's counterpart)."""
+
+    def __init__(self, o_del=6, e_del=1, o_ins=6, e_ins=1, zdrop=100, end_bonus=5,
+                 w_match=1, w_mismatch=4, device=0):
+        L = load()
+        self.params = Params(o_del, e_del, o_ins, e_ins, zdrop, end_bonus, w_match, w_mismatch)
+        h = ctypes.c_void_p()
+        rc = L.tal_bsw_create(ctypes.byref(self.params), int(device), ctypes.byref(h))
+        if rc != TAL_OK:
+            raise TalError(rc, "tal_bsw_create failed")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().tal_bsw_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, rc):
+        if rc != TAL_OK:
+            raise TalError(rc, load().tal_bsw_last_error(self._h).decode())
+
+    def _host(self, fn, pairs, ref, qer, w):
+        assert pairs.dtype == SEQPAIR_DTYPE and pairs.flags.c_contiguous
+        ref = np.ascontiguousarray(ref, dtype=np.uint8)
+        qer = np.ascontiguousarray(qer, dtype=np.uint8)
+        self._check(fn(self._h, _ptr(pairs), _ptr(ref), ref.nbytes, _ptr(qer), qer.nbytes,
+                       len(pairs), int(w)))
+        return pairs
+
+    def get_scores16(self, pairs, ref, qer, w):
+        return self._host(load().tal_bsw_get_scores16, pairs, ref, qer, w)
+
+    def get_scores8(self, pairs, ref, qer, w):
+        return self._host(load().tal_bsw_get_scores8, pairs, ref, qer, w)
+
+    def get_scores_auto(self, pairs, ref, qer, w):
+        return self._host(load().tal_bsw_get_scores_auto, pairs, ref, qer, w)
+
+    def get_scores_device(self, bits, d_pairs, d_ref, d_qer, n, w, stream=0):
+        """Device pointers (ints); bits in {16, 8, 0 (auto)}."""
+        fn = {16: load().tal_bsw_get_scores16_device, 8: load().tal_bsw_get_scores8_device,
+              0: load().tal_bsw_get_scores_auto_device}[bits]
+        self._check(fn(self._h, ctypes.c_void_p(d_pairs), ctypes.c_void_p(d_ref),
+                       ctypes.c_void_p(d_qer), int(n), int(w), ctypes.c_void_p(stream or None)))
+
+    def count_8bit(self, pairs):
+        return int(load().tal_bsw_count_8bit(self._h, _ptr(pairs), len(pairs)))
