@@ -1,0 +1,203 @@
+#!/usr/bin/env python3
+"""Benchmark of the banded Smith-Waterman extension engine on one MI355X.
+
+Workload: 2,000,000 extension pairs, query length U[30,250], reference =
+query + U[0,100] flank, 2% mismatches and 0.5% indels, h0 U[0,120], w = 100,
+scoring of bench-bsw (match 1, mismatch 4, gap open 6 / extend 1 both ways,
+z-drop 100, end bonus 5).  A step is one pass of the 8/16-bit dispatcher over
+the whole batch with the pairs and sequences already resident in HBM
+(tal_bsw_get_scores_auto_device); outputs are written in place.
+
+value = band cells / second (GCUPS) of the whole step, where a pair's band
+cells are sum_i max(0, min(len2, i+b+1) - max(0, i-b)) with b its band
+(bandedSWA.cpp:2882-2904).  Also reported: pairs/s, the plain form (per
+20,000-pair host calls of getScores16, as bench-bsw drives the reference), the
+roofline of the dominant kernel against integer VALU peak, and the oracle's
+CPU rate on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "trans-omics-acceleration-library_amd")
+sys.path.insert(0, PKG)
+
+import synth  # noqa: E402
+import tal_bsw  # noqa: E402
+
+# MI355X: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz = 39.3e12 int32 lane-ops/s.
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
+OPS_PER_CELL = 14  # S select 2, ambiguity 1, M 2, H 1, E 3, F 3, row max 2
+PARAMS = dict(o_del=6, e_del=1, o_ins=6, e_ins=1, zdrop=100, end_bonus=5, w_match=1, w_mismatch=4)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--pairs", type=int, default=2_000_000)
+    ap.add_argument("--plain-pairs", type=int, default=200_000)
+    ap.add_argument("--cpu-pairs", type=int, default=60_000)
+    ap.add_argument("--check-pairs", type=int, default=20_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU: the engine has no multi-GPU path in this build "
+              "(pairs are independent; see DESIGN.md for how it would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    w = 100
+    t = time.time()
+    pairs, ref, qer, _ = synth.generate(2, args.pairs)
+    cells = tal_bsw.band_cells(pairs["len1"], pairs["len2"], w, **PARAMS)
+    is8 = tal_bsw.is_8bit(pairs, **PARAMS)
+    log(f"[bench] generated {args.pairs} pairs in {time.time() - t:.1f}s; "
+        f"band cells {cells.sum():.3e}; 8-bit share {is8.mean():.3f}")
+
+    dev = torch.device("cuda:0")
+    d_pairs = torch.from_numpy(pairs.view(np.uint8)).to(dev)
+    d_ref = torch.from_numpy(ref).to(dev)
+    d_qer = torch.from_numpy(qer).to(dev)
+    stream = torch.cuda.current_stream(dev)
+    eng = tal_bsw.BandedPairWiseSW(**PARAMS)
+    n = len(pairs)
+
+    def step():
+        eng.get_scores_device(0, d_pairs.data_ptr(), d_ref.data_ptr(), d_qer.data_ptr(), n, w,
+                              stream.cuda_stream)
+
+    # first call on a fresh handle doubles as the parity check of a prefix
+    step()
+    torch.cuda.synchronize()
+    got = d_pairs.cpu().numpy().view(tal_bsw.SEQPAIR_DTYPE)
+    check = "skipped"
+    if args.check_pairs > 0:
+        check = _check_prefix(pairs, ref, qer, got, is8, w, args.check_pairs)
+        log(f"[bench] parity vs oracle on dispatcher prefixes: {check}")
+    for _ in range(max(0, args.warmup - 1)):
+        step()
+    torch.cuda.synchronize()
+
+    ms16, ms8 = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        st = eng.stats()
+        ms16.append(st["ms_main16"])
+        ms8.append(st["ms_main8"])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_step = (t1 - t0) * 1e3 / max(args.steps, 1)
+    total_cells = float(cells.sum())
+    gcups = total_cells / (ms_step / 1e3) / 1e9
+    st = eng.stats()
+    log(f"[bench] {ms_step:.1f} ms/step, {gcups:.1f} GCUPS; main16 {np.mean(ms16):.1f} ms, "
+        f"main8 {np.mean(ms8):.1f} ms, stats {st}")
+
+    cells16 = float(cells[~is8].sum())
+    k16_ms = float(np.mean(ms16))
+    achieved = OPS_PER_CELL * cells16 / (k16_ms / 1e3) / 1e12 if k16_ms > 0 else 0.0
+    roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": round(VALU_PEAK_TOPS, 2),
+                "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
+                "kernel": "bsw_main<16> (2 groups of 32 pairs per wave)",
+                "kernel_ms": round(k16_ms, 3), "kernel_gcups": round(cells16 / (k16_ms / 1e3) / 1e9, 1)
+                if k16_ms > 0 else None}
+
+    plain = None
+    if not args.no_plain:
+        plain = _plain(pairs, ref, qer, cells, w, args.plain_pairs)
+        log(f"[bench] plain per-chunk host calls: {plain}")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = _cpu_baseline(pairs, ref, qer, cells, is8, w, args.cpu_pairs)
+        log(f"[bench] cpu baseline: {cpu}")
+
+    line = {
+        "metric": "GCUPS (band cells/s) of bit-exact BSW, batched persistent dispatch; pairs/s",
+        "value": round(gcups, 2),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int16+int8",
+        "data": "synthetic (seeded generator, workload 2)",
+        "config": {"workload": "uneven batch, 8/16-bit dispatch", "pairs": n, "w": w,
+                   "query_len": "U[30,250]", "ref_flank": "U[0,100]", "h0": "U[0,120]",
+                   "pairs_8bit": int(is8.sum()), "band_cells": total_cells},
+        "pairs_per_s": round(n / (ms_step / 1e3), 1),
+        "parity": check,
+        "plain_gpu": plain,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if plain and plain.get("gcups"):
+        line["batched_vs_plain"] = round(gcups / plain["gcups"], 2)
+    print(json.dumps(line), flush=True)
+    eng.close()
+
+
+def _check_prefix(pairs, ref, qer, got, is8, w, m):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    o = oracle.Oracle(**PARAMS)
+    bad = 0
+    for sel, bits in ((np.where(is8)[0][:m], 8), (np.where(~is8)[0][:m], 16)):
+        sub = pairs[sel].copy()
+        want = o.scores8(sub, ref, qer, w) if bits == 8 else o.scores16(sub, ref, qer, w)
+        for f in tal_bsw.OUT_FIELDS:
+            bad += int((got[f][sel] != want[f]).sum())
+    return "bit-exact" if bad == 0 else f"{bad} mismatching fields"
+
+
+def _plain(pairs, ref, qer, cells, w, m):
+    """bench-bsw's pattern: one blocking getScores16 host call per 20,000
+    pairs (copies in and out each call)."""
+    eng = tal_bsw.BandedPairWiseSW(**PARAMS)
+    sub = pairs[:m].copy()
+    eng.get_scores16(sub[:20000].copy(), ref, qer, w)  # warm
+    t = time.perf_counter()
+    for b in range(0, m, 20000):
+        chunk = sub[b:b + 20000]
+        eng.get_scores16(chunk, ref, qer, w)
+    dt = time.perf_counter() - t
+    eng.close()
+    return {"form": "getScores16 per 20000-pair chunk, host buffers", "pairs": m,
+            "gcups": round(float(cells[:m].sum()) / dt / 1e9, 2), "pairs_per_s": round(m / dt, 1)}
+
+
+def _cpu_baseline(pairs, ref, qer, cells, is8, w, m):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = min(16, os.cpu_count() or 1)
+    sub, s8 = pairs[:m].copy(), is8[:m]
+    t = time.perf_counter()
+    oracle.run_mt(sub[s8].copy(), ref, qer, w, bits=8, nthreads=threads)
+    oracle.run_mt(sub[~s8].copy(), ref, qer, w, bits=16, nthreads=threads)
+    dt = time.perf_counter() - t
+    return {"value": round(float(cells[:m].sum()) / dt / 1e9, 3), "unit": "GCUPS",
+            "cores": threads, "kind": "port",
+            "sample": f"first {m} pairs of the workload, dispatcher split, 20000-pair chunks "
+                      f"per thread-owned oracle object, {dt:.1f}s"}
+
+
+if __name__ == "__main__":
+    main()

@@ -125,6 +125,24 @@ int tal_bsw_get_scores_auto_device(tal_bsw *h, tal_seqpair *pairs,
 int32_t tal_bsw_count_8bit(const tal_bsw *h, const tal_seqpair *pairs,
                            int32_t n);
 
+/* Timing and shape of the last call on the handle (no reference
+ * counterpart; the reference's RDT tick counters, bandedSWA.cpp:2947-2953,
+ * play this role on the CPU).  Kernel times come from HIP events recorded on
+ * the stream the kernels ran on; reading them waits for that stream. */
+typedef struct tal_bsw_stats {
+    float ms_main16;    /* 16-bit DP kernel, including stale-column reruns */
+    float ms_main8;     /* 8-bit DP kernel */
+    float ms_rerun16;   /* the rerun part of ms_main16 */
+    int32_t groups16;   /* 16-bit groups (32 pairs) in the last call */
+    int32_t groups8;    /* 8-bit groups (64 pairs) */
+    int32_t reruns16;   /* groups rerun for a stale F[ncol] value */
+    int32_t lds_state16; /* 1 if the 16-bit state lived in LDS */
+    int32_t pairs8;     /* pairs sent to the 8-bit path */
+    int32_t pairs16;    /* pairs sent to the 16-bit path */
+} tal_bsw_stats;
+
+int tal_bsw_get_stats(tal_bsw *h, tal_bsw_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

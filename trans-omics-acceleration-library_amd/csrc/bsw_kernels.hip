@@ -49,11 +49,17 @@ __device__ __forceinline__ int16_t abs16(int16_t a) { return a < 0 ? (int16_t)(-
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
-/* Reduction over the lanes of one group (xor offsets stay inside it). */
+/* Reductions over the lanes of one group (xor offsets stay inside it). */
 template <int GL> __device__ __forceinline__ int group_max(int x)
 {
 #pragma unroll
     for (int o = GL / 2; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, 64));
+    return x;
+}
+template <int GL> __device__ __forceinline__ int group_min(int x)
+{
+#pragma unroll
+    for (int o = GL / 2; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, 64));
     return x;
 }
 
@@ -84,26 +90,55 @@ __device__ __forceinline__ uint32_t pack(int16_t h, int16_t f)
     return (uint32_t)(uint16_t)h | ((uint32_t)(uint16_t)f << 16);
 }
 
+constexpr int NONE_COL = 0x7FFFFFFF;
+
+/* Query codes of the fast path: 0..4 as given, 5 past the end of the query
+ * (the reference's DUMMY2 pad), eight 4-bit codes per dword. */
+constexpr uint32_t QPAD = 5;
+
+/* Score of ref byte s1 against query code c (fast path) for building the
+ * per-row table. */
+template <int BITS> __device__ __forceinline__ int table_score(int s1, int c, int match, int mism)
+{
+    const int qbyte = c == (int)QPAD ? 100 : c;
+    const bool amb = (BITS == 16) ? (s1 == 4 || qbyte == 4)
+                                  : (s1 == 4 || s1 >= 128 || qbyte == 4);
+    return amb ? -1 : (s1 == qbyte ? match : mism);
+}
+
 /*
  * Runs group g (or nothing when g < 0) on this half-wave / wave.  The body is
  * the reference's smithWaterman512_{16,8} row loop with its batch wrapper's
  * per-group setup folded in.
+ *
+ * Q is the LDS area of the fast path's query codes ([word][64 lanes]), or
+ * null when the launch carries none.
  */
 template <int BITS>
-__device__ void run_group(const KParams &K, const KBuffers &B, int g, StateRef S)
+__device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRef S, uint32_t *Q)
 {
     typedef Width<BITS> Wd;
     constexpr int GL = Wd::GL;
+    constexpr int SH = Wd::SH;
     const int lane = threadIdx.x;
     const int half = lane / GL;
     const int gl = lane % GL;
     const uint64_t gmask = GL == 64 ? ~0ull : (0xFFFFFFFFull << (32 * half));
     const int16_t ONE = Wd::v(1), TWO = Wd::v(2);
 
+    /* parameters in registers (never indexed through the argument block) */
+    const int n_pairs = Kref.n, wband = Kref.w;
+    const int o_del = Kref.o_del, e_del = Kref.e_del, o_ins = Kref.o_ins, e_ins = Kref.e_ins;
+    const int end_bonus = Kref.end_bonus, smaxv = Kref.smax;
+    const int16_t k_match = Kref.match, k_mism = Kref.mism, k_ambig = Kref.ambig;
+    const int16_t oe_ins_s = Kref.oe_ins_s, e_ins_s = Kref.e_ins_s;
+    const int16_t oe_del_s = Kref.oe_del_s, e_del_s = Kref.e_del_s, zdrop_s = Kref.zdrop_s;
+    const int raw_match = (int)k_match >> SH, raw_mism = (int)k_mism >> SH;
+
     /* ---- per-lane pair (wrapper :2787-2826 / :2092-2120) ---- */
     const bool gvalid = g >= 0;
     const int p = gvalid ? g * GL + gl : 0;
-    const bool real = gvalid && p < K.n;
+    const bool real = gvalid && p < n_pairs;
     int len1 = 0, len2 = 0, h0 = 0, idr = 0, idq = 0;
     if (real) {
         const tal_seqpair *sp = B.pairs + p;
@@ -113,31 +148,35 @@ __device__ void run_group(const KParams &K, const KBuffers &B, int g, StateRef S
         len2 = sp->len2;
         h0 = sp->h0;
     } else if (gvalid && BITS == 8) {
-        len2 = K.last_len2; /* :2017 */
+        len2 = Kref.last_len2; /* :2017 */
     }
     const int nrow = group_max<GL>(len1);
     const int ncol = group_max<GL>(len2);
 
     /* per-lane band (:2882-2904): integer division then +1, floor 1, cap w */
-    int qlm = BITS == 16 ? (int)(uint16_t)(len2 * K.smax) : (int)(uint8_t)(len2 * K.smax);
-    int tb = BITS == 16 ? (int)(uint16_t)(qlm + K.end_bonus - K.o_ins)
-                        : (int)(uint8_t)(qlm + K.end_bonus - K.o_ins);
-    int b1 = tb / K.e_ins + 1;
+    int qlm = BITS == 16 ? (int)(uint16_t)(len2 * smaxv) : (int)(uint8_t)(len2 * smaxv);
+    int tb = BITS == 16 ? (int)(uint16_t)(qlm + end_bonus - o_ins)
+                        : (int)(uint8_t)(qlm + end_bonus - o_ins);
+    int b1 = tb / e_ins + 1;
     b1 = b1 > 1 ? b1 : 1;
-    int band = K.w < b1 ? K.w : b1;
+    int band = wband < b1 ? wband : b1;
     band = BITS == 16 ? (int)(uint16_t)band : (int)(uint8_t)band;
-    tb = BITS == 16 ? (int)(uint16_t)(qlm + K.end_bonus - K.o_del)
-                    : (int)(uint8_t)(qlm + K.end_bonus - K.o_del);
-    int b2 = tb / K.e_del + 1;
+    tb = BITS == 16 ? (int)(uint16_t)(qlm + end_bonus - o_del)
+                    : (int)(uint8_t)(qlm + end_bonus - o_del);
+    int b2 = tb / e_del + 1;
     b2 = b2 > 1 ? b2 : 1;
     band = band < b2 ? band : b2;
 
     const int16_t h0w = Wd::v(h0);
     const int16_t tlen = Wd::v(len1), qlen = Wd::v(len2), bandv = Wd::v(band);
+    const int qcol = (int)qlen >> SH; /* the query length as the lane sees it */
     int16_t head = 0, tail = qlen;
     const int16_t mlen = umin16(add16(qlen, bandv), tlen);
     int16_t maxScore = h0w, x = 0, y = 0, gscore = Wd::v(-1), max_off = 0, max_ie = 0;
     bool live = true; /* exit0 */
+
+    const uint8_t *rseq = B.ref + (int64_t)idr;
+    const uint8_t *qseq = B.qer + (int64_t)idq;
 
     /* ---- H_h row -1 and F (:2867-2879, :3034) ---- */
     const bool stale = BITS == 16 && B.stale_in && gvalid && ((B.stale_in[g] >> gl) & 1u);
@@ -145,20 +184,47 @@ __device__ void run_group(const KParams &K, const KBuffers &B, int g, StateRef S
         int16_t hv = h0w;
         for (int c = 0; c <= ncol; c++) {
             if (c == 1)
-                hv = h0w > K.oe_ins_s ? sub16(h0w, K.oe_ins_s) : (int16_t)0;
+                hv = h0w > oe_ins_s ? sub16(h0w, oe_ins_s) : (int16_t)0;
             else if (c >= 2)
-                hv = (c < ncol) ? smax16(sub16(hv, K.e_ins_s), 0) : (int16_t)0;
+                hv = (c < ncol) ? smax16(sub16(hv, e_ins_s), 0) : (int16_t)0;
             const int16_t f = (c < ncol) ? (int16_t)0 : (int16_t)(stale ? 1 : 0);
             S.st(c, pack(hv, f));
         }
     }
+    /* ---- query codes for the fast path ---- */
+    bool fast = Q != nullptr && raw_match >= -8 && raw_match <= 7 && raw_mism >= -8 &&
+                raw_mism <= 7;
+    if (fast) {
+        bool ok = true;
+        const int nwords = (ncol + 1 + 7) / 8 + 1;
+        for (int wd = 0; wd < nwords; wd++) {
+            uint32_t word = 0;
+            for (int k = 0; k < 8; k++) {
+                const int c = wd * 8 + k;
+                uint32_t code = QPAD;
+                if (c < len2) {
+                    const uint32_t b = qseq[c];
+                    ok = ok && b <= 4u;
+                    code = b;
+                }
+                word |= (code & 15u) << (4 * k);
+            }
+            Q[(size_t)wd * 64 + lane] = word;
+        }
+        fast = (ballot(!ok) & gmask) == 0;
+    }
 
     int beg = 0, end = ncol, nbeg = 0, nend = ncol;
+    /* previous row's column range and this lane's store band in it; before
+     * row 0 no column has been stored (pend = -1) */
+    int pbeg = 0, pend = -1, phd = 0, ptl = -1;
+    /* stale stretch [st_lo, st_hi) below the range start and its first/last
+     * nonzero column for this lane */
+    int st_lo = -1, st_hi = -1, st_first = NONE_COL, st_last = -1;
     bool gdone = !gvalid;
     bool fwritten = false;
     uint64_t dep = 0;
-    const uint8_t *rseq = B.ref + (int64_t)idr;
-    const uint8_t *qseq = B.qer + (int64_t)idq;
+    int s1_next = 0 < len1 ? (int)rseq[0] : 99;
 
     for (int i = 0;; i++) {
         const bool ract = !gdone && i < nrow;
@@ -167,73 +233,216 @@ __device__ void run_group(const KParams &K, const KBuffers &B, int g, StateRef S
             gdone = true;
             continue;
         }
+        const int s1 = s1_next; /* ref code of this row; next row's load in flight */
+        s1_next = i + 1 < len1 ? (int)rseq[i + 1] : 99;
         /* column range (:3059-3063) */
         beg = nbeg;
         end = nend;
-        if (beg < i - K.w) beg = i - K.w;
-        if (end > i + K.w + 1) end = i + K.w + 1;
+        if (beg < i - wband) beg = i - wband;
+        if (end > i + wband + 1) end = i + wband + 1;
         if (end > ncol) end = ncol;
         const int16_t i1 = Wd::v(i + 1);
         int16_t e11 = 0, maxRS = 0, y1 = 0;
         int16_t h10 = 0;
         if (beg == 0) { /* H_v[i+1] (:2828-2837, :3065-3067) */
             if (i + 1 < nrow)
-                h10 = smax16(Wd::v(h0 - K.o_del - (i + 1) * K.e_del), 0);
+                h10 = smax16(Wd::v(h0 - o_del - (i + 1) * e_del), 0);
             else
                 h10 = Wd::v(99); /* DUMMY1 left at H_v[maxLen1] (:2821-2825) */
         }
         /* per-lane band head/tail (:3080-3088) */
+        bool same;
         {
             const int16_t ph = head, pt = tail;
             head = smax16(head, sub16(Wd::v(i), bandv));
             tail = umin16(tail, add16(i1, bandv));
             tail = umin16(tail, qlen);
-            const bool same = (ballot(head == ph && tail == pt) & gmask) == gmask;
-            if (!same) { /* trim (:3092-3112) */
-                for (int c = beg; c < end; c++) {
-                    const bool below = head > Wd::v(c);
-                    if ((ballot(below) & gmask) == 0) break;
-                    if (below || Wd::v(c + 1) > tail) S.st(c, 0u);
-                }
-            }
+            same = (ballot(head == ph && tail == pt) & gmask) == gmask;
+        }
+        const int hd = (int)head >> SH, tl = (int)tail >> SH;
+        if (!same) {
+            /* Trim (:3092-3112).  The loop runs while some lane's head is
+             * beyond the column, i.e. over [beg, min(end, max head)), and
+             * zeroes a column in lanes whose band excludes it (c < head or
+             * c >= tail).  Every column of the previous row's range
+             * [pbeg, pend] was stored then and is already zero outside that
+             * row's [phd, ptl]; beg >= pbeg.  So only columns inside
+             * [phd, ptl] or above pend can still be nonzero. */
+            const int cstop = min(end, group_max<GL>(hd));
+            /* head side and tail side inside the previous band */
+            for (int c = max(beg, phd); c < min(min(hd, ptl + 1), cstop); c++) S.st(c, 0u);
+            for (int c = max(max(beg, phd), tl); c < min(ptl + 1, cstop); c++) S.st(c, 0u);
+            /* columns above the previous row's range */
+            for (int c = max(beg, pend + 1); c < cstop; c++)
+                if ((c < hd || c >= tl) && !(c >= phd && c <= ptl)) S.st(c, 0u);
         }
         if (i1 > mlen || tail == head || head > tail) live = false; /* :3118-3129 */
 
-        /* the column loop (:3134-3196) */
-        const int s1 = i < len1 ? (int)rseq[i] : 99;
-        int16_t jj = Wd::v(beg);
+        /* Column masks as unsigned range tests on j - head:
+         * stores keep column j iff head <= j <= tail, the row max counts it
+         * iff head <= j <= tail - 1 (:3150-3169). */
+        const uint32_t lim1 = hd <= tl ? (uint32_t)(tl - hd + 1) : 0u;
+        const uint32_t lim2 = hd <= tl - 1 ? (uint32_t)(tl - hd) : 0u;
+
+        /* the column loop (:3134-3196); fz/lz collect the first and last
+         * nonzero store of [beg, end) */
+        int fz = NONE_COL, lz = -1;
         int j = beg;
-        for (; j < end; j++) {
-            const int16_t pj = jj;
-            jj = add16(jj, ONE);
-            const uint32_t sv = S.ld(j);
-            const int16_t h00 = (int16_t)(sv & 0xFFFFu), f11 = (int16_t)(sv >> 16);
-            const int s2 = j < len2 ? (int)qseq[j] : 100;
-            const int16_t sbt = score_of<BITS>(s1, s2, K);
-            const int16_t m = h00 == 0 ? (int16_t)0 : add16(h00, sbt);
-            const int16_t h11 = smax16(smax16(m, e11), f11);
-            e11 = smax16(smax16(sub16(m, K.oe_ins_s), 0), sub16(e11, K.e_ins_s));
-            int16_t f21 = smax16(smax16(sub16(m, K.oe_del_s), 0), sub16(f11, K.e_del_s));
-            const bool left = head > pj;
-            const bool out1 = (pj > tail) || left;
-            const int16_t hst = out1 ? (int16_t)0 : h10;
-            if (out1) f21 = 0;
-            const bool out2 = (jj > tail) || left;
-            if (!out2) {
-                if (h11 >= maxRS) y1 = jj;
-                maxRS = smax16(maxRS, h11);
+        int16_t jj = Wd::v(beg);
+        if (fast) {
+            /* per-row score table: 4-bit signed score per query code */
+            uint32_t tbl = 0;
+#pragma unroll
+            for (int c = 0; c < 6; c++)
+                tbl |= ((uint32_t)table_score<BITS>(s1, c, raw_match, raw_mism) & 15u) << (4 * c);
+            /* Row max and its column as one key: h11 >= 0 always, so
+             * max(h11 << 16 | j+1) keeps the largest score and, among equal
+             * scores, the latest column -- the reference's update rule
+             * (maxRS' >= maxRS takes the new column, :3158-3169). */
+            uint32_t mkey = 0;
+            /* first and last 8-column block holding a nonzero store */
+            int bfirst = NONE_COL, blast = -1;
+            auto col = [&](int jc, uint32_t sv, uint32_t code) -> uint32_t {
+                const int16_t sbt =
+                    (int16_t)(__builtin_amdgcn_sbfe(tbl, code * 4, 4) * (1 << SH));
+                const int16_t h00 = (int16_t)(sv & 0xFFFFu), f11 = (int16_t)(sv >> 16);
+                const int16_t m = h00 == 0 ? (int16_t)0 : add16(h00, sbt);
+                const int16_t h11 = smax16(smax16(m, e11), f11);
+                e11 = smax16(smax16(sub16(m, oe_ins_s), 0), sub16(e11, e_ins_s));
+                const int16_t f21 = smax16(smax16(sub16(m, oe_del_s), 0), sub16(f11, e_del_s));
+                const uint32_t rel = (uint32_t)(jc - hd);
+                const uint32_t packed =
+                    __builtin_amdgcn_perm((uint32_t)f21, (uint32_t)h10, 0x05040100u);
+                const uint32_t key = ((uint32_t)(uint16_t)h11 << 16) | (uint32_t)(jc + 1);
+                mkey = max(mkey, rel < lim2 ? key : 0u);
+                h10 = h11;
+                return rel < lim1 ? packed : 0u;
+            };
+            /* 8-column blocks aligned to multiples of 8 (one query word,
+             * static code positions).  Edge blocks guard each column with
+             * beg <= j < end; middle blocks run unguarded. */
+            if (beg < end) {
+                const int b0 = beg & ~7;
+                uint32_t sv[8], sn[8];
+#pragma unroll
+                for (int k = 0; k < 8; k++) sv[k] = S.ld(b0 + k);
+                uint32_t qw = Q[(size_t)(b0 >> 3) * 64 + lane];
+                for (int b = b0; b < end; b += 8) {
+                    const bool more = b + 8 < end;
+                    uint32_t qn = 0;
+                    if (more) {
+#pragma unroll
+                        for (int k = 0; k < 8; k++) sn[k] = S.ld(b + 8 + k);
+                        qn = Q[(size_t)((b >> 3) + 1) * 64 + lane];
+                    }
+                    uint32_t any = 0;
+                    if (b >= beg && b + 8 <= end) {
+#pragma unroll
+                        for (int k = 0; k < 8; k++) {
+                            const uint32_t r = col(b + k, sv[k], (qw >> (4 * k)) & 15u);
+                            S.st(b + k, r);
+                            any |= r;
+                        }
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 8; k++)
+                            if (b + k >= beg && b + k < end) {
+                                const uint32_t r = col(b + k, sv[k], (qw >> (4 * k)) & 15u);
+                                S.st(b + k, r);
+                                any |= r;
+                            }
+                    }
+                    if (any != 0u) {
+                        bfirst = min(bfirst, b);
+                        blast = b;
+                    }
+                    if (more) {
+#pragma unroll
+                        for (int k = 0; k < 8; k++) sv[k] = sn[k];
+                        qw = qn;
+                    }
+                }
+                j = end;
             }
-            S.st(j, pack(hst, f21));
-            h10 = h11;
-            if (jj == qlen && live && !(jj > tail)) { /* gscore (:3176-3195) */
-                if (!(gscore > h11)) max_ie = i1;
-                gscore = smax16(gscore, h11);
+            jj = Wd::v(j);
+            maxRS = (int16_t)(mkey >> 16);
+            y1 = Wd::v((int)(mkey & 0xFFFFu));
+            /* exact first/last nonzero column of [beg, end) from the two
+             * flagged blocks (one 8-column read each) */
+            if (bfirst != NONE_COL) {
+                uint32_t nzm = 0;
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int c = bfirst + k;
+                    if (c >= beg && c < end && S.ld(c) != 0u) nzm |= 1u << k;
+                }
+                fz = bfirst + __builtin_ctz(nzm);
+                nzm = 0;
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int c = blast + k;
+                    if (c >= beg && c < end && S.ld(c) != 0u) nzm |= 1u << k;
+                }
+                lz = blast + 31 - __builtin_clz(nzm);
+            }
+        } else {
+            const int s1b = s1;
+            for (; j < end; j++) {
+                const int16_t pj = jj;
+                jj = add16(jj, ONE);
+                const uint32_t sv = S.ld(j);
+                const int16_t h00 = (int16_t)(sv & 0xFFFFu), f11 = (int16_t)(sv >> 16);
+                const int s2 = j < len2 ? (int)qseq[j] : 100;
+                bool amb;
+                if (BITS == 16)
+                    amb = (s1b == 4) | (s2 == 4);
+                else
+                    amb = (s1b == 4) | (s2 == 4) | (s1b >= 128) | (s2 >= 128);
+                const int16_t sbt = amb ? k_ambig : (s1b == s2 ? k_match : k_mism);
+                const int16_t m = h00 == 0 ? (int16_t)0 : add16(h00, sbt);
+                const int16_t h11 = smax16(smax16(m, e11), f11);
+                e11 = smax16(smax16(sub16(m, oe_ins_s), 0), sub16(e11, e_ins_s));
+                int16_t f21 = smax16(smax16(sub16(m, oe_del_s), 0), sub16(f11, e_del_s));
+                const bool left = head > pj;
+                const bool out1 = (pj > tail) || left;
+                const int16_t hst = out1 ? (int16_t)0 : h10;
+                if (out1) f21 = 0;
+                const bool out2 = (jj > tail) || left;
+                if (!out2 && h11 >= maxRS) {
+                    y1 = jj;
+                    maxRS = h11;
+                }
+                const uint32_t r = pack(hst, f21);
+                S.st(j, r);
+                h10 = h11;
+                if (r != 0u) {
+                    fz = min(fz, j);
+                    lz = j;
+                }
             }
         }
         { /* boundary store at j (end, or beg when the loop was empty) */
             const bool out1 = (jj > tail) || (head > jj);
-            S.st(j, pack(out1 ? (int16_t)0 : h10, 0));
+            const uint32_t r = pack(out1 ? (int16_t)0 : h10, 0);
+            S.st(j, r);
             if (j == ncol) fwritten = true;
+            /* fz: first nonzero in [beg, end) (end if none);
+             * lz: last nonzero in [beg, end] (beg - 1 if none) */
+            fz = fz == NONE_COL ? end : fz;
+            lz = (r != 0u && j >= beg) ? j : (lz >= beg ? lz : beg - 1);
+        }
+        pbeg = beg;
+        pend = j;
+        phd = hd;
+        ptl = tl;
+        /* gscore (:3176-3195): the cell of the last query column is updated
+         * when it was computed this row, the lane is live and tail == qlen;
+         * then H_h[qlen] holds it (head <= qlen, or the lane would not be
+         * live, so the store at column qlen was not masked). */
+        if (live && tail == qlen && qcol - 1 >= beg && qcol - 1 < end) {
+            const int16_t hq = (int16_t)(S.ld(qcol) & 0xFFFFu);
+            if (!(gscore > hq)) max_ie = i1;
+            gscore = smax16(gscore, hq);
         }
 
         /* all lanes zero in this row: the group stops (:3205-3209) */
@@ -254,68 +463,101 @@ __device__ void run_group(const KParams &K, const KBuffers &B, int g, StateRef S
             const int16_t di = sub16(i1, x), dj = sub16(y1, y);
             const int16_t off = di > dj ? sub16(di, dj) : sub16(dj, di);
             const int16_t drop = sub16(sub16(maxScore, maxRS), off);
-            if (drop > K.zdrop_s) live = false;
+            if (drop > zdrop_s) live = false;
         }
 
-        /* narrowing of the group's range (:3239-3260) */
-        int c;
-        for (c = beg; c < end; c++) {
-            if ((ballot(S.ld(c) == 0u) & gmask) == gmask)
-                nbeg = c;
-            else
-                break;
-        }
-        for (c = end; c >= beg; c--)
-            if ((ballot(S.ld(c) == 0u) & gmask) != gmask) break;
-        nend = (c + 2 < ncol) ? c + 2 : ncol;
+        /*
+         * Narrowing (:3239-3260) and the head/tail scans (:3269-3316) in
+         * closed form.  Every column of [beg, end] was stored this row, and a
+         * lane's stores are zero outside [head, tail], so the first and last
+         * nonzero columns of the lane in that range are found by searching
+         * from its own band edges:
+         *   fz = first nonzero in [beg, end)  (end if none)
+         *   lz = last nonzero in [beg, end]   (beg - 1 if none)
+         * The forward narrowing stops at the first column where some lane is
+         * nonzero (group min of fz); the backward one at the last (group max
+         * of lz).  A head scan from column bs gives a live lane that is zero
+         * at bs its first nonzero column in [bs, es) (es if none): the scan
+         * only breaks at a column where no live lane is zero, which is past
+         * every such lane's zero run.  The tail scan from es likewise gives a
+         * live lane that is zero at es the last nonzero column in [bs, es]
+         * (bs - 1 if none); other lanes keep tail.
+         */
+        const int c1 = group_min<GL>(fz);
+        if (c1 > beg) nbeg = c1 - 1;
+        const int lzmax = group_max<GL>(lz);
+        const int lmax = beg <= end ? lzmax : end;
+        nend = (lmax + 2 < ncol) ? lmax + 2 : ncol;
+        int bs = beg, es = end;
         if (BITS == 16) { /* :3267, 16-bit AVX-512 only */
-            beg = nbeg;
-            end = nend;
+            bs = nbeg;
+            es = nend;
         }
-
-        /* head scan (:3269-3291) */
-        uint64_t still = gmask;
-        int16_t lc = Wd::v(beg);
-        for (c = beg; c < end; c++) {
-            uint64_t z = ballot(S.ld(c) == 0u && live) & gmask;
-            if (z == 0) break;
-            z &= still;
-            lc = add16(lc, ONE);
-            if ((z >> lane) & 1ull) head = lc;
-            still = z;
+        /* The scans start at bs <= beg when the narrowing left the range
+         * start behind (16-bit).  Columns below beg are never stored again
+         * (beg only grows), so the first/last nonzero column of the stale
+         * stretch [bs, beg) is kept per lane and extended as beg moves. */
+        if (bs < beg) {
+            if (bs != st_lo) { /* a new stretch */
+                st_lo = bs;
+                st_hi = bs;
+                st_first = NONE_COL;
+                st_last = -1;
+            }
+            for (int c = st_hi; c < beg; c++)
+                if (S.ld(c) != 0u) {
+                    st_first = min(st_first, c);
+                    st_last = c;
+                }
+            st_hi = beg;
         }
+        /* first nonzero >= bs in [bs, es) (es if none) */
+        int fzs;
+        if (bs < beg && st_first != NONE_COL && st_first < es) {
+            fzs = st_first;
+        } else if (fz < end && fz >= bs && fz < es) {
+            fzs = fz;
+        } else {
+            int c = max(max(bs, beg), end);
+            while (c < es && S.ld(c) == 0u) c++;
+            fzs = c < es ? c : es;
+        }
+        /* last nonzero <= es in [bs, es] (bs - 1 if none) */
+        int lzs;
+        {
+            int c = es;
+            while (c > end && c >= bs && S.ld(c) == 0u) c--;
+            if (c > end && c >= bs)
+                lzs = c;
+            else if (lz >= beg && lz >= bs && lz <= es)
+                lzs = lz;
+            else if (bs < beg && st_last >= bs && st_last <= es)
+                lzs = st_last;
+            else
+                lzs = bs - 1;
+        }
+        if (live && bs < es && fzs > bs) head = Wd::v(fzs);
 
         /* 16-bit: is the tail scan about to read a stale F[ncol]?  A live
          * lane with H_h[ncol] == 0 then depends on it iff "nonzero at ncol"
          * (tail + 2) and "zero at ncol" (walk down) give different tails. */
-        if (BITS == 16 && end == ncol && !fwritten && end >= beg) {
+        if (BITS == 16 && es == ncol && !fwritten && es >= bs) {
             bool dec = false;
             if (live && (S.ld(ncol) & 0xFFFFu) == 0u) {
                 const int16_t tnz = smin16(add16(tail, TWO), qlen);
-                int16_t ix = Wd::v(ncol - 1);
-                for (int cc = ncol - 1; cc >= beg; cc--) {
-                    if (S.ld(cc) != 0u) break;
-                    ix = Wd::v(cc - 1);
-                }
-                const int16_t tz = smin16(add16(ix, TWO), qlen);
+                int c = ncol - 1;
+                while (c >= bs && S.ld(c) == 0u) c--;
+                const int16_t tz = smin16(add16(Wd::v(c >= bs ? c : bs - 1), TWO), qlen);
                 dec = tnz != tz;
             }
             dep |= ballot(dec) & gmask;
         }
 
-        /* tail scan (:3293-3316) */
-        int16_t idx = tail;
-        still = gmask;
-        lc = Wd::v(end);
-        for (c = end; c >= beg; c--) {
-            uint64_t z = ballot(S.ld(c) == 0u && live) & gmask;
-            if (z == 0) break;
-            z &= still;
-            lc = sub16(lc, ONE);
-            if ((z >> lane) & 1ull) idx = lc;
-            still = z;
+        /* tail scan result */
+        {
+            const int16_t idx = (live && es >= bs && lzs < es) ? Wd::v(lzs) : tail;
+            tail = smin16(add16(idx, TWO), qlen);
         }
-        tail = smin16(add16(idx, TWO), qlen);
     }
 
     if (real) { /* outputs (:3327-3353) */
@@ -350,8 +592,10 @@ __global__ __launch_bounds__(64) void bsw_main(KParams K, KBuffers B)
     typedef Width<BITS> Wd;
     const int lane = threadIdx.x;
     const int half = lane / Wd::GL;
+    /* state columns 0..max_ncol, then the query-code words */
     uint32_t *base = LDS ? lds_state : B.slabs + (size_t)blockIdx.x * K.state_cols * 64;
     StateRef S{base + lane};
+    uint32_t *Q = K.query_words > 0 ? base + (size_t)K.state_cols_q * 64 : nullptr;
     for (;;) {
         int t = 0;
         if (lane == 0) t = atomicAdd(B.task_counter, 1);
@@ -364,7 +608,7 @@ __global__ __launch_bounds__(64) void bsw_main(KParams K, KBuffers B)
             g = t * Wd::GPW + half;
             if (g >= K.ngroups) g = -1;
         }
-        run_group<BITS>(K, B, g, S);
+        run_group<BITS>(K, B, g, S, Q);
     }
 }
 

@@ -95,6 +95,12 @@ struct tal_bsw {
     DevBuf flag, idx_all, idx_sel, nsel, pairs8, pairs16, cubtmp;
     /* host-API staging */
     DevBuf d_pairs, d_ref, d_qer;
+    /* per-call timing */
+    /* 16-bit start/end, 8-bit start/end, rerun start/end */
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool ev16 = false, ev8 = false;
+    bool in_call = false; /* inside a dispatcher/host call */
+    tal_bsw_stats st{};
     std::string err;
 };
 
@@ -282,13 +288,21 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
     const int64_t total_cols = sum.total_cols;
 
     /* state placement: LDS if the widest group fits, else global slabs */
-    const int state_cols = max_ncol + 2;
-    const size_t lds_bytes = (size_t)state_cols * 64 * sizeof(uint32_t);
+    /* per wave: state columns 0..max_ncol, plus 17 columns of slack for the
+     * 8-column block loads that run past a group's last column, then the
+     * fast path's query-code words (8 columns per word, one word of prefetch
+     * slack) */
+    const int state_cols = max_ncol + 18;
+    const int qwords = (max_ncol >> 3) + 2;
+    const int slab_cols = state_cols + qwords;
+    const size_t lds_bytes = (size_t)slab_cols * 64 * sizeof(uint32_t);
     const bool lds = lds_bytes <= h->lds_per_block;
     const int ntasks = (G + GPW - 1) / GPW;
     int grid = grid_for(h, lds, lds_bytes, ntasks);
     K.ntasks = ntasks;
-    K.state_cols = lds ? 0 : state_cols;
+    K.state_cols = lds ? 0 : slab_cols;
+    K.state_cols_q = state_cols;
+    K.query_words = qwords;
 
     KBuffers B{};
     B.pairs = pairs;
@@ -297,7 +311,7 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
     HIPCHK(h, pb.counter.ensure(sizeof(int32_t) * 4), "alloc counter");
     B.task_counter = pb.counter.as<int32_t>();
     if (!lds) {
-        HIPCHK(h, pb.slabs.ensure((size_t)grid * state_cols * 64 * sizeof(uint32_t)),
+        HIPCHK(h, pb.slabs.ensure((size_t)grid * slab_cols * 64 * sizeof(uint32_t)),
                "alloc state slabs");
         B.slabs = pb.slabs.as<uint32_t>();
     }
@@ -317,7 +331,12 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
         B.fmask_off = pb.foff.as<int64_t>();
     }
     HIPCHK(h, hipMemsetAsync(B.task_counter, 0, sizeof(int32_t), s), "clear counter");
+    hipEvent_t ev_beg = h->ev[bits == 16 ? 0 : 2], ev_end = h->ev[bits == 16 ? 1 : 3];
+    HIPCHK(h, hipEventRecord(ev_beg, s), "event");
     HIPCHK(h, launch_main(bits, lds, grid, lds ? lds_bytes : 0, K, B, s), "main kernel");
+    HIPCHK(h, hipEventRecord(ev_end, s), "event");
+    int32_t reruns = 0;
+    float ms_rerun = 0.f;
 
     if (bits == 16) {
         int32_t *cnt = pb.counter.as<int32_t>() + 1;
@@ -332,6 +351,7 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
             HIPCHK(h, hipMemcpyAsync(&c, cnt, sizeof c, hipMemcpyDeviceToHost, s), "read reruns");
             HIPCHK(h, hipStreamSynchronize(s), "sync reruns");
             if (c == 0) break;
+            reruns += c;
             if (iter > G + 1) return fail(h, TAL_EDEVICE, "stale-column resolution did not converge");
             /* rerun the listed groups with their corrected F[ncol] masks */
             const int rt = (c + GPW - 1) / GPW;
@@ -344,14 +364,28 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
             B2.task_groups = pb.tasks.as<int32_t>();
             HIPCHK(h, hipMemsetAsync(B.task_counter, 0, sizeof(int32_t), s), "clear counter");
             const int g2 = grid_for(h, lds, lds_bytes, rt);
+            HIPCHK(h, hipEventRecord(h->ev[4], s), "event");
             HIPCHK(h, launch_main(16, lds, std::min(g2, grid), lds ? lds_bytes : 0, K2, B2, s),
                    "rerun kernel");
+            HIPCHK(h, hipEventRecord(h->ev[5], s), "event");
+            HIPCHK(h, hipEventSynchronize(h->ev[5]), "event sync");
+            float ms = 0.f;
+            HIPCHK(h, hipEventElapsedTime(&ms, h->ev[4], h->ev[5]), "event time");
+            ms_rerun += ms;
         }
+        h->ev16 = true;
+        h->st.ms_rerun16 = ms_rerun;
+        h->st.groups16 = G;
+        h->st.reruns16 = reruns;
+        h->st.lds_state16 = lds ? 1 : 0;
         HIPCHK(h, pb.last.ensure(sizeof(int32_t) * IMAGE_COLS), "alloc last");
         HIPCHK(h, hipMemsetAsync(pb.last.p, 0xFF, sizeof(int32_t) * IMAGE_COLS, s), "clear last");
         HIPCHK(h, launch_image_update(G, max_ncol + 1, shapes, B.wrote_out, pb.last.as<int32_t>(),
                                       B.fmask, B.fmask_off, h->image.as<uint32_t>(), s),
                "image update");
+    } else {
+        h->ev8 = true;
+        h->st.groups8 = G;
     }
     return TAL_OK;
 }
@@ -359,6 +393,10 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
 int check_common(tal_bsw *h, tal_seqpair *pairs, int32_t n, int32_t w)
 {
     if (!h) return TAL_EINVAL;
+    if (!h->in_call) { /* a new top-level call: forget the previous timing */
+        h->st = tal_bsw_stats{};
+        h->ev16 = h->ev8 = false;
+    }
     if (n < 0) return fail(h, TAL_EINVAL, "n < 0");
     if (w < 0) return fail(h, TAL_EINVAL, "w < 0");
     if (n > 0 && !pairs) return fail(h, TAL_EINVAL, "null pairs");
@@ -404,6 +442,11 @@ int host_call(tal_bsw *h, int mode, tal_seqpair *pairs, const uint8_t *ref, size
     if (qer_len)
         HIPCHK(h, hipMemcpyAsync(h->d_qer.p, qer, qer_len, hipMemcpyHostToDevice, s), "copy qer");
     tal_seqpair *dp = h->d_pairs.as<tal_seqpair>();
+    struct InCall {
+        tal_bsw *h;
+        ~InCall() { h->in_call = false; }
+    } guard{h};
+    h->in_call = true;
     if (mode == 16)
         rc = tal_bsw_get_scores16_device(h, dp, h->d_ref.as<uint8_t>(), h->d_qer.as<uint8_t>(), n,
                                          w, s);
@@ -450,6 +493,9 @@ int tal_bsw_create(const tal_bsw_params *params, int device, tal_bsw **out)
                                                               : prop.sharedMemPerBlock;
         if (set_main_lds_limit(h->lds_per_block) != hipSuccess) { rc = TAL_EDEVICE; break; }
         if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { rc = TAL_EDEVICE; break; }
+        bool evok = true;
+        for (hipEvent_t &e : h->ev) evok = evok && hipEventCreate(&e) == hipSuccess;
+        if (!evok) { rc = TAL_EDEVICE; break; }
         hipError_t e = h->image.ensure(sizeof(uint32_t) * IMAGE_COLS);
         if (e != hipSuccess) { rc = TAL_ENOMEM; break; }
         if (hipMemset(h->image.p, 0, sizeof(uint32_t) * IMAGE_COLS) != hipSuccess) { rc = TAL_EDEVICE; break; }
@@ -477,6 +523,8 @@ void tal_bsw_destroy(tal_bsw *h)
                         &pb->slabs,  &pb->last,  &pb->cubtmp, &pb->summary};
         for (DevBuf *b : bs) b->release();
     }
+    for (hipEvent_t e : h->ev)
+        if (e) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -570,6 +618,18 @@ int tal_bsw_get_scores_auto_device(tal_bsw *h, tal_seqpair *pairs, const uint8_t
                                                 s),
                "select 16-bit");
     }
+    h->st.pairs8 = n8;
+    h->st.pairs16 = n16;
+    const bool outer = !h->in_call;
+    h->in_call = true;
+    struct Restore {
+        tal_bsw *h;
+        bool outer;
+        ~Restore()
+        {
+            if (outer) h->in_call = false;
+        }
+    } restore{h, outer};
     HIPCHK(h, h->pairs8.ensure(sizeof(tal_seqpair) * std::max(n8, 1)), "alloc pairs8");
     HIPCHK(h, h->pairs16.ensure(sizeof(tal_seqpair) * std::max(n16, 1)), "alloc pairs16");
     if (n8) {
@@ -586,6 +646,25 @@ int tal_bsw_get_scores_auto_device(tal_bsw *h, tal_seqpair *pairs, const uint8_t
         HIPCHK(h, launch_scatter_out(h->pairs16.as<tal_seqpair>(), sel + n8, n16, pairs, s),
                "scatter16");
     }
+    return TAL_OK;
+}
+
+int tal_bsw_get_stats(tal_bsw *h, tal_bsw_stats *out)
+{
+    if (!h || !out) return TAL_EINVAL;
+    HIPCHK(h, hipSetDevice(h->device), "set device");
+    const float rerun = h->st.ms_rerun16;
+    h->st.ms_main16 = h->st.ms_main8 = 0.f;
+    if (h->ev16) {
+        HIPCHK(h, hipEventSynchronize(h->ev[1]), "event sync");
+        HIPCHK(h, hipEventElapsedTime(&h->st.ms_main16, h->ev[0], h->ev[1]), "event time");
+        h->st.ms_main16 += rerun;
+    }
+    if (h->ev8) {
+        HIPCHK(h, hipEventSynchronize(h->ev[3]), "event sync");
+        HIPCHK(h, hipEventElapsedTime(&h->st.ms_main8, h->ev[2], h->ev[3]), "event time");
+    }
+    *out = h->st;
     return TAL_OK;
 }
 

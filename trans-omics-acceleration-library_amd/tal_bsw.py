@@ -26,12 +26,19 @@ _STATUS = {1: "TAL_EINVAL", 2: "TAL_ENOMEM", 3: "TAL_EDEVICE", 4: "TAL_ERANGE"}
 EXPORTS = ["tal_bsw_create", "tal_bsw_destroy", "tal_bsw_last_error",
            "tal_bsw_get_scores16", "tal_bsw_get_scores8", "tal_bsw_get_scores_auto",
            "tal_bsw_get_scores16_device", "tal_bsw_get_scores8_device",
-           "tal_bsw_get_scores_auto_device", "tal_bsw_count_8bit"]
+           "tal_bsw_get_scores_auto_device", "tal_bsw_count_8bit", "tal_bsw_get_stats"]
 
 
 class Params(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in
                 ("o_del", "e_del", "o_ins", "e_ins", "zdrop", "end_bonus", "w_match", "w_mismatch")]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("ms_main16", ctypes.c_float), ("ms_main8", ctypes.c_float),
+                ("ms_rerun16", ctypes.c_float)] + [
+        (n, ctypes.c_int32) for n in ("groups16", "groups8", "reruns16", "lds_state16",
+                                      "pairs8", "pairs16")]
 
 
 class TalError(RuntimeError):
@@ -67,6 +74,8 @@ def load():
             f.argtypes = [P, P, P, P, i32, i32, P]
         L.tal_bsw_count_8bit.restype = i32
         L.tal_bsw_count_8bit.argtypes = [P, P, i32]
+        L.tal_bsw_get_stats.restype = ctypes.c_int
+        L.tal_bsw_get_stats.argtypes = [P, ctypes.POINTER(Stats)]
         _lib = L
     return _lib
 
@@ -124,5 +133,53 @@ class BandedPairWiseSW:
         self._check(fn(self._h, ctypes.c_void_p(d_pairs), ctypes.c_void_p(d_ref),
                        ctypes.c_void_p(d_qer), int(n), int(w), ctypes.c_void_p(stream or None)))
 
+    def stats(self):
+        st = Stats()
+        self._check(load().tal_bsw_get_stats(self._h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in Stats._fields_}
+
     def count_8bit(self, pairs):
         return int(load().tal_bsw_count_8bit(self._h, _ptr(pairs), len(pairs)))
+
+
+def _smax(w_match, w_mismatch):
+    # max over the object's scores: match, -mismatch penalty, ambiguity -1, and 0
+    return max(0, int(np.int8(w_match)), int(np.int8(-int(np.int8(w_mismatch)))), -1)
+
+
+def pair_band(len2, w, o_del=6, e_del=1, o_ins=6, e_ins=1, end_bonus=5, w_match=1,
+              w_mismatch=4, **_):
+    """Per-pair band of the 16-bit wrapper (bandedSWA.cpp:2882-2904)."""
+    sm = _smax(w_match, w_mismatch)
+    ql = (np.asarray(len2, np.int64) * sm) & 0xFFFF
+    b = np.full(ql.shape, w, np.int64)
+    for o, e in ((o_ins, e_ins), (o_del, e_del)):
+        t = (ql + end_bonus - o) & 0xFFFF
+        bb = np.maximum(np.trunc(t / e).astype(np.int64) + 1, 1)
+        b = np.minimum(b, bb) & 0xFFFF
+    return b
+
+
+def band_cells(len1, len2, w, **params):
+    """Cells inside each pair's band, sum_i max(0, min(len2, i+b+1) - max(0, i-b))
+    (the work unit of the GCUPS figures)."""
+    len1 = np.asarray(len1, np.int64)
+    len2 = np.asarray(len2, np.int64)
+    if len1.size == 0:
+        return np.zeros(0, np.int64)
+    keys, inv = np.unique((np.maximum(len1, 0) << 20) | np.maximum(len2, 0), return_inverse=True)
+    l1, l2 = keys >> 20, keys & 0xFFFFF
+    b = pair_band(l2, w, **params)
+    out = np.zeros(keys.shape, np.int64)
+    for i in range(int(l1.max())):
+        span = np.minimum(l2, i + b + 1) - np.maximum(0, i - b)
+        out += np.where(i < l1, np.maximum(span, 0), 0)
+    return out[inv.reshape(-1)]
+
+
+def is_8bit(pairs, w_match=1, w_mismatch=4, **_):
+    """Dispatcher rule of tal_bsw_get_scores_auto (include/tal_bsw.h)."""
+    sm = _smax(w_match, w_mismatch)
+    l1, l2, h0 = (pairs[f].astype(np.int64) for f in ("len1", "len2", "h0"))
+    return ((l1 >= 0) & (l2 >= 0) & (l1 <= 127) & (l2 <= 127) & (h0 >= 0)
+            & (h0 + sm * np.minimum(l1, l2) <= 127))
