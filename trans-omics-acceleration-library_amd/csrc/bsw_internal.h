@@ -29,9 +29,10 @@ struct KParams {
     int32_t o_del, e_del, o_ins, e_ins, end_bonus;
     int32_t smax;      /* max(0, w_match, -w_mismatch, ambig) */
     int32_t last_len2; /* 8-bit padding lanes take the last pair's len2 */
-    int32_t state_cols;   /* columns per global state slab (0 = LDS state) */
-    int32_t state_cols_q; /* state columns before the query-code words */
-    int32_t query_words;  /* query-code words per lane (0 = no fast path) */
+    int32_t state_cols;  /* state columns per wave (max ncol + slack) */
+    int32_t lds_cols;    /* state columns kept in LDS (multiple of 8) */
+    int32_t slab_cols;   /* columns per global slab (state_cols - lds_cols) */
+    int32_t query_words; /* query-code words per lane, in LDS after the state */
     int16_t match, mism, ambig;
     int16_t e_del_s, oe_del_s, e_ins_s, oe_ins_s, zdrop_s;
 };

@@ -78,11 +78,46 @@ __device__ __forceinline__ int16_t score_of(int s1, int s2, const KParams &K)
     return amb ? K.ambig : s;
 }
 
-/* One dword of state per (column, lane). */
+/* One dword of state per (column, lane): columns below `split` in LDS
+ * (lds[c * 64]), the rest in the block's global slab
+ * (glob[(c - split) * 64]).  split is a multiple of 8, so an aligned
+ * 8-column block lies wholly in one of the two. */
 struct StateRef {
-    uint32_t *base; /* column 0 of this lane: base[c * 64] */
-    __device__ __forceinline__ uint32_t ld(int c) const { return base[(size_t)c * 64]; }
-    __device__ __forceinline__ void st(int c, uint32_t v) const { base[(size_t)c * 64] = v; }
+    uint32_t *lds;
+    uint32_t *glob;
+    int split;
+    __device__ __forceinline__ uint32_t ld(int c) const
+    {
+        return c < split ? lds[(size_t)c * 64] : glob[(size_t)(c - split) * 64];
+    }
+    __device__ __forceinline__ void st(int c, uint32_t v) const
+    {
+        if (c < split)
+            lds[(size_t)c * 64] = v;
+        else
+            glob[(size_t)(c - split) * 64] = v;
+    }
+    /* the 8 columns of the aligned block at b */
+    __device__ __forceinline__ void ld8(int b, uint32_t *v) const
+    {
+        if (b < split) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = lds[(size_t)(b + k) * 64];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = glob[(size_t)(b - split + k) * 64];
+        }
+    }
+    __device__ __forceinline__ void st8(int b, const uint32_t *v) const
+    {
+        if (b < split) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) lds[(size_t)(b + k) * 64] = v[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < 8; k++) glob[(size_t)(b - split + k) * 64] = v[k];
+        }
+    }
 };
 
 __device__ __forceinline__ uint32_t pack(int16_t h, int16_t f)
@@ -192,7 +227,7 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
         }
     }
     /* ---- query codes for the fast path ---- */
-    bool fast = Q != nullptr && raw_match >= -8 && raw_match <= 7 && raw_mism >= -8 &&
+    bool fast = raw_match >= -8 && raw_match <= 7 && raw_mism >= -8 &&
                 raw_mism <= 7;
     if (fast) {
         bool ok = true;
@@ -209,7 +244,7 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                 }
                 word |= (code & 15u) << (4 * k);
             }
-            Q[(size_t)wd * 64 + lane] = word;
+            Q[(size_t)wd * 64] = word;
         }
         fast = (ballot(!ok) & gmask) == 0;
     }
@@ -324,25 +359,24 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
             if (beg < end) {
                 const int b0 = beg & ~7;
                 uint32_t sv[8], sn[8];
-#pragma unroll
-                for (int k = 0; k < 8; k++) sv[k] = S.ld(b0 + k);
-                uint32_t qw = Q[(size_t)(b0 >> 3) * 64 + lane];
+                S.ld8(b0, sv);
+                uint32_t qw = Q[(size_t)(b0 >> 3) * 64];
                 for (int b = b0; b < end; b += 8) {
                     const bool more = b + 8 < end;
                     uint32_t qn = 0;
                     if (more) {
-#pragma unroll
-                        for (int k = 0; k < 8; k++) sn[k] = S.ld(b + 8 + k);
-                        qn = Q[(size_t)((b >> 3) + 1) * 64 + lane];
+                        S.ld8(b + 8, sn);
+                        qn = Q[(size_t)((b >> 3) + 1) * 64];
                     }
                     uint32_t any = 0;
                     if (b >= beg && b + 8 <= end) {
+                        uint32_t r[8];
 #pragma unroll
                         for (int k = 0; k < 8; k++) {
-                            const uint32_t r = col(b + k, sv[k], (qw >> (4 * k)) & 15u);
-                            S.st(b + k, r);
-                            any |= r;
+                            r[k] = col(b + k, sv[k], (qw >> (4 * k)) & 15u);
+                            any |= r[k];
                         }
+                        S.st8(b, r);
                     } else {
 #pragma unroll
                         for (int k = 0; k < 8; k++)
@@ -585,17 +619,18 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
 }
 
 /* Persistent grid: each 64-thread block is one wave that pulls tasks. */
-template <int BITS, bool LDS>
+template <int BITS>
 __global__ __launch_bounds__(64) void bsw_main(KParams K, KBuffers B)
 {
     extern __shared__ uint32_t lds_state[];
     typedef Width<BITS> Wd;
     const int lane = threadIdx.x;
     const int half = lane / Wd::GL;
-    /* state columns 0..max_ncol, then the query-code words */
-    uint32_t *base = LDS ? lds_state : B.slabs + (size_t)blockIdx.x * K.state_cols * 64;
-    StateRef S{base + lane};
-    uint32_t *Q = K.query_words > 0 ? base + (size_t)K.state_cols_q * 64 : nullptr;
+    /* LDS: state columns [0, lds_cols), then the query-code words.  The
+     * block's global slab holds state columns [lds_cols, state_cols). */
+    uint32_t *glob = B.slabs ? B.slabs + (size_t)blockIdx.x * K.slab_cols * 64 + lane : nullptr;
+    StateRef S{lds_state + lane, glob, K.lds_cols};
+    uint32_t *Q = lds_state + (size_t)K.lds_cols * 64 + lane;
     for (;;) {
         int t = 0;
         if (lane == 0) t = atomicAdd(B.task_counter, 1);
@@ -758,26 +793,20 @@ __global__ void bsw_scatter_out(const tal_seqpair *src, const int32_t *idx, int3
 hipError_t launch_main(int bits, bool lds, int grid, size_t lds_bytes, const KParams &K,
                        const KBuffers &B, hipStream_t s)
 {
-    if (bits == 16) {
-        if (lds)
-            hipLaunchKernelGGL((bsw_main<16, true>), dim3(grid), dim3(64), lds_bytes, s, K, B);
-        else
-            hipLaunchKernelGGL((bsw_main<16, false>), dim3(grid), dim3(64), 0, s, K, B);
-    } else {
-        if (lds)
-            hipLaunchKernelGGL((bsw_main<8, true>), dim3(grid), dim3(64), lds_bytes, s, K, B);
-        else
-            hipLaunchKernelGGL((bsw_main<8, false>), dim3(grid), dim3(64), 0, s, K, B);
-    }
+    (void)lds;
+    if (bits == 16)
+        hipLaunchKernelGGL((bsw_main<16>), dim3(grid), dim3(64), lds_bytes, s, K, B);
+    else
+        hipLaunchKernelGGL((bsw_main<8>), dim3(grid), dim3(64), lds_bytes, s, K, B);
     return hipGetLastError();
 }
 
 hipError_t set_main_lds_limit(size_t bytes)
 {
-    hipError_t e = hipFuncSetAttribute((const void *)bsw_main<16, true>,
+    hipError_t e = hipFuncSetAttribute((const void *)bsw_main<16>,
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
     if (e != hipSuccess) return e;
-    return hipFuncSetAttribute((const void *)bsw_main<8, true>,
+    return hipFuncSetAttribute((const void *)bsw_main<8>,
                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
 
