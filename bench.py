@@ -51,7 +51,13 @@ def main():
     ap.add_argument("--check-pairs", type=int, default=20_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (for scaling the "
+                         "FETCH_SIZE/WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        _copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU: the engine has no multi-GPU path in this build "
               "(pairs are independent; see DESIGN.md for how it would shard).", file=sys.stderr)
@@ -153,6 +159,19 @@ def main():
         line["batched_vs_plain"] = round(gcups / plain["gcups"], 2)
     print(json.dumps(line), flush=True)
     eng.close()
+
+
+def _copy_calibration(mib):
+    """A plain copy kernel of known size: it reads and writes exactly
+    mib MiB, so the counters' reading for it gives their scale."""
+    import torch
+    n = mib * 2**20 // 4
+    src = torch.arange(n, dtype=torch.int32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": n * 4, "ok": bool((dst[-1] == n - 1).item())}), flush=True)
 
 
 def _check_prefix(pairs, ref, qer, got, is8, w, m):

@@ -626,11 +626,11 @@ __global__ __launch_bounds__(64) void bsw_main(KParams K, KBuffers B)
     typedef Width<BITS> Wd;
     const int lane = threadIdx.x;
     const int half = lane / Wd::GL;
-    /* LDS: state columns [0, lds_cols), then the query-code words.  The
-     * block's global slab holds state columns [lds_cols, state_cols). */
-    uint32_t *glob = B.slabs ? B.slabs + (size_t)blockIdx.x * K.slab_cols * 64 + lane : nullptr;
-    StateRef S{lds_state + lane, glob, K.lds_cols};
-    uint32_t *Q = lds_state + (size_t)K.lds_cols * 64 + lane;
+    /* LDS: state columns [0, lds_cols).  The block's global slab holds the
+     * query-code words, then state columns [lds_cols, state_cols). */
+    uint32_t *slab = B.slabs + (size_t)blockIdx.x * K.slab_cols * 64 + lane;
+    StateRef S{lds_state + lane, slab + (size_t)K.query_words * 64, K.lds_cols};
+    uint32_t *Q = slab;
     for (;;) {
         int t = 0;
         if (lane == 0) t = atomicAdd(B.task_counter, 1);

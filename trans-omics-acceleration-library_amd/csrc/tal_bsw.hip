@@ -90,7 +90,7 @@ struct tal_bsw {
     int cu_count = 0;
     size_t lds_per_block = 0;
     size_t lds_per_cu = 0;
-    int waves_per_cu = 10; /* target waves per CU for the LDS/global split */
+    int waves_per_cu = 12; /* target waves per CU for the LDS/global split */
     DevBuf image; /* IMAGE_COLS u32: F[c] nonzero lanes of the 16-bit scratch */
     PathBufs b16, b8;
     /* dispatcher scratch */
@@ -293,20 +293,20 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
     /* Per wave: state columns 0..max_ncol plus 17 columns of slack for the
      * 8-column block loads that run past a group's last column, and the
      * fast path's query-code words (8 columns per word, one word of prefetch
-     * slack).  The query words and the first lds_cols state columns live in
-     * LDS, sized so that `waves_per_cu` waves fit on a CU; the remaining
+     * slack).  The first lds_cols state columns live in LDS, sized so that
+     * `waves_per_cu` waves fit on a CU; the query words and the remaining
      * state columns go to a per-block slab in global memory. */
     const int state_cols = max_ncol + 18;
     const int qwords = (max_ncol >> 3) + 2;
     const int wpc = std::max(1, h->waves_per_cu);
-    const long budget_cols = (long)(h->lds_per_cu / wpc) / 256 - qwords;
+    const long budget_cols = (long)(h->lds_per_cu / wpc) / 256;
     const int fit_all = (state_cols + 7) & ~7;
-    int lds_cols = (int)std::max<long>(0, budget_cols) & ~7;
+    int lds_cols = (int)std::max<long>(8, budget_cols) & ~7;
     if (fit_all <= lds_cols) lds_cols = fit_all;
     const size_t max_cols = h->lds_per_block / 256;
-    if ((size_t)(lds_cols + qwords) > max_cols) lds_cols = (int)((max_cols - qwords) & ~(size_t)7);
-    const int slab_cols = std::max(0, state_cols - lds_cols);
-    const size_t lds_bytes = (size_t)(lds_cols + qwords) * 64 * sizeof(uint32_t);
+    if ((size_t)lds_cols > max_cols) lds_cols = (int)(max_cols & ~(size_t)7);
+    const int slab_cols = qwords + std::max(0, state_cols - lds_cols);
+    const size_t lds_bytes = (size_t)lds_cols * 64 * sizeof(uint32_t);
     const bool lds = true;
     const int ntasks = (G + GPW - 1) / GPW;
     int grid = grid_for(h, lds, lds_bytes, ntasks);
@@ -315,8 +315,6 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
     K.lds_cols = lds_cols;
     K.slab_cols = slab_cols;
     K.query_words = qwords;
-    if ((size_t)qwords * 256 > h->lds_per_block)
-        return fail(h, TAL_ERANGE, "query of %d columns does not fit in LDS", max_ncol);
 
     KBuffers B{};
     B.pairs = pairs;
@@ -324,11 +322,9 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
     B.qer = qer;
     HIPCHK(h, pb.counter.ensure(sizeof(int32_t) * 4), "alloc counter");
     B.task_counter = pb.counter.as<int32_t>();
-    if (slab_cols > 0) {
-        HIPCHK(h, pb.slabs.ensure((size_t)grid * slab_cols * 64 * sizeof(uint32_t)),
-               "alloc state slabs");
-        B.slabs = pb.slabs.as<uint32_t>();
-    }
+    HIPCHK(h, pb.slabs.ensure((size_t)grid * slab_cols * 64 * sizeof(uint32_t)),
+           "alloc state slabs");
+    B.slabs = pb.slabs.as<uint32_t>();
     if (bits == 16) {
         HIPCHK(h, pb.dep.ensure(sizeof(uint32_t) * G), "alloc dep");
         HIPCHK(h, pb.wrote.ensure(G), "alloc wrote");
@@ -391,7 +387,7 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
         h->st.ms_rerun16 = ms_rerun;
         h->st.groups16 = G;
         h->st.reruns16 = reruns;
-        h->st.lds_state16 = slab_cols == 0 ? 1 : 0;
+        h->st.lds_state16 = slab_cols == qwords ? 1 : 0;
         HIPCHK(h, pb.last.ensure(sizeof(int32_t) * IMAGE_COLS), "alloc last");
         HIPCHK(h, hipMemsetAsync(pb.last.p, 0xFF, sizeof(int32_t) * IMAGE_COLS, s), "clear last");
         HIPCHK(h, launch_image_update(G, max_ncol + 1, shapes, B.wrote_out, pb.last.as<int32_t>(),
