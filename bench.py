@@ -51,6 +51,9 @@ def main():
     ap.add_argument("--check-pairs", type=int, default=20_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-naive", action="store_true")
+    ap.add_argument("--bucket", type=int, default=20_000,
+                    help="pairs per launch of the naive form (bench-bsw's chunk size)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (for scaling the "
                          "FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -123,6 +126,11 @@ def main():
                 "kernel_ms": round(k16_ms, 3), "kernel_gcups": round(cells16 / (k16_ms / 1e3) / 1e9, 1)
                 if k16_ms > 0 else None}
 
+    naive = None
+    if not args.no_naive:
+        naive = _naive(eng, d_pairs, d_ref, d_qer, cells, n, w, stream, torch, args.bucket)
+        log(f"[bench] naive one launch per bucket: {naive}")
+
     plain = None
     if not args.no_plain:
         plain = _plain(pairs, ref, qer, cells, w, args.plain_pairs)
@@ -151,10 +159,13 @@ def main():
                    "pairs_8bit": int(is8.sum()), "band_cells": total_cells},
         "pairs_per_s": round(n / (ms_step / 1e3), 1),
         "parity": check,
+        "naive_gpu": naive,
         "plain_gpu": plain,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    if naive and naive.get("gcups"):
+        line["batched_vs_naive"] = round(gcups / naive["gcups"], 2)
     if plain and plain.get("gcups"):
         line["batched_vs_plain"] = round(gcups / plain["gcups"], 2)
     print(json.dumps(line), flush=True)
@@ -185,6 +196,26 @@ def _check_prefix(pairs, ref, qer, got, is8, w, m):
         for f in tal_bsw.OUT_FIELDS:
             bad += int((got[f][sel] != want[f]).sum())
     return "bit-exact" if bad == 0 else f"{bad} mismatching fields"
+
+
+def _naive(eng, d_pairs, d_ref, d_qer, cells, n, w, stream, torch, bucket=20_000):
+    """The naive GPU form: the same inputs resident in HBM, one dispatcher
+    launch per bucket of `bucket` consecutive pairs (bench-bsw's 20,000-pair
+    chunks; consecutive, so that the groups stay the reference's), waiting for
+    each bucket before the next, over the whole batch."""
+    rec = 56
+    base = d_pairs.data_ptr()
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for b in range(0, n, bucket):
+        m = min(bucket, n - b)
+        eng.get_scores_device(0, base + b * rec, d_ref.data_ptr(), d_qer.data_ptr(), m, w,
+                              stream.cuda_stream)
+        stream.synchronize()
+    dt = time.perf_counter() - t
+    return {"form": f"one dispatcher launch per {bucket}-pair bucket, device-resident, "
+                    f"synchronous", "pairs": n, "buckets": (n + bucket - 1) // bucket,
+            "gcups": round(float(cells.sum()) / dt / 1e9, 2), "ms": round(dt * 1e3, 1)}
 
 
 def _plain(pairs, ref, qer, cells, w, m):

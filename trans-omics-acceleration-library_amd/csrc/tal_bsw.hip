@@ -78,7 +78,7 @@ struct DevBuf {
 /* Per-width scratch of one call. */
 struct PathBufs {
     DevBuf shapes, ncp1, foff, dep, wrote, stale, fmask, rerun, tasks, counter, slabs, last,
-        cubtmp, summary;
+        cubtmp, summary, ckey, ckey2, gids, order;
 };
 
 } // namespace
@@ -91,6 +91,7 @@ struct tal_bsw {
     size_t lds_per_block = 0;
     size_t lds_per_cu = 0;
     int waves_per_cu = 12; /* target waves per CU for the LDS/global split */
+    int order = 1;         /* task order: 0 = input order, 1 = longest groups first */
     DevBuf image; /* IMAGE_COLS u32: F[c] nonzero lanes of the 16-bit scratch */
     PathBufs b16, b8;
     /* dispatcher scratch */
@@ -215,6 +216,26 @@ __global__ void iota32(int32_t *x, int32_t n)
     if (k < n) x[k] = k;
 }
 
+/* Cost key of each group for the task order: rows times the widest column
+ * range a row can sweep (the group's columns, at most 2w+2). */
+__global__ void group_cost(const GroupShape *shapes, int32_t ngroups, int32_t w, uint32_t *key,
+                           int32_t *ids)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    const int64_t width = min((int64_t)shapes[g].ncol, 2 * (int64_t)w + 2) + 1;
+    const int64_t c = (int64_t)shapes[g].nrow * width;
+    key[g] = (uint32_t)min(c, (int64_t)0xFFFFFFFF);
+    ids[g] = g;
+}
+
+/* Marks the unused task slots past the last group. */
+__global__ void fill_i32(int32_t *x, int32_t n, int32_t v)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) x[k] = v;
+}
+
 /* Pairs up a rerun list into 16-bit wave tasks (two groups per task). */
 __global__ void make_tasks(const int32_t *list, const int32_t *count, int32_t *tasks,
                            int32_t cap_tasks, int gpw)
@@ -320,6 +341,35 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
     B.pairs = pairs;
     B.ref = ref;
     B.qer = qer;
+    if (h->order == 1 && G > 1) {
+        /* Longest groups first, and the two groups of a 16-bit wave of
+         * similar size: the persistent grid then ends evenly, and the two
+         * half-waves of a wave run loops of similar length. */
+        HIPCHK(h, pb.ckey.ensure(sizeof(uint32_t) * G), "alloc keys");
+        HIPCHK(h, pb.ckey2.ensure(sizeof(uint32_t) * G), "alloc keys");
+        HIPCHK(h, pb.gids.ensure(sizeof(int32_t) * G), "alloc ids");
+        HIPCHK(h, pb.order.ensure(sizeof(int32_t) * ((size_t)ntasks * GPW)), "alloc order");
+        hipLaunchKernelGGL(group_cost, dim3((G + 255) / 256), dim3(256), 0, s, shapes, G, w,
+                           pb.ckey.as<uint32_t>(), pb.gids.as<int32_t>());
+        HIPCHK(h, hipGetLastError(), "group cost");
+        size_t tmp = 0;
+        HIPCHK(h, hipcub::DeviceRadixSort::SortPairsDescending(
+                      nullptr, tmp, pb.ckey.as<uint32_t>(), pb.ckey2.as<uint32_t>(),
+                      pb.gids.as<int32_t>(), pb.order.as<int32_t>(), G, 0, 32, s),
+               "sort size");
+        HIPCHK(h, pb.cubtmp.ensure(tmp + 16), "alloc sort tmp");
+        HIPCHK(h, hipcub::DeviceRadixSort::SortPairsDescending(
+                      pb.cubtmp.p, tmp, pb.ckey.as<uint32_t>(), pb.ckey2.as<uint32_t>(),
+                      pb.gids.as<int32_t>(), pb.order.as<int32_t>(), G, 0, 32, s),
+               "sort groups");
+        const int pad = ntasks * GPW - G;
+        if (pad > 0) {
+            hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, s, pb.order.as<int32_t>() + G, pad,
+                               -1);
+            HIPCHK(h, hipGetLastError(), "pad tasks");
+        }
+        B.task_groups = pb.order.as<int32_t>();
+    }
     HIPCHK(h, pb.counter.ensure(sizeof(int32_t) * 4), "alloc counter");
     B.task_counter = pb.counter.as<int32_t>();
     HIPCHK(h, pb.slabs.ensure((size_t)grid * slab_cols * 64 * sizeof(uint32_t)),
@@ -503,6 +553,7 @@ int tal_bsw_create(const tal_bsw_params *params, int device, tal_bsw **out)
                                                               : prop.sharedMemPerBlock;
         if (set_main_lds_limit(h->lds_per_block) != hipSuccess) { rc = TAL_EDEVICE; break; }
         if (const char *e = getenv("TAL_BSW_WAVES_PER_CU")) h->waves_per_cu = std::max(1, atoi(e));
+        if (const char *e = getenv("TAL_BSW_ORDER")) h->order = atoi(e) ? 1 : 0;
         if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { rc = TAL_EDEVICE; break; }
         bool evok = true;
         for (hipEvent_t &e : h->ev) evok = evok && hipEventCreate(&e) == hipSuccess;
@@ -531,7 +582,8 @@ void tal_bsw_destroy(tal_bsw *h)
     for (PathBufs *pb : {&h->b16, &h->b8}) {
         DevBuf *bs[] = {&pb->shapes, &pb->ncp1, &pb->foff,    &pb->dep,   &pb->wrote,
                         &pb->stale,  &pb->fmask, &pb->rerun,  &pb->tasks, &pb->counter,
-                        &pb->slabs,  &pb->last,  &pb->cubtmp, &pb->summary};
+                        &pb->slabs,  &pb->last,  &pb->cubtmp, &pb->summary, &pb->ckey,
+                        &pb->ckey2,  &pb->gids,  &pb->order};
         for (DevBuf *b : bs) b->release();
     }
     for (hipEvent_t e : h->ev)
