@@ -49,6 +49,13 @@ __device__ __forceinline__ int16_t abs16(int16_t a) { return a < 0 ? (int16_t)(-
 
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 
+/* Two int16 lanes in one dword: packed add/sub wrap per half and max is
+ * signed per half, exactly as the scalar int16 forms. */
+typedef short s2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ s2v as_s2(uint32_t x) { return __builtin_bit_cast(s2v, x); }
+__device__ __forceinline__ uint32_t as_u32(s2v x) { return __builtin_bit_cast(uint32_t, x); }
+__device__ __forceinline__ s2v max2(s2v a, s2v b) { return __builtin_elementwise_max(a, b); }
+
 /* Reductions over the lanes of one group (xor offsets stay inside it). */
 template <int GL> __device__ __forceinline__ int group_max(int x)
 {
@@ -82,40 +89,44 @@ __device__ __forceinline__ int16_t score_of(int s1, int s2, const KParams &K)
  * (lds[c * 64]), the rest in the block's global slab
  * (glob[(c - split) * 64]).  split is a multiple of 8, so an aligned
  * 8-column block lies wholly in one of the two. */
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(1))) uint32_t glb_u32;
+
 struct StateRef {
-    uint32_t *lds;
-    uint32_t *glob;
+    lds_u32 *lds;
+    glb_u32 *glob;
     int split;
-    __device__ __forceinline__ uint32_t ld(int c) const
+    /* generic address of column c */
+    __device__ __forceinline__ uint32_t *at(int c) const
     {
-        return c < split ? lds[(size_t)c * 64] : glob[(size_t)(c - split) * 64];
+        return c < split ? (uint32_t *)(lds + c * 64) : (uint32_t *)(glob + (size_t)(c - split) * 64);
     }
-    __device__ __forceinline__ void st(int c, uint32_t v) const
-    {
-        if (c < split)
-            lds[(size_t)c * 64] = v;
-        else
-            glob[(size_t)(c - split) * 64] = v;
-    }
-    /* the 8 columns of the aligned block at b */
+    __device__ __forceinline__ uint32_t ld(int c) const { return *at(c); }
+    __device__ __forceinline__ void st(int c, uint32_t v) const { *at(c) = v; }
+    /* the 8 columns of the aligned block at b (LDS and global accesses kept
+     * apart, so that neither becomes a generic access) */
     __device__ __forceinline__ void ld8(int b, uint32_t *v) const
     {
         if (b < split) {
+            const lds_u32 *p = lds + b * 64;
 #pragma unroll
-            for (int k = 0; k < 8; k++) v[k] = lds[(size_t)(b + k) * 64];
+            for (int k = 0; k < 8; k++) v[k] = p[k * 64];
         } else {
+            const glb_u32 *p = glob + (size_t)(b - split) * 64;
 #pragma unroll
-            for (int k = 0; k < 8; k++) v[k] = glob[(size_t)(b - split + k) * 64];
+            for (int k = 0; k < 8; k++) v[k] = p[k * 64];
         }
     }
     __device__ __forceinline__ void st8(int b, const uint32_t *v) const
     {
         if (b < split) {
+            lds_u32 *p = lds + b * 64;
 #pragma unroll
-            for (int k = 0; k < 8; k++) lds[(size_t)(b + k) * 64] = v[k];
+            for (int k = 0; k < 8; k++) p[k * 64] = v[k];
         } else {
+            glb_u32 *p = glob + (size_t)(b - split) * 64;
 #pragma unroll
-            for (int k = 0; k < 8; k++) glob[(size_t)(b - split + k) * 64] = v[k];
+            for (int k = 0; k < 8; k++) p[k * 64] = v[k];
         }
     }
 };
@@ -227,9 +238,10 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
         }
     }
     /* ---- query codes for the fast path ---- */
-    bool fast = raw_match >= -8 && raw_match <= 7 && raw_mism >= -8 &&
-                raw_mism <= 7;
-    if (fast) {
+    /* every int8 score fits the per-row byte table; the fast path only
+     * needs query bytes 0-4 */
+    bool fast = true;
+    {
         bool ok = true;
         const int nwords = (ncol + 1 + 7) / 8 + 1;
         for (int wd = 0; wd < nwords; wd++) {
@@ -313,11 +325,9 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
         }
         if (i1 > mlen || tail == head || head > tail) live = false; /* :3118-3129 */
 
-        /* Column masks as unsigned range tests on j - head:
-         * stores keep column j iff head <= j <= tail, the row max counts it
-         * iff head <= j <= tail - 1 (:3150-3169). */
+        /* Column mask as an unsigned range test on j - head: stores keep
+         * column j iff head <= j <= tail (:3150-3157). */
         const uint32_t lim1 = hd <= tl ? (uint32_t)(tl - hd + 1) : 0u;
-        const uint32_t lim2 = hd <= tl - 1 ? (uint32_t)(tl - hd) : 0u;
 
         /* the column loop (:3134-3196); fz/lz collect the first and last
          * nonzero store of [beg, end) */
@@ -325,34 +335,66 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
         int j = beg;
         int16_t jj = Wd::v(beg);
         if (fast) {
-            /* per-row score table: 4-bit signed score per query code */
-            uint32_t tbl = 0;
+            /* Per-row score table: one signed byte per query code, codes 0-3
+             * in tlo and 4-5 in thi, looked up for four columns at a time by
+             * a byte permute. */
+            uint32_t tlo = 0, thi = 0;
 #pragma unroll
-            for (int c = 0; c < 6; c++)
-                tbl |= ((uint32_t)table_score<BITS>(s1, c, raw_match, raw_mism) & 15u) << (4 * c);
-            /* Row max and its column as one key: h11 >= 0 always, so
-             * max(h11 << 16 | j+1) keeps the largest score and, among equal
-             * scores, the latest column -- the reference's update rule
-             * (maxRS' >= maxRS takes the new column, :3158-3169). */
+            for (int c = 0; c < 4; c++)
+                tlo |= ((uint32_t)table_score<BITS>(s1, c, raw_match, raw_mism) & 255u) << (8 * c);
+#pragma unroll
+            for (int c = 4; c < 6; c++)
+                thi |= ((uint32_t)table_score<BITS>(s1, c, raw_match, raw_mism) & 255u)
+                       << (8 * (c - 4));
+            /* The E and F recurrences have the same form,
+             *   e' = max(m - oe_ins, 0, e - e_ins),  f' = max(m - oe_del, 0, f - e_del),
+             * so they run as one packed pair: ez holds (e, f') and the state
+             * dword (H_h, F). */
+            const s2v OE = {oe_ins_s, oe_del_s};
+            const s2v EE = {e_ins_s, e_del_s};
+            const s2v ZERO2 = {0, 0};
+            uint32_t ez = 0;                       /* low half: E entering the column */
+            uint32_t hprev = (uint32_t)(uint16_t)h10; /* low half: H of the previous column */
+            /* sbt: the score in the low half of a dword (sign-extended);
+             * only low halves are meaningful in m, h11 and the adds below */
+            auto cell = [&](uint32_t sv, uint32_t sbt) -> uint32_t {
+                const uint32_t m = (sv & 0xFFFFu) == 0u ? 0u : sv + sbt;
+                /* (e, f) of this column: e from ez, f from the state */
+                const uint32_t ef = __builtin_amdgcn_perm(sv, ez, 0x07060100u);
+                const s2v mm = __builtin_shufflevector(as_s2(m), as_s2(m), 0, 0);
+                const s2v z = max2(max2(mm - OE, as_s2(ef) - EE), ZERO2);
+                /* h11 = max(m, e, f) in the low half; the high half is
+                 * never read (the permute below takes bytes 0-1 only) */
+                uint32_t h11;
+                asm("v_max_i16_sdwa %0, %1, %1 dst_sel:WORD_0 dst_unused:UNUSED_PAD "
+                    "src0_sel:WORD_0 src1_sel:WORD_1\n\t"
+                    "v_max_i16_e32 %0, %2, %0"
+                    : "=&v"(h11)
+                    : "v"(ef), "v"(m));
+                /* the store: H of the previous column, F of this one */
+                const uint32_t st = __builtin_amdgcn_perm(as_u32(z), hprev, 0x07060100u);
+                ez = as_u32(z);
+                hprev = h11;
+                return st;
+            };
+            auto score = [&](uint32_t sc, int byte) -> uint32_t {
+                const uint32_t b8 = (sc >> (8 * byte)) & 255u;
+                return BITS == 16 ? (uint32_t)(int32_t)(int8_t)b8 : b8 << 8;
+            };
+            /* Row max and its column as one key, read off the stores: the
+             * store at column j holds H of column j-1 (masked to the lane's
+             * band [head, tail]), so max over j in (beg, end] of
+             * (store_j << 16 | j) is max over columns j-1 of (h11 << 16 |
+             * column+1).  H >= 0, so ties keep the latest column -- the
+             * reference's rule (maxRS' >= maxRS takes the new column,
+             * :3158-3169).  The row max counts column c iff head <= c <=
+             * tail-1; the stores add only c = head-1, whose H is 0 (the
+             * columns from beg to head-1 were trimmed to zero), and masked
+             * zeros.  Zero-score keys never win over a positive row max, and
+             * when the row max is 0 its column is not used. */
             uint32_t mkey = 0;
             /* first and last 8-column block holding a nonzero store */
             int bfirst = NONE_COL, blast = -1;
-            auto col = [&](int jc, uint32_t sv, uint32_t code) -> uint32_t {
-                const int16_t sbt =
-                    (int16_t)(__builtin_amdgcn_sbfe(tbl, code * 4, 4) * (1 << SH));
-                const int16_t h00 = (int16_t)(sv & 0xFFFFu), f11 = (int16_t)(sv >> 16);
-                const int16_t m = h00 == 0 ? (int16_t)0 : add16(h00, sbt);
-                const int16_t h11 = smax16(smax16(m, e11), f11);
-                e11 = smax16(smax16(sub16(m, oe_ins_s), 0), sub16(e11, e_ins_s));
-                const int16_t f21 = smax16(smax16(sub16(m, oe_del_s), 0), sub16(f11, e_del_s));
-                const uint32_t rel = (uint32_t)(jc - hd);
-                const uint32_t packed =
-                    __builtin_amdgcn_perm((uint32_t)f21, (uint32_t)h10, 0x05040100u);
-                const uint32_t key = ((uint32_t)(uint16_t)h11 << 16) | (uint32_t)(jc + 1);
-                mkey = max(mkey, rel < lim2 ? key : 0u);
-                h10 = h11;
-                return rel < lim1 ? packed : 0u;
-            };
             /* 8-column blocks aligned to multiples of 8 (one query word,
              * static code positions).  Edge blocks guard each column with
              * beg <= j < end; middle blocks run unguarded. */
@@ -368,24 +410,35 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                         S.ld8(b + 8, sn);
                         qn = Q[(size_t)((b >> 3) + 1) * 64];
                     }
-                    uint32_t any = 0;
-                    if (b >= beg && b + 8 <= end) {
+                    /* scores of columns b+0,2,4,6 (sce) and b+1,3,5,7 (sco) */
+                    const uint32_t sce = __builtin_amdgcn_perm(thi, tlo, qw & 0x0F0F0F0Fu);
+                    const uint32_t sco = __builtin_amdgcn_perm(thi, tlo, (qw >> 4) & 0x0F0F0F0Fu);
+                    const uint32_t rel0 = (uint32_t)(b - hd);
+                    uint32_t any = 0, kb = 0;
+                    if (b > beg && b + 8 <= end) {
                         uint32_t r[8];
 #pragma unroll
                         for (int k = 0; k < 8; k++) {
-                            r[k] = col(b + k, sv[k], (qw >> (4 * k)) & 15u);
+                            const uint32_t st = cell(sv[k], score((k & 1) ? sco : sce, k >> 1));
+                            r[k] = rel0 + k < lim1 ? st : 0u;
+                            kb = max(kb, (r[k] << 16) | (uint32_t)k);
                             any |= r[k];
                         }
                         S.st8(b, r);
                     } else {
+                        uint32_t *ep = S.at(b);
 #pragma unroll
                         for (int k = 0; k < 8; k++)
                             if (b + k >= beg && b + k < end) {
-                                const uint32_t r = col(b + k, sv[k], (qw >> (4 * k)) & 15u);
-                                S.st(b + k, r);
+                                const uint32_t st =
+                                    cell(sv[k], score((k & 1) ? sco : sce, k >> 1));
+                                const uint32_t r = rel0 + k < lim1 ? st : 0u;
+                                ep[k * 64] = r;
                                 any |= r;
+                                if (b + k > beg) kb = max(kb, (r << 16) | (uint32_t)k);
                             }
                     }
+                    mkey = max(mkey, kb + (uint32_t)b);
                     if (any != 0u) {
                         bfirst = min(bfirst, b);
                         blast = b;
@@ -398,7 +451,14 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                 }
                 j = end;
             }
+            h10 = (int16_t)hprev;
             jj = Wd::v(j);
+            /* the key of column end-1 comes from the boundary store at end */
+            if (beg < end) {
+                const bool out1 = (jj > tail) || (head > jj);
+                const uint32_t rb = out1 ? 0u : (uint32_t)(uint16_t)h10;
+                mkey = max(mkey, (rb << 16) | (uint32_t)end);
+            }
             maxRS = (int16_t)(mkey >> 16);
             y1 = Wd::v((int)(mkey & 0xFFFFu));
             /* exact first/last nonzero column of [beg, end) from the two
@@ -629,7 +689,8 @@ __global__ __launch_bounds__(64) void bsw_main(KParams K, KBuffers B)
     /* LDS: state columns [0, lds_cols).  The block's global slab holds the
      * query-code words, then state columns [lds_cols, state_cols). */
     uint32_t *slab = B.slabs + (size_t)blockIdx.x * K.slab_cols * 64 + lane;
-    StateRef S{lds_state + lane, slab + (size_t)K.query_words * 64, K.lds_cols};
+    StateRef S{(lds_u32 *)(lds_state + lane), (glb_u32 *)(slab + (size_t)K.query_words * 64),
+               K.lds_cols};
     uint32_t *Q = slab;
     for (;;) {
         int t = 0;

@@ -90,7 +90,7 @@ struct tal_bsw {
     int cu_count = 0;
     size_t lds_per_block = 0;
     size_t lds_per_cu = 0;
-    int waves_per_cu = 12; /* target waves per CU for the LDS/global split */
+    int waves_per_cu = 8;  /* target waves per CU for the LDS/global split */
     int order = 1;         /* task order: 0 = input order, 1 = longest groups first */
     DevBuf image; /* IMAGE_COLS u32: F[c] nonzero lanes of the 16-bit scratch */
     PathBufs b16, b8;
