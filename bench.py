@@ -54,6 +54,10 @@ def main():
     ap.add_argument("--no-naive", action="store_true")
     ap.add_argument("--bucket", type=int, default=20_000,
                     help="pairs per launch of the naive form (bench-bsw's chunk size)")
+    ap.add_argument("--serve", action="store_true",
+                    help="streamed serving instead: Poisson arrivals of 1k-64k-pair requests, "
+                         "p50/p99 latency at 50/80/95%% of the measured peak")
+    ap.add_argument("--serve-requests", type=int, default=400, help="requests per load level")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (for scaling the "
                          "FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -65,6 +69,9 @@ def main():
         print("bench.py runs on one GPU: the engine has no multi-GPU path in this build "
               "(pairs are independent; see DESIGN.md for how it would shard).", file=sys.stderr)
         sys.exit(2)
+    if args.serve:
+        _serve(args)
+        return
 
     import torch
 
@@ -170,6 +177,101 @@ def main():
         line["batched_vs_plain"] = round(gcups / plain["gcups"], 2)
     print(json.dumps(line), flush=True)
     eng.close()
+
+
+def _slice_request(pairs, ref, qer, a, b):
+    """Pairs [a, b) of a generated pool as a self-contained request: its own
+    pair records, and views of the pool's sequence bytes (the generator lays
+    them out in pair order) with the offsets rebased."""
+    p = pairs[a:b].copy()
+    r0, r1 = int(p["idr"][0]), int(p["idr"][-1] + p["len1"][-1])
+    q0, q1 = int(p["idq"][0]), int(p["idq"][-1] + p["len2"][-1])
+    p["idr"] -= r0
+    p["idq"] -= q0
+    return p, ref[r0:r1], qer[q0:q1]
+
+
+def _serve(args):
+    """Streamed serving: requests of log-uniform 1k-64k pairs drawn from the
+    workload-1 and workload-2 generators arrive as a Poisson process at 50%,
+    80% and 95% of the peak pairs/s measured by submitting a full set at
+    once; latency is submit -> outputs in the caller's array."""
+    import threading
+    rng = np.random.default_rng(0xB5A0000000000004)
+    pools = {1: synth.generate(1, 400_000), 2: synth.generate(2, 1_000_000)}
+    nreq = args.serve_requests
+    sizes = np.exp(rng.uniform(np.log(1000), np.log(64000), 3 * nreq + nreq)).astype(int)
+    kinds = rng.integers(1, 3, len(sizes))
+    reqs = []
+    for sz, kd in zip(sizes, kinds):
+        pairs, ref, qer, _ = pools[int(kd)]
+        a = int(rng.integers(0, len(pairs) - sz))
+        reqs.append(_slice_request(pairs, ref, qer, a, a + int(sz)))
+    log(f"[serve] {len(reqs)} requests, mean {sizes.mean():.0f} pairs")
+    srv = tal_bsw.Server(hbm_budget=1 << 30, workers=2, **PARAMS)
+
+    def run_set(sub, rate):
+        """Submits sub (open loop at `rate` requests/s, or all at once if None)
+        and returns the per-request records and the wall time."""
+        live = [r[0].copy() for r in sub]
+        tickets = [None] * len(sub)
+        done = threading.Event()
+        infos = [None] * len(sub)
+
+        def waiter():
+            for k in range(len(sub)):
+                while tickets[k] is None:
+                    time.sleep(0.0002)
+                infos[k] = srv.wait(tickets[k])
+            done.set()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        nxt = t0
+        for k, r in enumerate(sub):
+            if rate:
+                nxt += rng.exponential(1.0 / rate)
+                d = nxt - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+            tickets[k] = srv.submit(live[k], r[1], r[2], 100)
+        th.join()
+        return infos, time.perf_counter() - t0
+
+    warm = reqs[:20]
+    run_set(warm, None)
+    peak_set = reqs[:nreq]
+    infos, dt = run_set(peak_set, None)
+    npairs = sum(len(r[0]) for r in peak_set)
+    peak = npairs / dt
+    log(f"[serve] peak {peak:.3e} pairs/s over {len(peak_set)} requests ({dt:.2f} s)")
+    levels = []
+    for j, load in enumerate((0.5, 0.8, 0.95)):
+        sub = reqs[nreq * (j + 1):nreq * (j + 2)]
+        mean_pairs = np.mean([len(r[0]) for r in sub])
+        rate = load * peak / mean_pairs
+        infos, dt = run_set(sub, rate)
+        lat = np.array([i["latency_ms"] for i in infos])
+        bad = sum(i["status"] != 0 for i in infos)
+        lv = {"load": load, "requests": len(sub), "offered_req_per_s": round(rate, 2),
+              "achieved_pairs_per_s": round(sum(len(r[0]) for r in sub) / dt, 1),
+              "p50_ms": round(float(np.percentile(lat, 50)), 3),
+              "p99_ms": round(float(np.percentile(lat, 99)), 3),
+              "mean_batch_requests": round(float(np.mean([i["batch_requests"] for i in infos])), 2),
+              "failed": int(bad)}
+        log(f"[serve] {lv}")
+        levels.append(lv)
+    st = srv.stats()
+    srv.close()
+    line = {"metric": "p50/p99 request latency, streamed serving (continuous batching)",
+            "value": levels[-1]["p99_ms"], "unit": "ms (p99 at 95% load)", "n_gpus": 1,
+            "higher_is_better": False, "peak_pairs_per_s": round(peak, 1),
+            "levels": levels, "data": "synthetic (workloads 1 and 2 generators)",
+            "config": {"workload": "streamed serving", "request_pairs": "log-uniform 1k-64k",
+                       "arrivals": "Poisson", "hbm_budget": 1 << 30, "workers": 2, "w": 100},
+            "server": st}
+    print(json.dumps(line), flush=True)
 
 
 def _copy_calibration(mib):

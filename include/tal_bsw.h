@@ -41,6 +41,7 @@ extern "C" {
 #define TAL_ENOMEM 2  /* device or pinned-host allocation failed */
 #define TAL_EDEVICE 3 /* HIP runtime error; see tal_bsw_last_error */
 #define TAL_ERANGE 4  /* a pair exceeds the path's limits (see below) */
+#define TAL_EAGAIN 5  /* a request is not finished yet (serving loop) */
 
 /* SeqPair, bandedSWA.h:90-99.  idr/idq: byte offsets of the reference
  * (rows, len1) and query (columns, len2) in their buffers; h0: initial score;
@@ -142,6 +143,71 @@ typedef struct tal_bsw_stats {
 } tal_bsw_stats;
 
 int tal_bsw_get_stats(tal_bsw *h, tal_bsw_stats *out);
+
+/* ---------------------------------------------------------------------------
+ * Streamed serving.  No reference counterpart: TAL has no serving loop (its
+ * callers batch pairs themselves, benchmarks/bench-bsw.cpp:216-253); this is
+ * the continuous-batching front end of the same engine.
+ *
+ * Requests (a SeqPair array with its packed sequence buffers, as for
+ * tal_bsw_get_scores_auto) are queued by tal_bsw_server_submit and admitted in
+ * arrival order while their device footprint fits in `hbm_budget`.  Admitted
+ * requests are copied into device memory; every free worker then takes all
+ * admitted requests with the head's band width (up to max_batch_pairs pairs)
+ * and scores them as one batch on its own stream, so requests arriving while
+ * others are on the GPU join the next batch, and up to `workers` batches are
+ * on the GPU at once.  Each request is scored as one tal_bsw_get_scores_auto
+ * call on a freshly constructed handle: its groups never mix with other
+ * requests' pairs and its stale-column state starts from zero, so its outputs
+ * do not depend on what it was batched with.
+ */
+typedef struct tal_bsw_server tal_bsw_server;
+
+typedef struct tal_bsw_server_config {
+    size_t hbm_budget;       /* device bytes for admitted requests, < 2 GiB
+                                (sequence offsets are int32); 0 = 1 GiB */
+    int32_t workers;         /* batches on the GPU at once; 0 = 2 */
+    int32_t max_batch_pairs; /* pairs per batch at most; 0 = 4194304 */
+} tal_bsw_server_config;
+
+/* Per-request record filled by tal_bsw_server_wait. */
+typedef struct tal_bsw_request_info {
+    int32_t status;     /* TAL_OK or the request's error code */
+    int32_t batch;      /* batch the request ran in (-1 if it did not run) */
+    int32_t batch_requests; /* requests in that batch */
+    int32_t pad;
+    double queue_ms;    /* submit -> admitted (copied to the device) */
+    double latency_ms;  /* submit -> outputs written into the caller's pairs */
+} tal_bsw_request_info;
+
+typedef struct tal_bsw_server_stats {
+    int64_t requests;   /* completed requests */
+    int64_t pairs;      /* pairs of the completed requests */
+    int64_t batches;    /* batches run */
+    int64_t peak_bytes; /* peak device bytes held by admitted requests */
+    double busy_ms;     /* summed wall time of the batches */
+} tal_bsw_server_stats;
+
+int tal_bsw_server_create(const tal_bsw_params *params, int device,
+                          const tal_bsw_server_config *config, tal_bsw_server **out);
+
+/* Queues a request and returns at once with its ticket.  pairs/ref/qer are
+ * host memory that must stay valid until the request is waited for; the six
+ * outputs are written into pairs.  Thread-safe. */
+int tal_bsw_server_submit(tal_bsw_server *s, tal_seqpair *pairs, const uint8_t *ref,
+                          size_t ref_len, const uint8_t *qer, size_t qer_len,
+                          int32_t n, int32_t w, uint64_t *ticket);
+
+/* Waits up to timeout_ms (< 0: no limit) for a request.  Returns TAL_EAGAIN if
+ * it is not finished, else the request's status; info (may be null) receives
+ * its record.  A finished request is forgotten once waited for. */
+int tal_bsw_server_wait(tal_bsw_server *s, uint64_t ticket, int32_t timeout_ms,
+                        tal_bsw_request_info *info);
+
+int tal_bsw_server_get_stats(tal_bsw_server *s, tal_bsw_server_stats *out);
+
+/* Finishes every queued request, then releases the server. */
+void tal_bsw_server_destroy(tal_bsw_server *s);
 
 #ifdef __cplusplus
 }

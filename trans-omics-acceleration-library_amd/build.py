@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TAL_BSW_ARCH", "gfx950")
 
-HIP_SOURCES = ["bsw_kernels.hip", "tal_bsw.hip"]
+HIP_SOURCES = ["bsw_kernels.hip", "tal_bsw.hip", "bsw_server.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -34,7 +34,7 @@ def build_hip(out=None, extra=()):
         obj = os.path.join(CSRC, src.replace(".hip", ".o"))
         _run([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", out])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", out, "-lpthread"])
     for o in objs:
         os.remove(o)
     return out

@@ -58,6 +58,23 @@ struct GroupShape {
     int32_t nrow, ncol;
 };
 
+/* Host-side entry points shared by the C ABI (tal_bsw.hip) and the serving
+ * loop (bsw_server.hip); not exported from the library. */
+#define TAL_INTERNAL __attribute__((visibility("hidden")))
+
+/* One width-specific batch on device buffers of the handle's device, queued on
+ * stream s.  seg_first (device, one int per 16-bit group, may be null) splits
+ * the batch into independent calls, each scored as on a fresh object. */
+TAL_INTERNAL int run_batch(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref,
+                           const uint8_t *qer, int32_t n, int32_t w, hipStream_t s,
+                           const int32_t *seg_first);
+TAL_INTERNAL hipStream_t handle_stream(const tal_bsw *h);
+/* Largest score of the parameter set (match, -mismatch, ambiguity -1, 0): the
+ * 8/16-bit dispatcher's bound. */
+TAL_INTERNAL int score_bound(const tal_bsw_params &p);
+/* The dispatcher rule: 1 if the pair goes to the 8-bit path. */
+TAL_INTERNAL int goes_8bit(const tal_seqpair &p, int smax);
+
 } // namespace talbsw
 
 #endif

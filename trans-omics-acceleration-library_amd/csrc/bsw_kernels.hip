@@ -740,15 +740,20 @@ __global__ void bsw_shapes(const tal_seqpair *pairs, int32_t n, int32_t ngroups,
 __global__ void bsw_resolve(int32_t ngroups, const GroupShape *shape, const uint32_t *dep,
                             const uint8_t *wrote, const uint32_t *fmask,
                             const int64_t *fmask_off, const uint32_t *image,
-                            uint32_t *stale, int32_t *rerun_list, int32_t *rerun_count)
+                            const int32_t *seg_first, uint32_t *stale, int32_t *rerun_list,
+                            int32_t *rerun_count)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ngroups) return;
     const uint32_t d = dep[g];
     if (d == 0u) return;
     const int c = shape[g].ncol;
+    /* With segments (independent calls batched together), the search stops
+     * at the segment's first group and the image is the zero image of a
+     * fresh object. */
+    const int lo = seg_first ? seg_first[g] : 0;
     uint32_t truth = image[c];
-    for (int h = g - 1; h >= 0; h--) {
+    for (int h = g - 1; h >= lo; h--) {
         const int nh = shape[h].ncol;
         if (nh > c) {
             truth = fmask[fmask_off[h] + c];
@@ -887,13 +892,13 @@ hipError_t launch_shapes(int bits, const tal_seqpair *pairs, int32_t n, int32_t 
 
 hipError_t launch_resolve(int32_t ngroups, const GroupShape *shape, const uint32_t *dep,
                           const uint8_t *wrote, const uint32_t *fmask, const int64_t *fmask_off,
-                          const uint32_t *image, uint32_t *stale, int32_t *rerun_list,
-                          int32_t *rerun_count, hipStream_t s)
+                          const uint32_t *image, const int32_t *seg_first, uint32_t *stale,
+                          int32_t *rerun_list, int32_t *rerun_count, hipStream_t s)
 {
     const int tpb = 256, grid = (ngroups + tpb - 1) / tpb;
     if (grid == 0) return hipSuccess;
     hipLaunchKernelGGL(bsw_resolve, dim3(grid), dim3(tpb), 0, s, ngroups, shape, dep, wrote,
-                       fmask, fmask_off, image, stale, rerun_list, rerun_count);
+                       fmask, fmask_off, image, seg_first, stale, rerun_list, rerun_count);
     return hipGetLastError();
 }
 

@@ -1,0 +1,537 @@
+/*
+ * bsw_server.hip -- the streamed serving loop behind tal_bsw_server_* in
+ * include/tal_bsw.h: arrival-order admission under a device-memory budget,
+ * continuous batching over several worker streams, and per-request
+ * fresh-object semantics.
+ *
+ * Threads.  One admission thread moves requests from `pending` to `ready`:
+ * it waits for room in the arena, prepares the request's padded 8-bit and
+ * 16-bit SeqPair blocks on the host and copies them with the sequence bytes
+ * into the arena.  Each worker owns an engine handle (its own stream and
+ * scratch) and repeatedly takes every ready request with the head's band
+ * width, runs them as one batch, writes the outputs into the callers' arrays
+ * and frees their arena slots.
+ *
+ * Independence.  A request's 16-bit pairs are padded to a multiple of 32 and
+ * its 8-bit pairs to a multiple of 64 with explicit padding records (len1 0,
+ * h0 0, offsets 0 in the request's own buffers, len2 0 or the request's last
+ * 8-bit len2 -- the padding of one getScores call), so a batch's groups never
+ * span requests; the 16-bit stale-column search runs per request from the
+ * zero image (run_batch with seg_first).
+ */
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "bsw_internal.h"
+
+using namespace talbsw;
+
+namespace {
+
+constexpr size_t ALIGN = 256;
+constexpr size_t MAX_BUDGET = ((size_t)1 << 31) - ALIGN;
+
+size_t align_up(size_t x) { return (x + ALIGN - 1) & ~(ALIGN - 1); }
+
+double now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+/* First-fit allocator over [0, size) with coalescing frees. */
+struct Arena {
+    std::map<size_t, size_t> free_; /* offset -> length */
+    size_t used = 0, peak = 0;
+    void reset(size_t size)
+    {
+        free_.clear();
+        free_[0] = size;
+        used = peak = 0;
+    }
+    bool take(size_t bytes, size_t *off)
+    {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            *off = it->first;
+            const size_t rest = it->second - bytes;
+            const size_t at = it->first + bytes;
+            free_.erase(it);
+            if (rest) free_[at] = rest;
+            used += bytes;
+            peak = std::max(peak, used);
+            return true;
+        }
+        return false;
+    }
+    void give(size_t off, size_t bytes)
+    {
+        used -= bytes;
+        auto it = free_.emplace(off, bytes).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+struct Request {
+    uint64_t ticket = 0;
+    tal_seqpair *pairs = nullptr;
+    const uint8_t *ref = nullptr, *qer = nullptr;
+    size_t ref_len = 0, qer_len = 0;
+    int32_t n = 0, w = 0;
+    /* prepared */
+    std::vector<int32_t> idx8, idx16; /* caller positions of the 8/16-bit pairs */
+    int32_t n8p = 0, n16p = 0;        /* padded counts */
+    size_t off = 0, bytes = 0;        /* arena slot */
+    size_t ref_off = 0, qer_off = 0, p8_off = 0, p16_off = 0; /* absolute arena offsets */
+    bool admitted = false;
+    /* result */
+    int status = TAL_OK;
+    int32_t batch = -1, batch_requests = 0;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    bool done = false;
+};
+
+} // namespace
+
+struct tal_bsw_server {
+    tal_bsw_params prm{};
+    int device = 0;
+    int smax = 0;
+    size_t budget = 0;
+    int32_t max_batch = 0;
+    uint8_t *arena = nullptr;
+    Arena alloc;
+    hipStream_t copy_stream = nullptr;
+    std::vector<tal_bsw *> engines;
+    std::mutex mu;
+    std::condition_variable cv; /* every state change */
+    std::deque<Request *> pending, ready;
+    std::unordered_map<uint64_t, Request *> all;
+    uint64_t next_ticket = 1;
+    int32_t next_batch = 0;
+    int active_batches = 0;
+    bool admitting = false; /* the admission thread holds a request between queues */
+    bool stop = false;
+    std::thread admitter;
+    std::vector<std::thread> workers;
+    tal_bsw_server_stats st{};
+};
+
+namespace {
+
+/* Host-side checks of one request (the limits of the 16-bit path and the
+ * buffer extents); the dispatcher split and padded blocks. */
+int prepare(tal_bsw_server *s, Request *r)
+{
+    for (int32_t k = 0; k < r->n; k++) {
+        const tal_seqpair &p = r->pairs[k];
+        if (p.len1 < 0 || p.len2 < 0 || p.idr < 0 || p.idq < 0) return TAL_EINVAL;
+        if (p.len1 > MAX_LEN16 || p.len2 > MAX_LEN16) return TAL_ERANGE;
+        if ((size_t)p.idr + (size_t)p.len1 > r->ref_len ||
+            (size_t)p.idq + (size_t)p.len2 > r->qer_len)
+            return TAL_EINVAL;
+        (goes_8bit(p, s->smax) ? r->idx8 : r->idx16).push_back(k);
+    }
+    const int32_t n8 = (int32_t)r->idx8.size(), n16 = (int32_t)r->idx16.size();
+    r->n8p = (n8 + 63) / 64 * 64;
+    r->n16p = (n16 + 31) / 32 * 32;
+    const size_t seq = align_up(std::max<size_t>(r->ref_len, 1)) +
+                       align_up(std::max<size_t>(r->qer_len, 1));
+    r->bytes = seq + align_up(sizeof(tal_seqpair) * (size_t)r->n8p) +
+               align_up(sizeof(tal_seqpair) * (size_t)r->n16p);
+    return r->bytes > s->budget ? TAL_ERANGE : TAL_OK;
+}
+
+/* The padded blocks with offsets moved into the arena. */
+void build_block(const Request *r, const std::vector<int32_t> &idx, int32_t npad, bool eight,
+                 std::vector<tal_seqpair> &out)
+{
+    out.assign(npad, tal_seqpair{});
+    const int32_t m = (int32_t)idx.size();
+    for (int32_t t = 0; t < m; t++) {
+        tal_seqpair p = r->pairs[idx[t]];
+        p.idr = (int32_t)(r->ref_off + (size_t)p.idr);
+        p.idq = (int32_t)(r->qer_off + (size_t)p.idq);
+        out[t] = p;
+    }
+    for (int32_t t = m; t < npad; t++) { /* the padding of one getScores call */
+        tal_seqpair p{};
+        p.idr = (int32_t)r->ref_off;
+        p.idq = (int32_t)r->qer_off;
+        p.len2 = eight ? r->pairs[idx[m - 1]].len2 : 0;
+        out[t] = p;
+    }
+}
+
+void finish(tal_bsw_server *s, Request *r, int status)
+{
+    /* called with s->mu held */
+    r->status = status;
+    r->t_done = now_ms();
+    r->done = true;
+    if (r->admitted) s->alloc.give(r->off, r->bytes);
+    if (status == TAL_OK) {
+        s->st.requests++;
+        s->st.pairs += r->n;
+    }
+}
+
+void admit_loop(tal_bsw_server *s)
+{
+    (void)hipSetDevice(s->device);
+    std::vector<tal_seqpair> b8, b16;
+    for (;;) {
+        Request *r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv.wait(lk, [&] { return s->stop || !s->pending.empty(); });
+            if (s->pending.empty()) return; /* stop with nothing queued */
+            r = s->pending.front();
+            s->admitting = true;
+        }
+        int rc = prepare(s, r);
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            if (rc == TAL_OK) /* arrival order: wait for room for the head */
+                s->cv.wait(lk, [&] { return s->alloc.take(r->bytes, &r->off); });
+            s->pending.pop_front();
+            if (rc != TAL_OK) {
+                finish(s, r, rc);
+                s->admitting = false;
+                s->cv.notify_all();
+                continue;
+            }
+            r->admitted = true;
+        }
+        r->ref_off = r->off;
+        r->qer_off = r->ref_off + align_up(std::max<size_t>(r->ref_len, 1));
+        r->p8_off = r->qer_off + align_up(std::max<size_t>(r->qer_len, 1));
+        r->p16_off = r->p8_off + align_up(sizeof(tal_seqpair) * (size_t)r->n8p);
+        if (r->n8p) build_block(r, r->idx8, r->n8p, true, b8);
+        if (r->n16p) build_block(r, r->idx16, r->n16p, false, b16);
+        hipError_t e = hipSuccess;
+        hipStream_t cs = s->copy_stream;
+        if (r->ref_len)
+            e = hipMemcpyAsync(s->arena + r->ref_off, r->ref, r->ref_len, hipMemcpyHostToDevice, cs);
+        if (e == hipSuccess && r->qer_len)
+            e = hipMemcpyAsync(s->arena + r->qer_off, r->qer, r->qer_len, hipMemcpyHostToDevice, cs);
+        if (e == hipSuccess && r->n8p)
+            e = hipMemcpyAsync(s->arena + r->p8_off, b8.data(), sizeof(tal_seqpair) * r->n8p,
+                               hipMemcpyHostToDevice, cs);
+        if (e == hipSuccess && r->n16p)
+            e = hipMemcpyAsync(s->arena + r->p16_off, b16.data(), sizeof(tal_seqpair) * r->n16p,
+                               hipMemcpyHostToDevice, cs);
+        if (e == hipSuccess) e = hipStreamSynchronize(cs);
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (e != hipSuccess) {
+            finish(s, r, TAL_EDEVICE);
+        } else {
+            r->t_admit = now_ms();
+            s->ready.push_back(r);
+        }
+        s->admitting = false;
+        s->cv.notify_all();
+    }
+}
+
+struct WorkerBufs {
+    tal_seqpair *p8 = nullptr, *p16 = nullptr;
+    int32_t *seg = nullptr;
+    size_t cap8 = 0, cap16 = 0, capseg = 0;
+    std::vector<tal_seqpair> host;
+    std::vector<int32_t> hseg;
+    ~WorkerBufs()
+    {
+        if (p8) (void)hipFree(p8);
+        if (p16) (void)hipFree(p16);
+        if (seg) (void)hipFree(seg);
+    }
+    template <class T> static hipError_t grow(T **p, size_t *cap, size_t n)
+    {
+        if (n <= *cap) return hipSuccess;
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+        *cap = 0;
+        const size_t want = std::max(n, *cap * 3 / 2);
+        hipError_t e = hipMalloc((void **)p, sizeof(T) * want);
+        if (e == hipSuccess) *cap = want;
+        return e;
+    }
+};
+
+/* One batch: gather the requests' blocks, run both widths, write outputs. */
+int run_one(tal_bsw_server *s, tal_bsw *eng, WorkerBufs &wb, std::vector<Request *> &batch)
+{
+    hipStream_t st = handle_stream(eng);
+    const int32_t w = batch[0]->w;
+    int64_t n8 = 0, n16 = 0;
+    for (Request *r : batch) {
+        n8 += r->n8p;
+        n16 += r->n16p;
+    }
+    if (wb.grow(&wb.p8, &wb.cap8, (size_t)std::max<int64_t>(n8, 1)) != hipSuccess ||
+        wb.grow(&wb.p16, &wb.cap16, (size_t)std::max<int64_t>(n16, 1)) != hipSuccess ||
+        wb.grow(&wb.seg, &wb.capseg, (size_t)std::max<int64_t>(n16 / 32, 1)) != hipSuccess)
+        return TAL_ENOMEM;
+    wb.hseg.clear();
+    int64_t o8 = 0, o16 = 0;
+    for (Request *r : batch) {
+        if (r->n8p &&
+            hipMemcpyAsync(wb.p8 + o8, s->arena + r->p8_off, sizeof(tal_seqpair) * r->n8p,
+                           hipMemcpyDeviceToDevice, st) != hipSuccess)
+            return TAL_EDEVICE;
+        if (r->n16p &&
+            hipMemcpyAsync(wb.p16 + o16, s->arena + r->p16_off, sizeof(tal_seqpair) * r->n16p,
+                           hipMemcpyDeviceToDevice, st) != hipSuccess)
+            return TAL_EDEVICE;
+        const int32_t first = (int32_t)(o16 / 32);
+        for (int32_t g = 0; g < r->n16p / 32; g++) wb.hseg.push_back(first);
+        o8 += r->n8p;
+        o16 += r->n16p;
+    }
+    if (n16 && hipMemcpyAsync(wb.seg, wb.hseg.data(), sizeof(int32_t) * wb.hseg.size(),
+                              hipMemcpyHostToDevice, st) != hipSuccess)
+        return TAL_EDEVICE;
+    const uint8_t *base = s->arena;
+    if (n8 > INT32_MAX || n16 > INT32_MAX) return TAL_ERANGE;
+    int rc = TAL_OK;
+    if (n8) rc = run_batch(eng, 8, wb.p8, base, base, (int32_t)n8, w, st, nullptr);
+    if (rc == TAL_OK && n16) rc = run_batch(eng, 16, wb.p16, base, base, (int32_t)n16, w, st, wb.seg);
+    if (rc != TAL_OK) return rc;
+    /* outputs back into the callers' arrays */
+    wb.host.resize((size_t)(n8 + n16));
+    if (n8 && hipMemcpyAsync(wb.host.data(), wb.p8, sizeof(tal_seqpair) * n8,
+                             hipMemcpyDeviceToHost, st) != hipSuccess)
+        return TAL_EDEVICE;
+    if (n16 && hipMemcpyAsync(wb.host.data() + n8, wb.p16, sizeof(tal_seqpair) * n16,
+                              hipMemcpyDeviceToHost, st) != hipSuccess)
+        return TAL_EDEVICE;
+    if (hipStreamSynchronize(st) != hipSuccess) return TAL_EDEVICE;
+    o8 = 0;
+    o16 = n8;
+    for (Request *r : batch) {
+        auto put = [&](const std::vector<int32_t> &idx, int64_t o) {
+            for (size_t t = 0; t < idx.size(); t++) {
+                const tal_seqpair &q = wb.host[(size_t)o + t];
+                tal_seqpair &d = r->pairs[idx[t]];
+                d.score = q.score;
+                d.tle = q.tle;
+                d.gtle = q.gtle;
+                d.qle = q.qle;
+                d.gscore = q.gscore;
+                d.max_off = q.max_off;
+            }
+        };
+        put(r->idx8, o8);
+        put(r->idx16, o16);
+        o8 += r->n8p;
+        o16 += r->n16p;
+    }
+    return TAL_OK;
+}
+
+void work_loop(tal_bsw_server *s, int k)
+{
+    (void)hipSetDevice(s->device);
+    tal_bsw *eng = s->engines[k];
+    WorkerBufs wb;
+    for (;;) {
+        std::vector<Request *> batch;
+        int32_t id = 0;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv.wait(lk, [&] {
+                return !s->ready.empty() || (s->stop && s->pending.empty() && !s->admitting);
+            });
+            if (s->ready.empty()) return;
+            /* every ready request with the head's band width, in order */
+            const int32_t w = s->ready.front()->w;
+            int64_t total = 0;
+            for (auto it = s->ready.begin(); it != s->ready.end();) {
+                Request *r = *it;
+                if (r->w == w && (batch.empty() || total + r->n <= s->max_batch)) {
+                    total += r->n;
+                    batch.push_back(r);
+                    it = s->ready.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            id = s->next_batch++;
+            s->active_batches++;
+        }
+        const double t0 = now_ms();
+        const int rc = run_one(s, eng, wb, batch);
+        const double t1 = now_ms();
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->active_batches--;
+        s->st.batches++;
+        s->st.busy_ms += t1 - t0;
+        for (Request *r : batch) {
+            r->batch = id;
+            r->batch_requests = (int32_t)batch.size();
+            finish(s, r, rc);
+        }
+        s->st.peak_bytes = (int64_t)s->alloc.peak;
+        s->cv.notify_all();
+    }
+}
+
+} // namespace
+
+extern "C" {
+
+int tal_bsw_server_create(const tal_bsw_params *params, int device,
+                          const tal_bsw_server_config *config, tal_bsw_server **out)
+{
+    if (!params || !out) return TAL_EINVAL;
+    *out = nullptr;
+    tal_bsw_server_config cfg{};
+    if (config) cfg = *config;
+    if (cfg.hbm_budget == 0) cfg.hbm_budget = (size_t)1 << 30;
+    if (cfg.workers <= 0) cfg.workers = 2;
+    if (cfg.max_batch_pairs <= 0) cfg.max_batch_pairs = 1 << 22;
+    if (cfg.hbm_budget > MAX_BUDGET || cfg.workers > 8) return TAL_EINVAL;
+    tal_bsw_server *s = new (std::nothrow) tal_bsw_server();
+    if (!s) return TAL_ENOMEM;
+    s->prm = *params;
+    s->device = device;
+    s->smax = score_bound(*params);
+    s->budget = cfg.hbm_budget;
+    s->max_batch = cfg.max_batch_pairs;
+    int rc = TAL_OK;
+    for (int k = 0; k < cfg.workers && rc == TAL_OK; k++) {
+        tal_bsw *e = nullptr;
+        rc = tal_bsw_create(params, device, &e);
+        if (rc == TAL_OK) s->engines.push_back(e);
+    }
+    if (rc == TAL_OK && hipSetDevice(device) != hipSuccess) rc = TAL_EDEVICE;
+    if (rc == TAL_OK && hipMalloc((void **)&s->arena, s->budget) != hipSuccess) {
+        s->arena = nullptr;
+        rc = TAL_ENOMEM;
+    }
+    if (rc == TAL_OK && hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess)
+        rc = TAL_EDEVICE;
+    if (rc != TAL_OK) {
+        for (tal_bsw *e : s->engines) tal_bsw_destroy(e);
+        if (s->arena) (void)hipFree(s->arena);
+        delete s;
+        return rc;
+    }
+    s->alloc.reset(s->budget);
+    s->admitter = std::thread(admit_loop, s);
+    for (int k = 0; k < cfg.workers; k++) s->workers.emplace_back(work_loop, s, k);
+    *out = s;
+    return TAL_OK;
+}
+
+int tal_bsw_server_submit(tal_bsw_server *s, tal_seqpair *pairs, const uint8_t *ref,
+                          size_t ref_len, const uint8_t *qer, size_t qer_len, int32_t n, int32_t w,
+                          uint64_t *ticket)
+{
+    if (!s || !ticket || n <= 0 || w < 0 || !pairs) return TAL_EINVAL;
+    if ((!ref && ref_len) || (!qer && qer_len)) return TAL_EINVAL;
+    Request *r = new (std::nothrow) Request();
+    if (!r) return TAL_ENOMEM;
+    r->pairs = pairs;
+    r->ref = ref;
+    r->ref_len = ref_len;
+    r->qer = qer;
+    r->qer_len = qer_len;
+    r->n = n;
+    r->w = w;
+    r->t_submit = now_ms();
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->stop) {
+        delete r;
+        return TAL_EINVAL;
+    }
+    r->ticket = s->next_ticket++;
+    s->all[r->ticket] = r;
+    s->pending.push_back(r);
+    *ticket = r->ticket;
+    s->cv.notify_all();
+    return TAL_OK;
+}
+
+int tal_bsw_server_wait(tal_bsw_server *s, uint64_t ticket, int32_t timeout_ms,
+                        tal_bsw_request_info *info)
+{
+    if (!s) return TAL_EINVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end()) return TAL_EINVAL;
+    Request *r = it->second;
+    auto done = [&] { return r->done; };
+    if (timeout_ms < 0)
+        s->cv.wait(lk, done);
+    else if (!s->cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), done))
+        return TAL_EAGAIN;
+    if (info) {
+        info->status = r->status;
+        info->batch = r->batch;
+        info->batch_requests = r->batch_requests;
+        info->pad = 0;
+        info->queue_ms = r->admitted ? r->t_admit - r->t_submit : 0.0;
+        info->latency_ms = r->t_done - r->t_submit;
+    }
+    const int status = r->status;
+    s->all.erase(it);
+    delete r;
+    return status;
+}
+
+int tal_bsw_server_get_stats(tal_bsw_server *s, tal_bsw_server_stats *out)
+{
+    if (!s || !out) return TAL_EINVAL;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->st.peak_bytes = (int64_t)s->alloc.peak;
+    *out = s->st;
+    return TAL_OK;
+}
+
+void tal_bsw_server_destroy(tal_bsw_server *s)
+{
+    if (!s) return;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv.notify_all();
+    }
+    if (s->admitter.joinable()) s->admitter.join();
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv.notify_all();
+    }
+    for (std::thread &t : s->workers)
+        if (t.joinable()) t.join();
+    for (auto &kv : s->all) delete kv.second;
+    for (tal_bsw *e : s->engines) tal_bsw_destroy(e);
+    (void)hipSetDevice(s->device);
+    if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
+    if (s->arena) (void)hipFree(s->arena);
+    delete s;
+}
+
+} /* extern "C" */

@@ -27,8 +27,8 @@ hipError_t launch_shapes(int bits, const tal_seqpair *pairs, int32_t n, int32_t 
                          int32_t pad_len2, GroupShape *out, int64_t *ncol_plus1, hipStream_t s);
 hipError_t launch_resolve(int32_t ngroups, const GroupShape *shape, const uint32_t *dep,
                           const uint8_t *wrote, const uint32_t *fmask, const int64_t *fmask_off,
-                          const uint32_t *image, uint32_t *stale, int32_t *rerun_list,
-                          int32_t *rerun_count, hipStream_t s);
+                          const uint32_t *image, const int32_t *seg_first, uint32_t *stale,
+                          int32_t *rerun_list, int32_t *rerun_count, hipStream_t s);
 hipError_t launch_image_update(int32_t ngroups, int32_t ncols, const GroupShape *shape,
                                const uint8_t *wrote, int32_t *last, const uint32_t *fmask,
                                const int64_t *fmask_off, uint32_t *image, hipStream_t s);
@@ -93,6 +93,7 @@ struct tal_bsw {
     int waves_per_cu = 8;  /* target waves per CU for the LDS/global split */
     int order = 1;         /* task order: 0 = input order, 1 = longest groups first */
     DevBuf image; /* IMAGE_COLS u32: F[c] nonzero lanes of the 16-bit scratch */
+    DevBuf zero_image; /* the image of a freshly constructed object */
     PathBufs b16, b8;
     /* dispatcher scratch */
     DevBuf flag, idx_all, idx_sel, nsel, pairs8, pairs16, cubtmp;
@@ -265,8 +266,14 @@ int grid_for(const tal_bsw *h, bool lds, size_t lds_bytes, int ntasks)
  * resolution loop and updates the handle's image afterwards.
  */
 int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const uint8_t *qer,
-             int32_t n, int32_t w, int32_t last_len2, hipStream_t s)
+             int32_t n, int32_t w, int32_t last_len2, hipStream_t s,
+             const int32_t *seg_first = nullptr)
 {
+    /* seg_first (device, per 16-bit group: the first group of its segment)
+     * marks a batch of independent calls, each scored as on a freshly
+     * constructed object: the stale-column search stays inside a segment,
+     * starts from the zero image, and the handle's image is left alone. */
+    const bool fresh = seg_first != nullptr;
     if (n == 0) return TAL_OK;
     PathBufs &pb = bits == 16 ? h->b16 : h->b8;
     const int GL = bits == 16 ? 32 : 64, GPW = 64 / GL;
@@ -404,8 +411,8 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
         for (int iter = 0;; iter++) {
             HIPCHK(h, hipMemsetAsync(cnt, 0, sizeof(int32_t), s), "clear rerun count");
             HIPCHK(h, launch_resolve(G, shapes, B.dep_out, B.wrote_out, B.fmask, B.fmask_off,
-                                     h->image.as<uint32_t>(), pb.stale.as<uint32_t>(), list, cnt,
-                                     s),
+                                     fresh ? h->zero_image.as<uint32_t>() : h->image.as<uint32_t>(),
+                                     seg_first, pb.stale.as<uint32_t>(), list, cnt, s),
                    "resolve");
             int32_t c = 0;
             HIPCHK(h, hipMemcpyAsync(&c, cnt, sizeof c, hipMemcpyDeviceToHost, s), "read reruns");
@@ -438,11 +445,15 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
         h->st.groups16 = G;
         h->st.reruns16 = reruns;
         h->st.lds_state16 = slab_cols == qwords ? 1 : 0;
-        HIPCHK(h, pb.last.ensure(sizeof(int32_t) * IMAGE_COLS), "alloc last");
-        HIPCHK(h, hipMemsetAsync(pb.last.p, 0xFF, sizeof(int32_t) * IMAGE_COLS, s), "clear last");
-        HIPCHK(h, launch_image_update(G, max_ncol + 1, shapes, B.wrote_out, pb.last.as<int32_t>(),
-                                      B.fmask, B.fmask_off, h->image.as<uint32_t>(), s),
-               "image update");
+        if (!fresh) {
+            HIPCHK(h, pb.last.ensure(sizeof(int32_t) * IMAGE_COLS), "alloc last");
+            HIPCHK(h, hipMemsetAsync(pb.last.p, 0xFF, sizeof(int32_t) * IMAGE_COLS, s),
+                   "clear last");
+            HIPCHK(h, launch_image_update(G, max_ncol + 1, shapes, B.wrote_out,
+                                          pb.last.as<int32_t>(), B.fmask, B.fmask_off,
+                                          h->image.as<uint32_t>(), s),
+                   "image update");
+        }
     } else {
         h->ev8 = true;
         h->st.groups8 = G;
@@ -525,6 +536,32 @@ int host_call(tal_bsw *h, int mode, tal_seqpair *pairs, const uint8_t *ref, size
 
 } // namespace
 
+namespace talbsw {
+
+int run_batch(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const uint8_t *qer,
+              int32_t n, int32_t w, hipStream_t s, const int32_t *seg_first)
+{
+    if (!h) return TAL_EINVAL;
+    if (n < 0 || w < 0 || (n > 0 && !pairs)) return fail(h, TAL_EINVAL, "bad batch arguments");
+    h->st = tal_bsw_stats{};
+    h->ev16 = h->ev8 = false;
+    HIPCHK(h, hipSetDevice(h->device), "set device");
+    return run_path(h, bits, pairs, ref, qer, n, w, 0, s, seg_first);
+}
+
+hipStream_t handle_stream(const tal_bsw *h) { return h->stream; }
+
+int score_bound(const tal_bsw_params &p) { return smax_of(p); }
+
+int goes_8bit(const tal_seqpair &p, int smax)
+{
+    const int mn = std::min(p.len1, p.len2);
+    return p.len1 >= 0 && p.len2 >= 0 && p.len1 <= MAX_LEN8 && p.len2 <= MAX_LEN8 && p.h0 >= 0 &&
+           (long long)p.h0 + (long long)smax * mn <= 127;
+}
+
+} // namespace talbsw
+
 extern "C" {
 
 int tal_bsw_create(const tal_bsw_params *params, int device, tal_bsw **out)
@@ -560,7 +597,13 @@ int tal_bsw_create(const tal_bsw_params *params, int device, tal_bsw **out)
         if (!evok) { rc = TAL_EDEVICE; break; }
         hipError_t e = h->image.ensure(sizeof(uint32_t) * IMAGE_COLS);
         if (e != hipSuccess) { rc = TAL_ENOMEM; break; }
-        if (hipMemset(h->image.p, 0, sizeof(uint32_t) * IMAGE_COLS) != hipSuccess) { rc = TAL_EDEVICE; break; }
+        e = h->zero_image.ensure(sizeof(uint32_t) * IMAGE_COLS);
+        if (e != hipSuccess) { rc = TAL_ENOMEM; break; }
+        if (hipMemset(h->image.p, 0, sizeof(uint32_t) * IMAGE_COLS) != hipSuccess ||
+            hipMemset(h->zero_image.p, 0, sizeof(uint32_t) * IMAGE_COLS) != hipSuccess) {
+            rc = TAL_EDEVICE;
+            break;
+        }
     } while (0);
     if (rc) {
         tal_bsw_destroy(h);
@@ -575,7 +618,7 @@ void tal_bsw_destroy(tal_bsw *h)
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    DevBuf *all[] = {&h->image,   &h->flag,    &h->idx_all, &h->idx_sel, &h->nsel,
+    DevBuf *all[] = {&h->image,   &h->zero_image, &h->flag, &h->idx_all, &h->idx_sel, &h->nsel,
                      &h->pairs8,  &h->pairs16, &h->cubtmp,  &h->d_pairs, &h->d_ref,
                      &h->d_qer};
     for (DevBuf *b : all) b->release();
@@ -736,13 +779,7 @@ int32_t tal_bsw_count_8bit(const tal_bsw *h, const tal_seqpair *pairs, int32_t n
     if (!h || (!pairs && n > 0)) return -1;
     const int sm = smax_of(h->prm);
     int32_t c = 0;
-    for (int32_t k = 0; k < n; k++) {
-        const tal_seqpair &p = pairs[k];
-        const int mn = std::min(p.len1, p.len2);
-        if (p.len1 >= 0 && p.len2 >= 0 && p.len1 <= MAX_LEN8 && p.len2 <= MAX_LEN8 && p.h0 >= 0 &&
-            (long long)p.h0 + (long long)sm * mn <= 127)
-            c++;
-    }
+    for (int32_t k = 0; k < n; k++) c += goes_8bit(pairs[k], sm);
     return c;
 }
 

@@ -19,14 +19,16 @@ SEQPAIR_FIELDS = ["idr", "idq", "id", "len1", "len2", "h0", "seqid", "regid",
 SEQPAIR_DTYPE = np.dtype([(f, "<i4") for f in SEQPAIR_FIELDS])
 OUT_FIELDS = ["score", "tle", "gtle", "qle", "gscore", "max_off"]
 
-TAL_OK, TAL_EINVAL, TAL_ENOMEM, TAL_EDEVICE, TAL_ERANGE = 0, 1, 2, 3, 4
-_STATUS = {1: "TAL_EINVAL", 2: "TAL_ENOMEM", 3: "TAL_EDEVICE", 4: "TAL_ERANGE"}
+TAL_OK, TAL_EINVAL, TAL_ENOMEM, TAL_EDEVICE, TAL_ERANGE, TAL_EAGAIN = 0, 1, 2, 3, 4, 5
+_STATUS = {1: "TAL_EINVAL", 2: "TAL_ENOMEM", 3: "TAL_EDEVICE", 4: "TAL_ERANGE", 5: "TAL_EAGAIN"}
 
 # every C-ABI entry point of include/tal_bsw.h
 EXPORTS = ["tal_bsw_create", "tal_bsw_destroy", "tal_bsw_last_error",
            "tal_bsw_get_scores16", "tal_bsw_get_scores8", "tal_bsw_get_scores_auto",
            "tal_bsw_get_scores16_device", "tal_bsw_get_scores8_device",
-           "tal_bsw_get_scores_auto_device", "tal_bsw_count_8bit", "tal_bsw_get_stats"]
+           "tal_bsw_get_scores_auto_device", "tal_bsw_count_8bit", "tal_bsw_get_stats",
+           "tal_bsw_server_create", "tal_bsw_server_submit", "tal_bsw_server_wait",
+           "tal_bsw_server_get_stats", "tal_bsw_server_destroy"]
 
 
 class Params(ctypes.Structure):
@@ -39,6 +41,23 @@ class Stats(ctypes.Structure):
                 ("ms_rerun16", ctypes.c_float)] + [
         (n, ctypes.c_int32) for n in ("groups16", "groups8", "reruns16", "lds_state16",
                                       "pairs8", "pairs16")]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_size_t), ("workers", ctypes.c_int32),
+                ("max_batch_pairs", ctypes.c_int32)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("batch", ctypes.c_int32),
+                ("batch_requests", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("queue_ms", ctypes.c_double), ("latency_ms", ctypes.c_double)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("requests", ctypes.c_int64), ("pairs", ctypes.c_int64),
+                ("batches", ctypes.c_int64), ("peak_bytes", ctypes.c_int64),
+                ("busy_ms", ctypes.c_double)]
 
 
 class TalError(RuntimeError):
@@ -76,6 +95,17 @@ def load():
         L.tal_bsw_count_8bit.argtypes = [P, P, i32]
         L.tal_bsw_get_stats.restype = ctypes.c_int
         L.tal_bsw_get_stats.argtypes = [P, ctypes.POINTER(Stats)]
+        L.tal_bsw_server_create.restype = ctypes.c_int
+        L.tal_bsw_server_create.argtypes = [ctypes.POINTER(Params), ctypes.c_int,
+                                            ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]
+        L.tal_bsw_server_submit.restype = ctypes.c_int
+        L.tal_bsw_server_submit.argtypes = [P, P, P, sz, P, sz, i32, i32,
+                                            ctypes.POINTER(ctypes.c_uint64)]
+        L.tal_bsw_server_wait.restype = ctypes.c_int
+        L.tal_bsw_server_wait.argtypes = [P, ctypes.c_uint64, i32, ctypes.POINTER(RequestInfo)]
+        L.tal_bsw_server_get_stats.restype = ctypes.c_int
+        L.tal_bsw_server_get_stats.argtypes = [P, ctypes.POINTER(ServerStats)]
+        L.tal_bsw_server_destroy.argtypes = [P]
         _lib = L
     return _lib
 
@@ -140,6 +170,65 @@ class BandedPairWiseSW:
 
     def count_8bit(self, pairs):
         return int(load().tal_bsw_count_8bit(self._h, _ptr(pairs), len(pairs)))
+
+
+class Server:
+    """Streamed serving with continuous batching (tal_bsw_server_* of
+    include/tal_bsw.h).  Each request is scored as one dispatcher call on a
+    fresh engine object, whatever it is batched with."""
+
+    def __init__(self, hbm_budget=0, workers=0, max_batch_pairs=0, device=0, o_del=6, e_del=1,
+                 o_ins=6, e_ins=1, zdrop=100, end_bonus=5, w_match=1, w_mismatch=4):
+        L = load()
+        self.params = Params(o_del, e_del, o_ins, e_ins, zdrop, end_bonus, w_match, w_mismatch)
+        cfg = ServerConfig(int(hbm_budget), int(workers), int(max_batch_pairs))
+        h = ctypes.c_void_p()
+        rc = L.tal_bsw_server_create(ctypes.byref(self.params), int(device), ctypes.byref(cfg),
+                                     ctypes.byref(h))
+        if rc != TAL_OK:
+            raise TalError(rc, "tal_bsw_server_create failed")
+        self._h = h
+        self._live = {}
+
+    def submit(self, pairs, ref, qer, w):
+        """Queues a request; the arrays are kept alive until it is waited for."""
+        assert pairs.dtype == SEQPAIR_DTYPE and pairs.flags.c_contiguous
+        ref = np.ascontiguousarray(ref, dtype=np.uint8)
+        qer = np.ascontiguousarray(qer, dtype=np.uint8)
+        t = ctypes.c_uint64()
+        rc = load().tal_bsw_server_submit(self._h, _ptr(pairs), _ptr(ref), ref.nbytes, _ptr(qer),
+                                          qer.nbytes, len(pairs), int(w), ctypes.byref(t))
+        if rc != TAL_OK:
+            raise TalError(rc, "submit failed")
+        self._live[t.value] = (pairs, ref, qer)
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1):
+        """Returns the request's record (status, batch, batch_requests,
+        queue_ms, latency_ms), or None if it is not finished in time."""
+        info = RequestInfo()
+        rc = load().tal_bsw_server_wait(self._h, ctypes.c_uint64(ticket), int(timeout_ms),
+                                        ctypes.byref(info))
+        if rc == TAL_EAGAIN:
+            return None
+        self._live.pop(ticket, None)
+        return {f: getattr(info, f) for f, _ in RequestInfo._fields_ if f != "pad"}
+
+    def stats(self):
+        st = ServerStats()
+        rc = load().tal_bsw_server_get_stats(self._h, ctypes.byref(st))
+        if rc != TAL_OK:
+            raise TalError(rc, "stats failed")
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().tal_bsw_server_destroy(self._h)
+            self._h = None
+            self._live.clear()
+
+    def __del__(self):
+        self.close()
 
 
 def _smax(w_match, w_mismatch):
