@@ -52,6 +52,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-naive", action="store_true")
+    ap.add_argument("--eager", action="store_true",
+                    help="time the eager dispatcher instead of the graph-captured plan")
+    ap.add_argument("--tune-pairs", type=int, default=262_144,
+                    help="pairs the autotuner times its candidates on (0 = no tuning)")
     ap.add_argument("--bucket", type=int, default=20_000,
                     help="pairs per launch of the naive form (bench-bsw's chunk size)")
     ap.add_argument("--serve", action="store_true",
@@ -90,10 +94,27 @@ def main():
     stream = torch.cuda.current_stream(dev)
     eng = tal_bsw.BandedPairWiseSW(**PARAMS)
     n = len(pairs)
+    ptrs = (d_pairs.data_ptr(), d_ref.data_ptr(), d_qer.data_ptr())
+    tuning = None
+    if args.tune_pairs > 0:
+        t = time.time()
+        tuning = eng.autotune(*ptrs, n, w, sample=args.tune_pairs)
+        log(f"[bench] autotune on {min(n, args.tune_pairs)} pairs in {time.time() - t:.1f}s: "
+            f"waves/CU {tuning['waves_per_cu']}, order {tuning['order']}; "
+            + ", ".join(f"{x['waves_per_cu']}/{x['order']}:{x['ms']:.2f}" for x in tuning["trials"]))
 
-    def step():
-        eng.get_scores_device(0, d_pairs.data_ptr(), d_ref.data_ptr(), d_qer.data_ptr(), n, w,
-                              stream.cuda_stream)
+    def eager_step():
+        eng.get_scores_device(0, *ptrs, n, w, stream.cuda_stream)
+
+    if args.eager:
+        step = eager_step
+    else:
+        t = time.time()
+        plan = eng.plan(*ptrs, n, w)
+        log(f"[bench] plan (hipGraph of the dispatcher pipeline) built in {time.time() - t:.2f}s")
+
+        def step():
+            plan.launch(stream.cuda_stream)
 
     # first call on a fresh handle doubles as the parity check of a prefix
     step()
@@ -123,6 +144,23 @@ def main():
     st = eng.stats()
     log(f"[bench] {ms_step:.1f} ms/step, {gcups:.1f} GCUPS; main16 {np.mean(ms16):.1f} ms, "
         f"main8 {np.mean(ms8):.1f} ms, stats {st}")
+
+    eager_ms = None
+    if not args.eager:
+        # the same steps through the eager dispatcher, whose per-kernel HIP
+        # events (recorded on the kernels' stream) give the 16-bit kernel time
+        ms16, ms8 = [], []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(max(1, args.steps // 2)):
+            eager_step()
+            st = eng.stats()
+            ms16.append(st["ms_main16"])
+            ms8.append(st["ms_main8"])
+        torch.cuda.synchronize()
+        eager_ms = (time.perf_counter() - t0) * 1e3 / max(1, args.steps // 2)
+        log(f"[bench] eager dispatcher (no graph): {eager_ms:.1f} ms/step; main16 "
+            f"{np.mean(ms16):.1f} ms, main8 {np.mean(ms8):.1f} ms")
 
     cells16 = float(cells[~is8].sum())
     k16_ms = float(np.mean(ms16))
@@ -166,6 +204,9 @@ def main():
                    "pairs_8bit": int(is8.sum()), "band_cells": total_cells},
         "pairs_per_s": round(n / (ms_step / 1e3), 1),
         "parity": check,
+        "pipeline": "eager dispatcher" if args.eager else "hipGraph plan replay",
+        "eager_ms_per_step": round(eager_ms, 3) if eager_ms else None,
+        "tuning": {k: tuning[k] for k in ("waves_per_cu", "order", "ms")} if tuning else None,
         "naive_gpu": naive,
         "plain_gpu": plain,
         "roofline": roofline,

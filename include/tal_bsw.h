@@ -140,9 +140,63 @@ typedef struct tal_bsw_stats {
     int32_t lds_state16; /* 1 if the 16-bit state lived in LDS */
     int32_t pairs8;     /* pairs sent to the 8-bit path */
     int32_t pairs16;    /* pairs sent to the 16-bit path */
+    float ms_replay;    /* whole graph replay of the last plan launch (the
+                           per-kernel fields stay 0 for plan launches) */
 } tal_bsw_stats;
 
 int tal_bsw_get_stats(tal_bsw *h, tal_bsw_stats *out);
+
+/* ---------------------------------------------------------------------------
+ * Replayable batch plans (no reference counterpart: the reference has no
+ * device or launch layer).  A plan binds a device batch (pairs/ref/qer/n/w,
+ * as for tal_bsw_get_scores_auto_device) to its handle, sizes every buffer
+ * from the batch as it is at creation (one synchronous pass), and captures the
+ * whole dispatcher pipeline -- the 8/16-bit split, both DP launches, the
+ * stale-column resolution with two rerun rounds, the update of the handle's
+ * stale-column state and the result scatters -- in one hipGraph.
+ *
+ * tal_bsw_plan_launch replays it on `stream` (NULL = the handle's stream) and
+ * scores the batch's current contents exactly as tal_bsw_get_scores_auto_device
+ * would; it waits for the replay and reads one status word, and finishes any
+ * rerun the graph's rounds left.  The pair records may change between launches
+ * as long as the 8/16-bit split keeps its size and no 16-bit group gets wider
+ * than at creation; otherwise the launch returns TAL_ERANGE (the outputs are
+ * then unspecified, the handle's stale-column state is untouched) and a new
+ * plan is needed.  While a plan exists its handle may be used for other calls;
+ * the plan owns its own buffers.
+ */
+typedef struct tal_bsw_plan tal_bsw_plan;
+
+int tal_bsw_plan_create(tal_bsw *h, tal_seqpair *pairs, const uint8_t *ref,
+                        const uint8_t *qer, int32_t n, int32_t w, tal_bsw_plan **out);
+int tal_bsw_plan_launch(tal_bsw_plan *plan, void *stream);
+void tal_bsw_plan_destroy(tal_bsw_plan *plan);
+
+/* Autotuner (no reference counterpart).  Times candidate launch
+ * configurations -- waves per CU, which sets how many DP state columns each
+ * wave keeps in LDS (the LDS tile width), and the task order (input order, or
+ * longest groups first with similar groups paired in a wave) -- by replaying
+ * a plan of the batch's first `sample` pairs (0 = all) on a scratch handle
+ * and a scratch copy of the pairs, so neither the batch nor h's stale-column
+ * state changes; then applies the fastest to h.  Results are the same under
+ * every configuration; only speed differs. */
+#define TAL_TUNING_MAX_TRIALS 16
+typedef struct tal_bsw_tuning {
+    int32_t waves_per_cu; /* chosen */
+    int32_t order;        /* chosen: 0 input order, 1 longest first */
+    float ms;             /* replay time of the chosen configuration */
+    int32_t trials;
+    int32_t trial_waves[TAL_TUNING_MAX_TRIALS];
+    int32_t trial_order[TAL_TUNING_MAX_TRIALS];
+    float trial_ms[TAL_TUNING_MAX_TRIALS];
+} tal_bsw_tuning;
+
+int tal_bsw_autotune(tal_bsw *h, const tal_seqpair *pairs, const uint8_t *ref,
+                     const uint8_t *qer, int32_t n, int32_t w, int32_t sample,
+                     tal_bsw_tuning *out);
+
+/* Applies a configuration (e.g. a cached autotuner result). */
+int tal_bsw_set_tuning(tal_bsw *h, int32_t waves_per_cu, int32_t order);
 
 /* ---------------------------------------------------------------------------
  * Streamed serving.  No reference counterpart: TAL has no serving loop (its

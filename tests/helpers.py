@@ -18,3 +18,40 @@ def assert_same_outputs(got, want, label=""):
         rows = [(int(i), [int(got[f][i]) for f in OUT_FIELDS], [int(want[f][i]) for f in OUT_FIELDS])
                 for i in k]
         raise AssertionError(f"{label}: {int(d.sum())} of {len(d)} pairs differ; first: {rows}")
+
+
+class DeviceArrays:
+    """numpy arrays copied into device memory through libamdhip64 (tests of
+    the device-pointer entry points)."""
+
+    def __init__(self, *arrays):
+        import ctypes
+        self._ct = ctypes
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                       ctypes.c_int]
+        self.hip.hipFree.argtypes = [ctypes.c_void_p]
+        self.hip.hipDeviceSynchronize.argtypes = []
+        self.ptrs = []
+        for a in arrays:
+            d = ctypes.c_void_p()
+            assert self.hip.hipMalloc(ctypes.byref(d), max(a.nbytes, 1)) == 0
+            assert self.hip.hipMemcpy(d, a.ctypes.data, a.nbytes, 1) == 0
+            self.ptrs.append(d)
+
+    def ptr(self, k):
+        return self.ptrs[k].value
+
+    def upload(self, k, a):
+        assert self.hip.hipMemcpy(self.ptrs[k], a.ctypes.data, a.nbytes, 1) == 0
+
+    def download(self, k, a):
+        assert self.hip.hipDeviceSynchronize() == 0
+        assert self.hip.hipMemcpy(a.ctypes.data, self.ptrs[k], a.nbytes, 2) == 0
+        return a
+
+    def close(self):
+        for d in self.ptrs:
+            self.hip.hipFree(d)
+        self.ptrs = []

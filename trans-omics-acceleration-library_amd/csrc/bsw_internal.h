@@ -51,6 +51,11 @@ struct KBuffers {
     uint8_t *wrote_out;       /* per group: F[ncol] stored during the group */
     uint32_t *fmask;          /* per group final F nonzero masks, c < ncol */
     const int64_t *fmask_off; /* per group offset into fmask */
+    /* graph-replayed batches: values that live in device memory (null =
+     * use the KParams field) */
+    const int32_t *ntasks_dev;    /* tasks of this launch */
+    const int32_t *last_len2_dev; /* 8-bit padding length */
+    const int32_t *abort_flag;    /* nonzero: the batch left its plan; do nothing */
 };
 
 /* Per-group shape: rows (max len1) and columns (max len2) of the group. */

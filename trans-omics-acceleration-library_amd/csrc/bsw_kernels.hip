@@ -194,7 +194,7 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
         len2 = sp->len2;
         h0 = sp->h0;
     } else if (gvalid && BITS == 8) {
-        len2 = Kref.last_len2; /* :2017 */
+        len2 = B.last_len2_dev ? *B.last_len2_dev : Kref.last_len2; /* :2017 */
     }
     const int nrow = group_max<GL>(len1);
     const int ncol = group_max<GL>(len2);
@@ -692,11 +692,13 @@ __global__ __launch_bounds__(64) void bsw_main(KParams K, KBuffers B)
     StateRef S{(lds_u32 *)(lds_state + lane), (glb_u32 *)(slab + (size_t)K.query_words * 64),
                K.lds_cols};
     uint32_t *Q = slab;
+    if (B.abort_flag && *B.abort_flag != 0) return;
+    const int ntasks = B.ntasks_dev ? *B.ntasks_dev : K.ntasks;
     for (;;) {
         int t = 0;
         if (lane == 0) t = atomicAdd(B.task_counter, 1);
         t = __shfl(t, 0, 64);
-        if (t >= K.ntasks) break;
+        if (t >= ntasks) break;
         int g;
         if (B.task_groups)
             g = B.task_groups[t * Wd::GPW + half];
@@ -711,10 +713,12 @@ __global__ __launch_bounds__(64) void bsw_main(KParams K, KBuffers B)
 /* Group shapes: rows/columns of each group and the 8-bit padding length. */
 template <int GL>
 __global__ void bsw_shapes(const tal_seqpair *pairs, int32_t n, int32_t ngroups,
-                           int32_t pad_len2, GroupShape *out, int64_t *ncol_plus1)
+                           int32_t pad_len2, const int32_t *pad_len2_dev, GroupShape *out,
+                           int64_t *ncol_plus1)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= ngroups) return;
+    if (pad_len2_dev) pad_len2 = *pad_len2_dev;
     int nr = 0, nc = 0;
     for (int l = 0; l < GL; l++) {
         const int p = g * GL + l;
@@ -741,10 +745,10 @@ __global__ void bsw_resolve(int32_t ngroups, const GroupShape *shape, const uint
                             const uint8_t *wrote, const uint32_t *fmask,
                             const int64_t *fmask_off, const uint32_t *image,
                             const int32_t *seg_first, uint32_t *stale, int32_t *rerun_list,
-                            int32_t *rerun_count)
+                            int32_t *rerun_count, const int32_t *abort_flag)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ngroups) return;
+    if (g >= ngroups || (abort_flag && *abort_flag != 0)) return;
     const uint32_t d = dep[g];
     if (d == 0u) return;
     const int c = shape[g].ncol;
@@ -773,10 +777,10 @@ __global__ void bsw_resolve(int32_t ngroups, const GroupShape *shape, const uint
 
 /* Highest column each group stores: ncol (if wrote) or ncol-1. */
 __global__ void bsw_last_writer_scatter(int32_t ngroups, const GroupShape *shape,
-                                        const uint8_t *wrote, int32_t *last)
+                                        const uint8_t *wrote, int32_t *last, const int32_t *skip)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ngroups) return;
+    if (g >= ngroups || (skip && *skip != 0)) return;
     const int top = wrote[g] ? shape[g].ncol : shape[g].ncol - 1;
     if (top >= 0) atomicMax(last + top, g);
 }
@@ -785,8 +789,10 @@ __global__ void bsw_last_writer_scatter(int32_t ngroups, const GroupShape *shape
  * writer into the handle's image of the F scratch zero-masks. */
 __global__ __launch_bounds__(1024) void bsw_image_update(int32_t ncols, const GroupShape *shape,
                                                          const int32_t *last, const uint32_t *fmask,
-                                                         const int64_t *fmask_off, uint32_t *image)
+                                                         const int64_t *fmask_off, uint32_t *image,
+                                                         const int32_t *skip)
 {
+    if (skip && *skip != 0) return; /* reruns still pending: the caller updates later */
     __shared__ int32_t carry;
     if (threadIdx.x == 0) carry = -1;
     __syncthreads();
@@ -854,6 +860,103 @@ __global__ void bsw_scatter_out(const tal_seqpair *src, const int32_t *idx, int3
     d->max_off = s.max_off;
 }
 
+/* ------------------------------------------- graph-replayed batch helpers */
+
+/* Stable 8/16-bit partition without host reads, for captured graphs:
+ * part_count counts the 8-bit pairs of each 256-pair block, part_offsets
+ * turns the counts into exclusive offsets (one block) and checks the total
+ * against the planned split, part_scatter writes each pair's index to its
+ * side in input order. */
+__device__ __forceinline__ bool flag8_of(const tal_seqpair &p, int match_pos)
+{
+    const int mn = min(p.len1, p.len2);
+    return p.len1 >= 0 && p.len2 >= 0 && p.len1 <= MAX_LEN8 && p.len2 <= MAX_LEN8 &&
+           p.h0 >= 0 && (long long)p.h0 + (long long)match_pos * mn <= 127;
+}
+
+__global__ __launch_bounds__(256) void part_count(const tal_seqpair *pairs, int32_t n,
+                                                  int32_t match_pos, int32_t *blk)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    const bool f = k < n && flag8_of(pairs[k], match_pos);
+    const int c = __popcll(__ballot(f));
+    __shared__ int32_t wsum[4];
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) blk[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+__global__ __launch_bounds__(1024) void part_offsets(int32_t *blk, int32_t nblk, int32_t planned8,
+                                                     int32_t *err)
+{
+    __shared__ int32_t buf[1024];
+    __shared__ int32_t carry;
+    if (threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for (int base = 0; base < nblk; base += 1024) {
+        const int k = base + (int)threadIdx.x;
+        const int v = k < nblk ? blk[k] : 0;
+        buf[threadIdx.x] = v;
+        __syncthreads();
+        for (int o = 1; o < 1024; o <<= 1) {
+            const int t = threadIdx.x >= (unsigned)o ? buf[threadIdx.x - o] : 0;
+            __syncthreads();
+            buf[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (k < nblk) blk[k] = carry + buf[threadIdx.x] - v; /* exclusive */
+        __syncthreads();
+        if (threadIdx.x == 1023) carry += buf[1023];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && carry != planned8) atomicOr(err, 1);
+}
+
+__global__ __launch_bounds__(256) void part_scatter(const tal_seqpair *pairs, int32_t n,
+                                                    int32_t match_pos, const int32_t *blk,
+                                                    int32_t n8, int32_t *sel8, int32_t *sel16)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    const bool f = k < n && flag8_of(pairs[k], match_pos);
+    const uint64_t bal = __ballot(f);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __shared__ int32_t wsum[4];
+    if (lane == 0) wsum[wv] = __popcll(bal);
+    __syncthreads();
+    int before = 0;
+    for (int t = 0; t < wv; t++) before += wsum[t];
+    const int r8 = before + __popcll(bal & ((1ull << lane) - 1ull)); /* 8-bit rank in block */
+    const int o8 = blk[blockIdx.x];
+    if (k >= n) return;
+    if (f) {
+        if (o8 + r8 < n8) sel8[o8 + r8] = k;
+    } else {
+        const int o16 = blockIdx.x * 256 - o8;
+        const int r16 = (int)threadIdx.x - r8;
+        if (o16 + r16 < n - n8) sel16[o16 + r16] = k;
+    }
+}
+
+/* Bounds of a planned 16-bit batch: every group within the planned columns
+ * and the length limit. */
+__global__ void plan_check16(const GroupShape *shape, int32_t ngroups, int32_t max_ncol,
+                             int32_t *err)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    if (shape[g].ncol > max_ncol || shape[g].nrow > MAX_LEN16) atomicOr(err, 1);
+}
+
+/* The 8-bit padding length: len2 of the last 8-bit pair (bounded). */
+__global__ void plan_pad8(const tal_seqpair *pairs8, int32_t n8, int32_t *out, int32_t *err)
+{
+    const int v = pairs8[n8 - 1].len2;
+    *out = v;
+    if (v < 0 || v > MAX_LEN8) atomicOr(err, 1);
+}
+
+__global__ void copy_i32(const int32_t *src, int32_t *dst) { *dst = *src; }
+
 /* ---------------------------------------------------------------- launchers */
 
 hipError_t launch_main(int bits, bool lds, int grid, size_t lds_bytes, const KParams &K,
@@ -877,41 +980,81 @@ hipError_t set_main_lds_limit(size_t bytes)
 }
 
 hipError_t launch_shapes(int bits, const tal_seqpair *pairs, int32_t n, int32_t ngroups,
-                         int32_t pad_len2, GroupShape *out, int64_t *ncol_plus1, hipStream_t s)
+                         int32_t pad_len2, GroupShape *out, int64_t *ncol_plus1, hipStream_t s,
+                         const int32_t *pad_len2_dev)
 {
     const int tpb = 256, grid = (ngroups + tpb - 1) / tpb;
     if (grid == 0) return hipSuccess;
     if (bits == 16)
-        hipLaunchKernelGGL(bsw_shapes<32>, dim3(grid), dim3(tpb), 0, s, pairs, n, ngroups, 0, out,
-                           ncol_plus1);
+        hipLaunchKernelGGL(bsw_shapes<32>, dim3(grid), dim3(tpb), 0, s, pairs, n, ngroups, 0,
+                           (const int32_t *)nullptr, out, ncol_plus1);
     else
         hipLaunchKernelGGL(bsw_shapes<64>, dim3(grid), dim3(tpb), 0, s, pairs, n, ngroups,
-                           pad_len2, out, ncol_plus1);
+                           pad_len2, pad_len2_dev, out, ncol_plus1);
     return hipGetLastError();
 }
 
 hipError_t launch_resolve(int32_t ngroups, const GroupShape *shape, const uint32_t *dep,
                           const uint8_t *wrote, const uint32_t *fmask, const int64_t *fmask_off,
                           const uint32_t *image, const int32_t *seg_first, uint32_t *stale,
-                          int32_t *rerun_list, int32_t *rerun_count, hipStream_t s)
+                          int32_t *rerun_list, int32_t *rerun_count, hipStream_t s,
+                          const int32_t *abort_flag)
 {
     const int tpb = 256, grid = (ngroups + tpb - 1) / tpb;
     if (grid == 0) return hipSuccess;
     hipLaunchKernelGGL(bsw_resolve, dim3(grid), dim3(tpb), 0, s, ngroups, shape, dep, wrote,
-                       fmask, fmask_off, image, seg_first, stale, rerun_list, rerun_count);
+                       fmask, fmask_off, image, seg_first, stale, rerun_list, rerun_count,
+                       abort_flag);
     return hipGetLastError();
 }
 
 hipError_t launch_image_update(int32_t ngroups, int32_t ncols, const GroupShape *shape,
                                const uint8_t *wrote, int32_t *last, const uint32_t *fmask,
-                               const int64_t *fmask_off, uint32_t *image, hipStream_t s)
+                               const int64_t *fmask_off, uint32_t *image, hipStream_t s,
+                               const int32_t *skip)
 {
     const int tpb = 256, grid = (ngroups + tpb - 1) / tpb;
     if (grid == 0) return hipSuccess;
     hipLaunchKernelGGL(bsw_last_writer_scatter, dim3(grid), dim3(tpb), 0, s, ngroups, shape,
-                       wrote, last);
+                       wrote, last, skip);
     hipLaunchKernelGGL(bsw_image_update, dim3(1), dim3(1024), 0, s, ncols, shape, last, fmask,
-                       fmask_off, image);
+                       fmask_off, image, skip);
+    return hipGetLastError();
+}
+
+hipError_t launch_partition(const tal_seqpair *pairs, int32_t n, int32_t match_pos,
+                            int32_t planned8, int32_t *blk, int32_t *sel8, int32_t *sel16,
+                            int32_t *err, hipStream_t s)
+{
+    const int nblk = (n + 255) / 256;
+    if (nblk == 0) return hipSuccess;
+    hipLaunchKernelGGL(part_count, dim3(nblk), dim3(256), 0, s, pairs, n, match_pos, blk);
+    hipLaunchKernelGGL(part_offsets, dim3(1), dim3(1024), 0, s, blk, nblk, planned8, err);
+    hipLaunchKernelGGL(part_scatter, dim3(nblk), dim3(256), 0, s, pairs, n, match_pos, blk,
+                       planned8, sel8, sel16);
+    return hipGetLastError();
+}
+
+hipError_t launch_plan_check16(const GroupShape *shape, int32_t ngroups, int32_t max_ncol,
+                               int32_t *err, hipStream_t s)
+{
+    const int tpb = 256, grid = (ngroups + tpb - 1) / tpb;
+    if (grid == 0) return hipSuccess;
+    hipLaunchKernelGGL(plan_check16, dim3(grid), dim3(tpb), 0, s, shape, ngroups, max_ncol, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_plan_pad8(const tal_seqpair *pairs8, int32_t n8, int32_t *out, int32_t *err,
+                            hipStream_t s)
+{
+    if (n8 <= 0) return hipSuccess;
+    hipLaunchKernelGGL(plan_pad8, dim3(1), dim3(1), 0, s, pairs8, n8, out, err);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_i32(const int32_t *src, int32_t *dst, hipStream_t s)
+{
+    hipLaunchKernelGGL(copy_i32, dim3(1), dim3(1), 0, s, src, dst);
     return hipGetLastError();
 }
 

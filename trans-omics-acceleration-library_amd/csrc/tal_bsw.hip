@@ -9,6 +9,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
+#include <chrono>
 #include <cstdarg>
 #include <cstdlib>
 #include <cstdio>
@@ -24,14 +25,25 @@ hipError_t launch_main(int bits, bool lds, int grid, size_t lds_bytes, const KPa
                        const KBuffers &B, hipStream_t s);
 hipError_t set_main_lds_limit(size_t bytes);
 hipError_t launch_shapes(int bits, const tal_seqpair *pairs, int32_t n, int32_t ngroups,
-                         int32_t pad_len2, GroupShape *out, int64_t *ncol_plus1, hipStream_t s);
+                         int32_t pad_len2, GroupShape *out, int64_t *ncol_plus1, hipStream_t s,
+                         const int32_t *pad_len2_dev = nullptr);
 hipError_t launch_resolve(int32_t ngroups, const GroupShape *shape, const uint32_t *dep,
                           const uint8_t *wrote, const uint32_t *fmask, const int64_t *fmask_off,
                           const uint32_t *image, const int32_t *seg_first, uint32_t *stale,
-                          int32_t *rerun_list, int32_t *rerun_count, hipStream_t s);
+                          int32_t *rerun_list, int32_t *rerun_count, hipStream_t s,
+                          const int32_t *abort_flag = nullptr);
 hipError_t launch_image_update(int32_t ngroups, int32_t ncols, const GroupShape *shape,
                                const uint8_t *wrote, int32_t *last, const uint32_t *fmask,
-                               const int64_t *fmask_off, uint32_t *image, hipStream_t s);
+                               const int64_t *fmask_off, uint32_t *image, hipStream_t s,
+                               const int32_t *skip = nullptr);
+hipError_t launch_partition(const tal_seqpair *pairs, int32_t n, int32_t match_pos,
+                            int32_t planned8, int32_t *blk, int32_t *sel8, int32_t *sel16,
+                            int32_t *err, hipStream_t s);
+hipError_t launch_plan_check16(const GroupShape *shape, int32_t ngroups, int32_t max_ncol,
+                               int32_t *err, hipStream_t s);
+hipError_t launch_plan_pad8(const tal_seqpair *pairs8, int32_t n8, int32_t *out, int32_t *err,
+                            hipStream_t s);
+hipError_t launch_copy_i32(const int32_t *src, int32_t *dst, hipStream_t s);
 hipError_t launch_flag8(const tal_seqpair *pairs, int32_t n, int32_t match_pos, uint8_t *flag,
                         hipStream_t s);
 hipError_t launch_gather(const tal_seqpair *src, const int32_t *idx, int32_t m,
@@ -237,6 +249,29 @@ __global__ void fill_i32(int32_t *x, int32_t n, int32_t v)
     if (k < n) x[k] = v;
 }
 
+/* Fixed-stride F-mask offsets of a planned batch (stride = planned
+ * columns + 1), so that they stay valid for any batch within the plan. */
+__global__ void stride_offsets(int64_t *off, int32_t n, int64_t stride)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) off[k] = (int64_t)k * stride;
+}
+
+/* Tasks of a rerun launch from the device rerun count. */
+__global__ void tasks_from_count(const int32_t *count, int32_t gpw, int32_t *ntasks)
+{
+    *ntasks = (*count + gpw - 1) / gpw;
+}
+
+/* Control word of a replayed batch: skip the in-graph image update when
+ * reruns are still pending or the batch left the plan's bounds. */
+__global__ void make_skip(const int32_t *count, const int32_t *err, int32_t *skip,
+                          int32_t *count_out)
+{
+    *skip = (*count != 0 || *err != 0) ? 1 : 0;
+    *count_out = *count;
+}
+
 /* Pairs up a rerun list into 16-bit wave tasks (two groups per task). */
 __global__ void make_tasks(const int32_t *list, const int32_t *count, int32_t *tasks,
                            int32_t cap_tasks, int gpw)
@@ -259,6 +294,68 @@ int grid_for(const tal_bsw *h, bool lds, size_t lds_bytes, int ntasks)
     }
     long long g = (long long)h->cu_count * per_cu;
     return (int)std::max<long long>(1, std::min<long long>(g, ntasks));
+}
+
+/* Placement of a wave's state: columns 0..max_ncol plus 17 columns of
+ * slack for the 8-column block loads that run past a group's last column,
+ * and the fast path's query-code words (8 columns per word, one word of
+ * prefetch slack).  The first lds_cols state columns live in LDS, sized so
+ * that `waves_per_cu` waves fit on a CU; the query words and the remaining
+ * state columns go to a per-block slab in global memory. */
+struct Sizing {
+    int state_cols, qwords, lds_cols, slab_cols;
+    size_t lds_bytes;
+};
+
+Sizing sizing(const tal_bsw *h, int max_ncol)
+{
+    Sizing z{};
+    z.state_cols = max_ncol + 18;
+    z.qwords = (max_ncol >> 3) + 2;
+    const int wpc = std::max(1, h->waves_per_cu);
+    const long budget_cols = (long)(h->lds_per_cu / wpc) / 256;
+    const int fit_all = (z.state_cols + 7) & ~7;
+    int lds_cols = (int)std::max<long>(8, budget_cols) & ~7;
+    if (fit_all <= lds_cols) lds_cols = fit_all;
+    const size_t max_cols = h->lds_per_block / 256;
+    if ((size_t)lds_cols > max_cols) lds_cols = (int)(max_cols & ~(size_t)7);
+    z.lds_cols = lds_cols;
+    z.slab_cols = z.qwords + std::max(0, z.state_cols - lds_cols);
+    z.lds_bytes = (size_t)lds_cols * 64 * sizeof(uint32_t);
+    return z;
+}
+
+/* Task order: longest groups first, and the two groups of a 16-bit wave of
+ * similar size, so that the persistent grid ends evenly and the two
+ * half-waves of a wave run loops of similar length.  Any order gives the
+ * same results; this one only changes the schedule.  Leaves the task list
+ * (ntasks * GPW ids, -1 padded) in pb.order. */
+int make_order(tal_bsw *h, PathBufs &pb, const GroupShape *shapes, int32_t G, int32_t w,
+               int ntasks, int GPW, hipStream_t s)
+{
+    HIPCHK(h, pb.ckey.ensure(sizeof(uint32_t) * G), "alloc keys");
+    HIPCHK(h, pb.ckey2.ensure(sizeof(uint32_t) * G), "alloc keys");
+    HIPCHK(h, pb.gids.ensure(sizeof(int32_t) * G), "alloc ids");
+    HIPCHK(h, pb.order.ensure(sizeof(int32_t) * ((size_t)ntasks * GPW)), "alloc order");
+    hipLaunchKernelGGL(group_cost, dim3((G + 255) / 256), dim3(256), 0, s, shapes, G, w,
+                       pb.ckey.as<uint32_t>(), pb.gids.as<int32_t>());
+    HIPCHK(h, hipGetLastError(), "group cost");
+    size_t tmp = 0;
+    HIPCHK(h, hipcub::DeviceRadixSort::SortPairsDescending(
+                  nullptr, tmp, pb.ckey.as<uint32_t>(), pb.ckey2.as<uint32_t>(),
+                  pb.gids.as<int32_t>(), pb.order.as<int32_t>(), G, 0, 32, s),
+           "sort size");
+    HIPCHK(h, pb.cubtmp.ensure(tmp + 16), "alloc sort tmp");
+    HIPCHK(h, hipcub::DeviceRadixSort::SortPairsDescending(
+                  pb.cubtmp.p, tmp, pb.ckey.as<uint32_t>(), pb.ckey2.as<uint32_t>(),
+                  pb.gids.as<int32_t>(), pb.order.as<int32_t>(), G, 0, 32, s),
+           "sort groups");
+    const int pad = ntasks * GPW - G;
+    if (pad > 0) {
+        hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, s, pb.order.as<int32_t>() + G, pad, -1);
+        HIPCHK(h, hipGetLastError(), "pad tasks");
+    }
+    return TAL_OK;
 }
 
 /*
@@ -317,24 +414,10 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
     const int32_t max_ncol = bits == 16 ? sum.max_ncol : MAX_LEN8;
     const int64_t total_cols = sum.total_cols;
 
-    /* state placement: LDS if the widest group fits, else global slabs */
-    /* Per wave: state columns 0..max_ncol plus 17 columns of slack for the
-     * 8-column block loads that run past a group's last column, and the
-     * fast path's query-code words (8 columns per word, one word of prefetch
-     * slack).  The first lds_cols state columns live in LDS, sized so that
-     * `waves_per_cu` waves fit on a CU; the query words and the remaining
-     * state columns go to a per-block slab in global memory. */
-    const int state_cols = max_ncol + 18;
-    const int qwords = (max_ncol >> 3) + 2;
-    const int wpc = std::max(1, h->waves_per_cu);
-    const long budget_cols = (long)(h->lds_per_cu / wpc) / 256;
-    const int fit_all = (state_cols + 7) & ~7;
-    int lds_cols = (int)std::max<long>(8, budget_cols) & ~7;
-    if (fit_all <= lds_cols) lds_cols = fit_all;
-    const size_t max_cols = h->lds_per_block / 256;
-    if ((size_t)lds_cols > max_cols) lds_cols = (int)(max_cols & ~(size_t)7);
-    const int slab_cols = qwords + std::max(0, state_cols - lds_cols);
-    const size_t lds_bytes = (size_t)lds_cols * 64 * sizeof(uint32_t);
+    const Sizing z = sizing(h, max_ncol);
+    const int state_cols = z.state_cols, qwords = z.qwords, lds_cols = z.lds_cols;
+    const int slab_cols = z.slab_cols;
+    const size_t lds_bytes = z.lds_bytes;
     const bool lds = true;
     const int ntasks = (G + GPW - 1) / GPW;
     int grid = grid_for(h, lds, lds_bytes, ntasks);
@@ -349,32 +432,8 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
     B.ref = ref;
     B.qer = qer;
     if (h->order == 1 && G > 1) {
-        /* Longest groups first, and the two groups of a 16-bit wave of
-         * similar size: the persistent grid then ends evenly, and the two
-         * half-waves of a wave run loops of similar length. */
-        HIPCHK(h, pb.ckey.ensure(sizeof(uint32_t) * G), "alloc keys");
-        HIPCHK(h, pb.ckey2.ensure(sizeof(uint32_t) * G), "alloc keys");
-        HIPCHK(h, pb.gids.ensure(sizeof(int32_t) * G), "alloc ids");
-        HIPCHK(h, pb.order.ensure(sizeof(int32_t) * ((size_t)ntasks * GPW)), "alloc order");
-        hipLaunchKernelGGL(group_cost, dim3((G + 255) / 256), dim3(256), 0, s, shapes, G, w,
-                           pb.ckey.as<uint32_t>(), pb.gids.as<int32_t>());
-        HIPCHK(h, hipGetLastError(), "group cost");
-        size_t tmp = 0;
-        HIPCHK(h, hipcub::DeviceRadixSort::SortPairsDescending(
-                      nullptr, tmp, pb.ckey.as<uint32_t>(), pb.ckey2.as<uint32_t>(),
-                      pb.gids.as<int32_t>(), pb.order.as<int32_t>(), G, 0, 32, s),
-               "sort size");
-        HIPCHK(h, pb.cubtmp.ensure(tmp + 16), "alloc sort tmp");
-        HIPCHK(h, hipcub::DeviceRadixSort::SortPairsDescending(
-                      pb.cubtmp.p, tmp, pb.ckey.as<uint32_t>(), pb.ckey2.as<uint32_t>(),
-                      pb.gids.as<int32_t>(), pb.order.as<int32_t>(), G, 0, 32, s),
-               "sort groups");
-        const int pad = ntasks * GPW - G;
-        if (pad > 0) {
-            hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, s, pb.order.as<int32_t>() + G, pad,
-                               -1);
-            HIPCHK(h, hipGetLastError(), "pad tasks");
-        }
+        const int rc = make_order(h, pb, shapes, G, w, ntasks, GPW, s);
+        if (rc) return rc;
         B.task_groups = pb.order.as<int32_t>();
     }
     HIPCHK(h, pb.counter.ensure(sizeof(int32_t) * 4), "alloc counter");
@@ -458,6 +517,253 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
         h->ev8 = true;
         h->st.groups8 = G;
     }
+    return TAL_OK;
+}
+
+
+/* ------------------------------------------------------------ batch plans */
+
+/* One width of a plan: its sizing, launch shapes and buffers. */
+struct WidthPlan {
+    int32_t n = 0, G = 0, ntasks = 0, grid = 0, rgrid = 0, max_ncol = 0;
+    Sizing z{};
+    KParams K{};
+    KBuffers B{};
+    PathBufs pb;
+};
+
+} // namespace
+
+struct tal_bsw_plan {
+    tal_bsw *h = nullptr;
+    tal_seqpair *pairs = nullptr;
+    const uint8_t *ref = nullptr, *qer = nullptr;
+    int32_t n = 0, w = 0, n8 = 0, n16 = 0;
+    int rounds = 2; /* rerun rounds inside the graph */
+    WidthPlan p8, p16;
+    /* ctl: [0] error, [1] 8-bit padding len2, [2] rerun tasks, [3] skip,
+     * [4] pending reruns after the graph */
+    DevBuf blk, sel8, sel16, pairs8, pairs16, ctl;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void plan_release(tal_bsw_plan *P)
+{
+    if (P->exec) (void)hipGraphExecDestroy(P->exec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    DevBuf *all[] = {&P->blk, &P->sel8, &P->sel16, &P->pairs8, &P->pairs16, &P->ctl};
+    for (DevBuf *b : all) b->release();
+    for (PathBufs *pb : {&P->p8.pb, &P->p16.pb}) {
+        DevBuf *bs[] = {&pb->shapes, &pb->ncp1, &pb->foff,    &pb->dep,   &pb->wrote,
+                        &pb->stale,  &pb->fmask, &pb->rerun,  &pb->tasks, &pb->counter,
+                        &pb->slabs,  &pb->last,  &pb->cubtmp, &pb->summary, &pb->ckey,
+                        &pb->ckey2,  &pb->gids,  &pb->order};
+        for (DevBuf *b : bs) b->release();
+    }
+}
+
+/* Sizes one width of a plan from the batch as it is now (eager, with host
+ * reads) and allocates everything its replay needs. */
+int plan_width(tal_bsw *h, tal_bsw_plan *P, int bits, hipStream_t s)
+{
+    WidthPlan &W = bits == 16 ? P->p16 : P->p8;
+    W.n = bits == 16 ? P->n16 : P->n8;
+    if (W.n == 0) return TAL_OK;
+    PathBufs &pb = W.pb;
+    const int GL = bits == 16 ? 32 : 64, GPW = 64 / GL;
+    tal_seqpair *pairs = bits == 16 ? P->pairs16.as<tal_seqpair>() : P->pairs8.as<tal_seqpair>();
+    W.G = (W.n + GL - 1) / GL;
+    const int32_t G = W.G;
+    int32_t pad_len2 = 0;
+    if (bits == 8) {
+        HIPCHK(h, hipMemcpyAsync(&pad_len2, &pairs[W.n - 1].len2, sizeof pad_len2,
+                                 hipMemcpyDeviceToHost, s),
+               "read last len2");
+        HIPCHK(h, hipStreamSynchronize(s), "sync last len2");
+    }
+    HIPCHK(h, pb.shapes.ensure(sizeof(GroupShape) * G), "alloc shapes");
+    HIPCHK(h, pb.summary.ensure(sizeof(ShapeSummary)), "alloc summary");
+    GroupShape *shapes = pb.shapes.as<GroupShape>();
+    HIPCHK(h, launch_shapes(bits, pairs, W.n, G, pad_len2, shapes, nullptr, s), "shapes");
+    hipLaunchKernelGGL(summarize, dim3(1), dim3(256), 0, s, shapes, (const int64_t *)nullptr, G,
+                       pb.summary.as<ShapeSummary>());
+    HIPCHK(h, hipGetLastError(), "summary");
+    ShapeSummary sum{};
+    HIPCHK(h, hipMemcpyAsync(&sum, pb.summary.p, sizeof sum, hipMemcpyDeviceToHost, s),
+           "read summary");
+    HIPCHK(h, hipStreamSynchronize(s), "sync summary");
+    const int limit = bits == 16 ? MAX_LEN16 : MAX_LEN8;
+    if (sum.max_ncol > limit || sum.max_nrow > limit)
+        return fail(h, TAL_ERANGE, "a pair length is negative or exceeds %d", limit);
+    W.max_ncol = bits == 16 ? sum.max_ncol : MAX_LEN8;
+    W.z = sizing(h, W.max_ncol);
+    W.ntasks = (G + GPW - 1) / GPW;
+    W.grid = grid_for(h, true, W.z.lds_bytes, W.ntasks);
+    W.rgrid = std::min(W.grid, h->cu_count * 2);
+    KParams &K = W.K;
+    K = base_params(h, bits, W.n, P->w);
+    K.ngroups = G;
+    K.ntasks = W.ntasks;
+    K.state_cols = W.z.state_cols;
+    K.lds_cols = W.z.lds_cols;
+    K.slab_cols = W.z.slab_cols;
+    K.query_words = W.z.qwords;
+    KBuffers &B = W.B;
+    B = KBuffers{};
+    B.pairs = pairs;
+    B.ref = P->ref;
+    B.qer = P->qer;
+    if (h->order == 1 && G > 1) {
+        const int rc = make_order(h, pb, shapes, G, P->w, W.ntasks, GPW, s);
+        if (rc) return rc;
+        B.task_groups = pb.order.as<int32_t>();
+    }
+    HIPCHK(h, pb.counter.ensure(sizeof(int32_t) * 4), "alloc counter");
+    B.task_counter = pb.counter.as<int32_t>();
+    B.abort_flag = P->ctl.as<int32_t>();
+    HIPCHK(h, pb.slabs.ensure((size_t)W.grid * W.z.slab_cols * 64 * sizeof(uint32_t)),
+           "alloc state slabs");
+    B.slabs = pb.slabs.as<uint32_t>();
+    if (bits == 8) {
+        B.last_len2_dev = P->ctl.as<int32_t>() + 1;
+    } else {
+        const int64_t stride = (int64_t)W.max_ncol + 1;
+        HIPCHK(h, pb.foff.ensure(sizeof(int64_t) * G), "alloc offsets");
+        HIPCHK(h, pb.dep.ensure(sizeof(uint32_t) * G), "alloc dep");
+        HIPCHK(h, pb.wrote.ensure(G), "alloc wrote");
+        HIPCHK(h, pb.stale.ensure(sizeof(uint32_t) * G), "alloc stale");
+        HIPCHK(h, pb.fmask.ensure(sizeof(uint32_t) * (size_t)G * stride), "alloc fmask");
+        HIPCHK(h, pb.rerun.ensure(sizeof(int32_t) * (G + 1)), "alloc rerun");
+        HIPCHK(h, pb.tasks.ensure(sizeof(int32_t) * (size_t)(G + 2) * 2), "alloc tasks");
+        HIPCHK(h, pb.last.ensure(sizeof(int32_t) * IMAGE_COLS), "alloc last");
+        hipLaunchKernelGGL(stride_offsets, dim3((G + 255) / 256), dim3(256), 0, s,
+                           pb.foff.as<int64_t>(), G, stride);
+        HIPCHK(h, hipGetLastError(), "offsets");
+        B.stale_in = pb.stale.as<uint32_t>();
+        B.dep_out = pb.dep.as<uint32_t>();
+        B.wrote_out = pb.wrote.as<uint8_t>();
+        B.fmask = pb.fmask.as<uint32_t>();
+        B.fmask_off = pb.foff.as<int64_t>();
+    }
+    return TAL_OK;
+}
+
+/* The replayed pipeline: dispatcher split, gathers, both DP launches, the
+ * stale-column resolution with `rounds` rerun rounds, the image update when
+ * nothing is pending, and the result scatters.  No host reads, no memsets. */
+int plan_enqueue(tal_bsw *h, tal_bsw_plan *P, hipStream_t s)
+{
+    int32_t *ctl = P->ctl.as<int32_t>();
+    hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, s, ctl, 8, 0);
+    HIPCHK(h, launch_partition(P->pairs, P->n, smax_of(h->prm), P->n8, P->blk.as<int32_t>(),
+                               P->sel8.as<int32_t>(), P->sel16.as<int32_t>(), ctl, s),
+           "partition");
+    if (P->n8)
+        HIPCHK(h, launch_gather(P->pairs, P->sel8.as<int32_t>(), P->n8,
+                                P->pairs8.as<tal_seqpair>(), s),
+               "gather8");
+    if (P->n16)
+        HIPCHK(h, launch_gather(P->pairs, P->sel16.as<int32_t>(), P->n16,
+                                P->pairs16.as<tal_seqpair>(), s),
+               "gather16");
+    if (P->n8) {
+        WidthPlan &W = P->p8;
+        HIPCHK(h, launch_plan_pad8(P->pairs8.as<tal_seqpair>(), P->n8, ctl + 1, ctl, s), "pad8");
+        hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, s, W.B.task_counter, 1, 0);
+        HIPCHK(h, launch_main(8, true, W.grid, W.z.lds_bytes, W.K, W.B, s), "main8");
+    }
+    if (P->n16) {
+        WidthPlan &W = P->p16;
+        const int32_t G = W.G;
+        GroupShape *shapes = W.pb.shapes.as<GroupShape>();
+        int32_t *cnt = W.B.task_counter + 1;
+        int32_t *list = W.pb.rerun.as<int32_t>();
+        HIPCHK(h, launch_shapes(16, P->pairs16.as<tal_seqpair>(), W.n, G, 0, shapes, nullptr, s),
+               "shapes16");
+        HIPCHK(h, launch_plan_check16(shapes, G, W.max_ncol, ctl, s), "check16");
+        hipLaunchKernelGGL(fill_i32, dim3((G + 255) / 256), dim3(256), 0, s,
+                           (int32_t *)W.pb.stale.p, G, 0);
+        hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, s, W.B.task_counter, 2, 0);
+        HIPCHK(h, launch_main(16, true, W.grid, W.z.lds_bytes, W.K, W.B, s), "main16");
+        HIPCHK(h, launch_resolve(G, shapes, W.B.dep_out, W.B.wrote_out, W.B.fmask,
+                                 W.B.fmask_off, h->image.as<uint32_t>(), nullptr,
+                                 W.pb.stale.as<uint32_t>(), list, cnt, s, ctl),
+               "resolve");
+        const int cap = (G + 1) / 2;
+        for (int r = 0; r < P->rounds; r++) {
+            hipLaunchKernelGGL(make_tasks, dim3((cap + 255) / 256), dim3(256), 0, s, list, cnt,
+                               W.pb.tasks.as<int32_t>(), cap, 2);
+            hipLaunchKernelGGL(tasks_from_count, dim3(1), dim3(1), 0, s, cnt, 2, ctl + 2);
+            hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, s, W.B.task_counter, 2, 0);
+            KBuffers B2 = W.B;
+            B2.task_groups = W.pb.tasks.as<int32_t>();
+            B2.ntasks_dev = ctl + 2;
+            HIPCHK(h, launch_main(16, true, W.rgrid, W.z.lds_bytes, W.K, B2, s), "rerun16");
+            HIPCHK(h, launch_resolve(G, shapes, W.B.dep_out, W.B.wrote_out, W.B.fmask,
+                                     W.B.fmask_off, h->image.as<uint32_t>(), nullptr,
+                                     W.pb.stale.as<uint32_t>(), list, cnt, s, ctl),
+                   "resolve");
+        }
+        hipLaunchKernelGGL(make_skip, dim3(1), dim3(1), 0, s, cnt, ctl, ctl + 3, ctl + 4);
+        hipLaunchKernelGGL(fill_i32, dim3(IMAGE_COLS / 256), dim3(256), 0, s, W.pb.last.as<int32_t>(),
+                           IMAGE_COLS, -1);
+        HIPCHK(h, launch_image_update(G, W.max_ncol + 1, shapes, W.B.wrote_out,
+                                      W.pb.last.as<int32_t>(), W.B.fmask, W.B.fmask_off,
+                                      h->image.as<uint32_t>(), s, ctl + 3),
+               "image update");
+    }
+    if (P->n8)
+        HIPCHK(h, launch_scatter_out(P->pairs8.as<tal_seqpair>(), P->sel8.as<int32_t>(), P->n8,
+                                     P->pairs, s),
+               "scatter8");
+    if (P->n16)
+        HIPCHK(h, launch_scatter_out(P->pairs16.as<tal_seqpair>(), P->sel16.as<int32_t>(),
+                                     P->n16, P->pairs, s),
+               "scatter16");
+    return TAL_OK;
+}
+
+/* After a replay: the reruns the graph's rounds left, eagerly, then the
+ * image update. */
+int plan_finish(tal_bsw *h, tal_bsw_plan *P, hipStream_t s, int32_t pending)
+{
+    WidthPlan &W = P->p16;
+    const int32_t G = W.G;
+    GroupShape *shapes = W.pb.shapes.as<GroupShape>();
+    int32_t *cnt = W.B.task_counter + 1;
+    int32_t *list = W.pb.rerun.as<int32_t>();
+    int32_t c = pending;
+    for (int iter = 0; c > 0; iter++) {
+        if (iter > G + 1) return fail(h, TAL_EDEVICE, "stale-column resolution did not converge");
+        h->st.reruns16 += c;
+        const int rt = (c + 1) / 2;
+        hipLaunchKernelGGL(make_tasks, dim3((rt + 255) / 256), dim3(256), 0, s, list, cnt,
+                           W.pb.tasks.as<int32_t>(), rt, 2);
+        HIPCHK(h, hipGetLastError(), "make tasks");
+        KParams K2 = W.K;
+        K2.ntasks = rt;
+        KBuffers B2 = W.B;
+        B2.task_groups = W.pb.tasks.as<int32_t>();
+        HIPCHK(h, hipMemsetAsync(W.B.task_counter, 0, sizeof(int32_t), s), "clear counter");
+        HIPCHK(h, launch_main(16, true, std::min(W.grid, grid_for(h, true, W.z.lds_bytes, rt)),
+                              W.z.lds_bytes, K2, B2, s),
+               "rerun kernel");
+        HIPCHK(h, hipMemsetAsync(cnt, 0, sizeof(int32_t), s), "clear rerun count");
+        HIPCHK(h, launch_resolve(G, shapes, W.B.dep_out, W.B.wrote_out, W.B.fmask,
+                                 W.B.fmask_off, h->image.as<uint32_t>(), nullptr,
+                                 W.pb.stale.as<uint32_t>(), list, cnt, s),
+               "resolve");
+        HIPCHK(h, hipMemcpyAsync(&c, cnt, sizeof c, hipMemcpyDeviceToHost, s), "read reruns");
+        HIPCHK(h, hipStreamSynchronize(s), "sync reruns");
+    }
+    HIPCHK(h, hipMemsetAsync(W.pb.last.p, 0xFF, sizeof(int32_t) * IMAGE_COLS, s), "clear last");
+    HIPCHK(h, launch_image_update(G, W.max_ncol + 1, shapes, W.B.wrote_out,
+                                  W.pb.last.as<int32_t>(), W.B.fmask, W.B.fmask_off,
+                                  h->image.as<uint32_t>(), s),
+           "image update");
     return TAL_OK;
 }
 
@@ -752,6 +1058,206 @@ int tal_bsw_get_scores_auto_device(tal_bsw *h, tal_seqpair *pairs, const uint8_t
         HIPCHK(h, launch_scatter_out(h->pairs16.as<tal_seqpair>(), sel + n8, n16, pairs, s),
                "scatter16");
     }
+    return TAL_OK;
+}
+
+int tal_bsw_plan_create(tal_bsw *h, tal_seqpair *pairs, const uint8_t *ref, const uint8_t *qer,
+                        int32_t n, int32_t w, tal_bsw_plan **out)
+{
+    if (!out) return TAL_EINVAL;
+    *out = nullptr;
+    int rc = check_common(h, pairs, n, w);
+    if (rc) return rc;
+    if (n <= 0) return fail(h, TAL_EINVAL, "a plan needs n > 0");
+    HIPCHK(h, hipSetDevice(h->device), "set device");
+    hipStream_t s = h->stream;
+    tal_bsw_plan *P = new (std::nothrow) tal_bsw_plan();
+    if (!P) return TAL_ENOMEM;
+    P->h = h;
+    P->pairs = pairs;
+    P->ref = ref;
+    P->qer = qer;
+    P->n = n;
+    P->w = w;
+    struct Guard {
+        tal_bsw_plan *P;
+        bool keep = false;
+        ~Guard()
+        {
+            if (!keep) {
+                plan_release(P);
+                delete P;
+            }
+        }
+    } guard{P};
+    const int nblk = (n + 255) / 256;
+    HIPCHK(h, P->blk.ensure(sizeof(int32_t) * nblk), "alloc partition");
+    HIPCHK(h, P->sel8.ensure(sizeof(int32_t) * n), "alloc index");
+    HIPCHK(h, P->sel16.ensure(sizeof(int32_t) * n), "alloc index");
+    HIPCHK(h, P->ctl.ensure(sizeof(int32_t) * 8), "alloc control");
+    /* the planned split, from the batch as it is now */
+    HIPCHK(h, hipMemsetAsync(P->ctl.p, 0, sizeof(int32_t) * 8, s), "clear control");
+    std::vector<int32_t> hb(nblk);
+    HIPCHK(h, launch_partition(pairs, n, smax_of(h->prm), -1, P->blk.as<int32_t>(),
+                               P->sel8.as<int32_t>(), P->sel16.as<int32_t>(),
+                               P->ctl.as<int32_t>(), s),
+           "partition");
+    /* part_offsets left exclusive offsets; the 8-bit total is the last
+     * offset plus the last block's count, recounted on the host */
+    std::vector<tal_seqpair> tail(std::min(n, 256));
+    HIPCHK(h, hipMemcpyAsync(hb.data(), P->blk.p, sizeof(int32_t) * nblk, hipMemcpyDeviceToHost, s),
+           "read partition");
+    HIPCHK(h, hipMemcpyAsync(tail.data(), pairs + (size_t)(nblk - 1) * 256,
+                             sizeof(tal_seqpair) * (size_t)(n - (nblk - 1) * 256),
+                             hipMemcpyDeviceToHost, s),
+           "read last block");
+    HIPCHK(h, hipStreamSynchronize(s), "sync partition");
+    int32_t last8 = 0;
+    for (int32_t k = 0; k < n - (nblk - 1) * 256; k++) last8 += goes_8bit(tail[k], smax_of(h->prm));
+    P->n8 = hb[nblk - 1] + last8;
+    P->n16 = n - P->n8;
+    HIPCHK(h, launch_partition(pairs, n, smax_of(h->prm), P->n8, P->blk.as<int32_t>(),
+                               P->sel8.as<int32_t>(), P->sel16.as<int32_t>(),
+                               P->ctl.as<int32_t>(), s),
+           "partition");
+    HIPCHK(h, P->pairs8.ensure(sizeof(tal_seqpair) * std::max(P->n8, 1)), "alloc pairs8");
+    HIPCHK(h, P->pairs16.ensure(sizeof(tal_seqpair) * std::max(P->n16, 1)), "alloc pairs16");
+    if (P->n8)
+        HIPCHK(h, launch_gather(pairs, P->sel8.as<int32_t>(), P->n8, P->pairs8.as<tal_seqpair>(), s),
+               "gather8");
+    if (P->n16)
+        HIPCHK(h, launch_gather(pairs, P->sel16.as<int32_t>(), P->n16,
+                                P->pairs16.as<tal_seqpair>(), s),
+               "gather16");
+    rc = plan_width(h, P, 8, s);
+    if (rc) return rc;
+    rc = plan_width(h, P, 16, s);
+    if (rc) return rc;
+    HIPCHK(h, hipStreamSynchronize(s), "sync plan");
+    /* capture (kernel events are not recorded inside the graph: the replay
+     * is timed as a whole by tal_bsw_plan_launch) */
+    HIPCHK(h, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+    rc = plan_enqueue(h, P, s);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &g);
+    if (rc != TAL_OK || e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipGetLastError();
+        return rc ? rc : fail(h, TAL_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    }
+    P->graph = g;
+    HIPCHK(h, hipGraphInstantiate(&P->exec, g, nullptr, nullptr, 0), "graph instantiate");
+    guard.keep = true;
+    *out = P;
+    return TAL_OK;
+}
+
+int tal_bsw_plan_launch(tal_bsw_plan *P, void *stream)
+{
+    if (!P) return TAL_EINVAL;
+    tal_bsw *h = P->h;
+    h->st = tal_bsw_stats{};
+    h->ev16 = h->ev8 = false;
+    HIPCHK(h, hipSetDevice(h->device), "set device");
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    HIPCHK(h, hipEventRecord(h->ev[4], s), "event");
+    HIPCHK(h, hipGraphLaunch(P->exec, s), "graph launch");
+    HIPCHK(h, hipEventRecord(h->ev[5], s), "event");
+    int32_t ctl[8];
+    HIPCHK(h, hipMemcpyAsync(ctl, P->ctl.p, sizeof ctl, hipMemcpyDeviceToHost, s), "read control");
+    HIPCHK(h, hipStreamSynchronize(s), "sync graph");
+    HIPCHK(h, hipEventElapsedTime(&h->st.ms_replay, h->ev[4], h->ev[5]), "event time");
+    h->st.pairs8 = P->n8;
+    h->st.pairs16 = P->n16;
+    h->st.groups8 = P->p8.G;
+    h->st.groups16 = P->p16.G;
+    if (ctl[0] != 0)
+        return fail(h, TAL_ERANGE, "the batch no longer fits its plan (dispatcher split or "
+                                   "group columns changed); build a new plan");
+    if (P->n16 && ctl[4] != 0) return plan_finish(h, P, s, ctl[4]);
+    return TAL_OK;
+}
+
+void tal_bsw_plan_destroy(tal_bsw_plan *P)
+{
+    if (!P) return;
+    (void)hipSetDevice(P->h->device);
+    (void)hipStreamSynchronize(P->h->stream);
+    plan_release(P);
+    delete P;
+}
+
+int tal_bsw_set_tuning(tal_bsw *h, int32_t waves_per_cu, int32_t order)
+{
+    if (!h) return TAL_EINVAL;
+    if (waves_per_cu < 1 || waves_per_cu > 32 || (order != 0 && order != 1))
+        return fail(h, TAL_EINVAL, "bad tuning");
+    h->waves_per_cu = waves_per_cu;
+    h->order = order;
+    return TAL_OK;
+}
+
+int tal_bsw_autotune(tal_bsw *h, const tal_seqpair *pairs, const uint8_t *ref,
+                     const uint8_t *qer, int32_t n, int32_t w, int32_t sample,
+                     tal_bsw_tuning *out)
+{
+    if (!h || !out) return TAL_EINVAL;
+    if (n <= 0 || !pairs || w < 0 || sample < 0) return fail(h, TAL_EINVAL, "bad autotune arguments");
+    const int32_t m = sample > 0 ? std::min(sample, n) : n;
+    HIPCHK(h, hipSetDevice(h->device), "set device");
+    tal_bsw *t = nullptr;
+    int rc = tal_bsw_create(&h->prm, h->device, &t);
+    if (rc) return fail(h, rc, "autotune: scratch handle");
+    DevBuf copy;
+    struct Cleanup {
+        tal_bsw *t;
+        DevBuf *copy;
+        ~Cleanup()
+        {
+            copy->release();
+            tal_bsw_destroy(t);
+        }
+    } cleanup{t, &copy};
+    HIPCHK(h, copy.ensure(sizeof(tal_seqpair) * (size_t)m), "alloc autotune copy");
+    HIPCHK(h, hipMemcpyAsync(copy.p, pairs, sizeof(tal_seqpair) * (size_t)m,
+                             hipMemcpyDeviceToDevice, t->stream),
+           "copy batch");
+    HIPCHK(h, hipStreamSynchronize(t->stream), "sync copy");
+    static const int waves[] = {4, 6, 8, 10, 12, 16};
+    tal_bsw_tuning res{};
+    res.ms = -1.f;
+    for (int order = 0; order < 2; order++)
+        for (int wv : waves) {
+            t->waves_per_cu = wv;
+            t->order = order;
+            tal_bsw_plan *P = nullptr;
+            rc = tal_bsw_plan_create(t, copy.as<tal_seqpair>(), ref, qer, m, w, &P);
+            if (rc) return fail(h, rc, "autotune: %s", t->err.c_str());
+            float best = -1.f;
+            for (int rep = 0; rep < 3 && rc == TAL_OK; rep++) {
+                const auto t0 = std::chrono::steady_clock::now();
+                rc = tal_bsw_plan_launch(P, nullptr);
+                const auto t1 = std::chrono::steady_clock::now();
+                const float ms = std::chrono::duration<float, std::milli>(t1 - t0).count();
+                if (rep > 0 && (best < 0 || ms < best)) best = ms; /* first replay warms */
+            }
+            tal_bsw_plan_destroy(P);
+            if (rc) return fail(h, rc, "autotune: %s", t->err.c_str());
+            if (res.trials < TAL_TUNING_MAX_TRIALS) {
+                res.trial_waves[res.trials] = wv;
+                res.trial_order[res.trials] = order;
+                res.trial_ms[res.trials] = best;
+                res.trials++;
+            }
+            if (res.ms < 0 || best < res.ms) {
+                res.ms = best;
+                res.waves_per_cu = wv;
+                res.order = order;
+            }
+        }
+    h->waves_per_cu = res.waves_per_cu;
+    h->order = res.order;
+    *out = res;
     return TAL_OK;
 }
 

@@ -27,6 +27,8 @@ EXPORTS = ["tal_bsw_create", "tal_bsw_destroy", "tal_bsw_last_error",
            "tal_bsw_get_scores16", "tal_bsw_get_scores8", "tal_bsw_get_scores_auto",
            "tal_bsw_get_scores16_device", "tal_bsw_get_scores8_device",
            "tal_bsw_get_scores_auto_device", "tal_bsw_count_8bit", "tal_bsw_get_stats",
+           "tal_bsw_plan_create", "tal_bsw_plan_launch", "tal_bsw_plan_destroy",
+           "tal_bsw_autotune", "tal_bsw_set_tuning",
            "tal_bsw_server_create", "tal_bsw_server_submit", "tal_bsw_server_wait",
            "tal_bsw_server_get_stats", "tal_bsw_server_destroy"]
 
@@ -40,7 +42,14 @@ class Stats(ctypes.Structure):
     _fields_ = [("ms_main16", ctypes.c_float), ("ms_main8", ctypes.c_float),
                 ("ms_rerun16", ctypes.c_float)] + [
         (n, ctypes.c_int32) for n in ("groups16", "groups8", "reruns16", "lds_state16",
-                                      "pairs8", "pairs16")]
+                                      "pairs8", "pairs16")] + [("ms_replay", ctypes.c_float)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("waves_per_cu", ctypes.c_int32), ("order", ctypes.c_int32),
+                ("ms", ctypes.c_float), ("trials", ctypes.c_int32),
+                ("trial_waves", ctypes.c_int32 * 16), ("trial_order", ctypes.c_int32 * 16),
+                ("trial_ms", ctypes.c_float * 16)]
 
 
 class ServerConfig(ctypes.Structure):
@@ -95,6 +104,15 @@ def load():
         L.tal_bsw_count_8bit.argtypes = [P, P, i32]
         L.tal_bsw_get_stats.restype = ctypes.c_int
         L.tal_bsw_get_stats.argtypes = [P, ctypes.POINTER(Stats)]
+        L.tal_bsw_plan_create.restype = ctypes.c_int
+        L.tal_bsw_plan_create.argtypes = [P, P, P, P, i32, i32, ctypes.POINTER(P)]
+        L.tal_bsw_plan_launch.restype = ctypes.c_int
+        L.tal_bsw_plan_launch.argtypes = [P, P]
+        L.tal_bsw_plan_destroy.argtypes = [P]
+        L.tal_bsw_autotune.restype = ctypes.c_int
+        L.tal_bsw_autotune.argtypes = [P, P, P, P, i32, i32, i32, ctypes.POINTER(Tuning)]
+        L.tal_bsw_set_tuning.restype = ctypes.c_int
+        L.tal_bsw_set_tuning.argtypes = [P, i32, i32]
         L.tal_bsw_server_create.restype = ctypes.c_int
         L.tal_bsw_server_create.argtypes = [ctypes.POINTER(Params), ctypes.c_int,
                                             ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]
@@ -170,6 +188,49 @@ class BandedPairWiseSW:
 
     def count_8bit(self, pairs):
         return int(load().tal_bsw_count_8bit(self._h, _ptr(pairs), len(pairs)))
+
+    def plan(self, d_pairs, d_ref, d_qer, n, w):
+        """A replayable, graph-captured plan of a device batch (Plan)."""
+        return Plan(self, d_pairs, d_ref, d_qer, n, w)
+
+    def autotune(self, d_pairs, d_ref, d_qer, n, w, sample=0):
+        """Times the launch configurations on the batch (device pointers) and
+        applies the fastest; returns the trials."""
+        t = Tuning()
+        self._check(load().tal_bsw_autotune(self._h, ctypes.c_void_p(d_pairs),
+                                            ctypes.c_void_p(d_ref), ctypes.c_void_p(d_qer),
+                                            int(n), int(w), int(sample), ctypes.byref(t)))
+        k = t.trials
+        return {"waves_per_cu": t.waves_per_cu, "order": t.order, "ms": t.ms,
+                "trials": [{"waves_per_cu": t.trial_waves[i], "order": t.trial_order[i],
+                            "ms": t.trial_ms[i]} for i in range(k)]}
+
+    def set_tuning(self, waves_per_cu, order):
+        self._check(load().tal_bsw_set_tuning(self._h, int(waves_per_cu), int(order)))
+
+
+class Plan:
+    """tal_bsw_plan_*: the dispatcher pipeline of one device batch captured in
+    a hipGraph; launch() scores the batch's current contents."""
+
+    def __init__(self, eng, d_pairs, d_ref, d_qer, n, w):
+        self._eng = eng
+        p = ctypes.c_void_p()
+        eng._check(load().tal_bsw_plan_create(eng._h, ctypes.c_void_p(d_pairs),
+                                              ctypes.c_void_p(d_ref), ctypes.c_void_p(d_qer),
+                                              int(n), int(w), ctypes.byref(p)))
+        self._p = p
+
+    def launch(self, stream=0):
+        self._eng._check(load().tal_bsw_plan_launch(self._p, ctypes.c_void_p(stream or None)))
+
+    def close(self):
+        if getattr(self, "_p", None):
+            load().tal_bsw_plan_destroy(self._p)
+            self._p = None
+
+    def __del__(self):
+        self.close()
 
 
 class Server:
