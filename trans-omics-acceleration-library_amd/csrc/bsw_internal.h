@@ -31,7 +31,8 @@ struct KParams {
     int32_t last_len2; /* 8-bit padding lanes take the last pair's len2 */
     int32_t state_cols;  /* state columns per wave (max ncol + slack) */
     int32_t lds_cols;    /* state columns kept in LDS (multiple of 8) */
-    int32_t slab_cols;   /* columns per global slab: query words + state past lds_cols */
+    int32_t slab_cols;   /* 64-dword rows per global slab: query words, then the
+                            state past lds_cols in whole 8-column blocks */
     int32_t query_words; /* query-code words per lane, at the start of the slab */
     int16_t match, mism, ambig;
     int16_t e_del_s, oe_del_s, e_ins_s, oe_ins_s, zdrop_s;

@@ -91,42 +91,64 @@ __device__ __forceinline__ int16_t score_of(int s1, int s2, const KParams &K)
  * 8-column block lies wholly in one of the two. */
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 typedef __attribute__((address_space(1))) uint32_t glb_u32;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) u32x4 glb_u32x4;
 
+/* One dword of state per (column, lane).  Columns below `split` live in
+ * LDS as [column][64 lanes] (lds points at this lane's dword of column 0).
+ * The rest live in the block's global slab as 8-column blocks of each lane,
+ * [block][64 lanes][8 columns] (glob points at this lane's 8 dwords of the
+ * first block), so that a lane's block is 32 contiguous bytes: a wave reads a
+ * block with two 16-byte loads per lane, and lanes that skip a block (outside
+ * their band) leave its cache lines untouched.  split is a multiple of 8, so
+ * an aligned 8-column block lies wholly in one of the two. */
 struct StateRef {
     lds_u32 *lds;
     glb_u32 *glob;
     int split;
+    __device__ __forceinline__ glb_u32 *gcol(int c) const
+    {
+        const int d = c - split;
+        return glob + (size_t)(d >> 3) * 512 + (d & 7);
+    }
     /* generic address of column c */
     __device__ __forceinline__ uint32_t *at(int c) const
     {
-        return c < split ? (uint32_t *)(lds + c * 64) : (uint32_t *)(glob + (size_t)(c - split) * 64);
+        return c < split ? (uint32_t *)(lds + c * 64) : (uint32_t *)gcol(c);
     }
+    /* distance between consecutive columns of one block, in dwords */
+    __device__ __forceinline__ int stride(int c) const { return c < split ? 64 : 1; }
     __device__ __forceinline__ uint32_t ld(int c) const { return *at(c); }
     __device__ __forceinline__ void st(int c, uint32_t v) const { *at(c) = v; }
-    /* the 8 columns of the aligned block at b (LDS and global accesses kept
-     * apart, so that neither becomes a generic access) */
-    __device__ __forceinline__ void ld8(int b, uint32_t *v) const
+    /* the 8 columns of the aligned block at b; `need` false: this lane does
+     * not read the block (its values are taken as zero) */
+    __device__ __forceinline__ void ld8(int b, uint32_t *v, bool need = true) const
     {
         if (b < split) {
             const lds_u32 *p = lds + b * 64;
 #pragma unroll
             for (int k = 0; k < 8; k++) v[k] = p[k * 64];
+        } else if (need) {
+            const glb_u32x4 *p = (const glb_u32x4 *)gcol(b);
+            const u32x4 x = p[0], y = p[1];
+            v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+            v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
         } else {
-            const glb_u32 *p = glob + (size_t)(b - split) * 64;
 #pragma unroll
-            for (int k = 0; k < 8; k++) v[k] = p[k * 64];
+            for (int k = 0; k < 8; k++) v[k] = 0u;
         }
     }
-    __device__ __forceinline__ void st8(int b, const uint32_t *v) const
+    /* `need` false: the block already holds these values for this lane */
+    __device__ __forceinline__ void st8(int b, const uint32_t *v, bool need = true) const
     {
         if (b < split) {
             lds_u32 *p = lds + b * 64;
 #pragma unroll
             for (int k = 0; k < 8; k++) p[k * 64] = v[k];
-        } else {
-            glb_u32 *p = glob + (size_t)(b - split) * 64;
-#pragma unroll
-            for (int k = 0; k < 8; k++) p[k * 64] = v[k];
+        } else if (need) {
+            glb_u32x4 *p = (glb_u32x4 *)gcol(b);
+            p[0] = (u32x4){v[0], v[1], v[2], v[3]};
+            p[1] = (u32x4){v[4], v[5], v[6], v[7]};
         }
     }
 };
@@ -183,19 +205,48 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
 
     /* ---- per-lane pair (wrapper :2787-2826 / :2092-2120) ---- */
     const bool gvalid = g >= 0;
-    const int p = gvalid ? g * GL + gl : 0;
-    const bool real = gvalid && p < n_pairs;
     int len1 = 0, len2 = 0, h0 = 0, idr = 0, idq = 0;
-    if (real) {
-        const tal_seqpair *sp = B.pairs + p;
-        idr = sp->idr;
-        idq = sp->idq;
-        len1 = sp->len1;
-        len2 = sp->len2;
-        h0 = sp->h0;
-    } else if (gvalid && BITS == 8) {
-        len2 = B.last_len2_dev ? *B.last_len2_dev : Kref.last_len2; /* :2017 */
+    bool real = false;
+    {
+        const int p = gvalid ? g * GL + gl : 0;
+        real = gvalid && p < n_pairs;
+        if (real) {
+            const tal_seqpair *sp = B.pairs + p;
+            idr = sp->idr;
+            idq = sp->idq;
+            len1 = sp->len1;
+            len2 = sp->len2;
+            h0 = sp->h0;
+        } else if (gvalid && BITS == 8) {
+            len2 = B.last_len2_dev ? *B.last_len2_dev : Kref.last_len2; /* :2017 */
+        }
     }
+    /* Lanes sorted by query length, longest first.  Everything the group
+     * shares (rows, columns, range, narrowing, scans, exits) is a max, min,
+     * any or all over its lanes, so the order of pairs across lanes changes
+     * nothing; per-lane bit masks (stale column, dependence, final F) are
+     * kept in the reference's lane order through `orig`/`inv`.  Sorted, the
+     * lanes inside their band at a column are a prefix of the group, so the
+     * state blocks of lanes outside it can be skipped whole. */
+    int orig = gl; /* the pair's lane in the reference's group */
+    int inv = lane; /* the lane now holding the pair of reference lane (lane % GL) */
+    {
+        int rank = 0;
+        for (int o = 0; o < GL; o++) {
+            const int ol = __shfl(len2, half * GL + o, 64);
+            rank += (ol > len2 || (ol == len2 && o < gl)) ? 1 : 0;
+        }
+        const int to = (half * GL + rank) * 4;
+        idr = __builtin_amdgcn_ds_permute(to, idr);
+        idq = __builtin_amdgcn_ds_permute(to, idq);
+        len1 = __builtin_amdgcn_ds_permute(to, len1);
+        len2 = __builtin_amdgcn_ds_permute(to, len2);
+        h0 = __builtin_amdgcn_ds_permute(to, h0);
+        real = __builtin_amdgcn_ds_permute(to, real ? 1 : 0) != 0;
+        orig = __builtin_amdgcn_ds_permute(to, gl);
+        inv = __builtin_amdgcn_ds_permute((half * GL + orig) * 4, lane);
+    }
+    const int p = gvalid ? g * GL + orig : 0;
     const int nrow = group_max<GL>(len1);
     const int ncol = group_max<GL>(len2);
 
@@ -225,7 +276,10 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
     const uint8_t *qseq = B.qer + (int64_t)idq;
 
     /* ---- H_h row -1 and F (:2867-2879, :3034) ---- */
-    const bool stale = BITS == 16 && B.stale_in && gvalid && ((B.stale_in[g] >> gl) & 1u);
+    const bool stale = BITS == 16 && B.stale_in && gvalid && ((B.stale_in[g] >> orig) & 1u);
+    /* nzhi: the highest state column of this lane that may hold a nonzero
+     * value; blocks above it and above the band need no store */
+    int nzhi = -1;
     {
         int16_t hv = h0w;
         for (int c = 0; c <= ncol; c++) {
@@ -234,7 +288,9 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
             else if (c >= 2)
                 hv = (c < ncol) ? smax16(sub16(hv, e_ins_s), 0) : (int16_t)0;
             const int16_t f = (c < ncol) ? (int16_t)0 : (int16_t)(stale ? 1 : 0);
-            S.st(c, pack(hv, f));
+            const uint32_t v = pack(hv, f);
+            S.st(c, v);
+            if (v != 0u) nzhi = c;
         }
     }
     /* ---- query codes for the fast path ---- */
@@ -398,16 +454,25 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
             /* 8-column blocks aligned to multiples of 8 (one query word,
              * static code positions).  Edge blocks guard each column with
              * beg <= j < end; middle blocks run unguarded. */
+            /* This lane reads a block iff it meets [lo, tl] and stores it iff
+             * it meets [lo, max(tl, nzhi)], lo = min(hd, pend + 1).  Below
+             * lo the state is zero: the previous row stored [pbeg, pend] with
+             * its band mask and the trim zeroed [beg, hd) where head moved, so
+             * only column pend + 1 (never stored last row) can still hold an
+             * older value there.  Above tl the inputs only feed masked cells,
+             * and memory above nzhi already holds the masked zeros. */
+            const int lo = min(hd, pend + 1);
+            const int sthi = max(tl, nzhi);
             if (beg < end) {
                 const int b0 = beg & ~7;
                 uint32_t sv[8], sn[8];
-                S.ld8(b0, sv);
+                S.ld8(b0, sv, b0 <= tl && b0 + 7 >= lo);
                 uint32_t qw = Q[(size_t)(b0 >> 3) * 64];
                 for (int b = b0; b < end; b += 8) {
                     const bool more = b + 8 < end;
                     uint32_t qn = 0;
                     if (more) {
-                        S.ld8(b + 8, sn);
+                        S.ld8(b + 8, sn, b + 8 <= tl && b + 15 >= lo);
                         qn = Q[(size_t)((b >> 3) + 1) * 64];
                     }
                     /* scores of columns b+0,2,4,6 (sce) and b+1,3,5,7 (sco) */
@@ -424,16 +489,17 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                             kb = max(kb, (r[k] << 16) | (uint32_t)k);
                             any |= r[k];
                         }
-                        S.st8(b, r);
+                        S.st8(b, r, b <= sthi && b + 7 >= lo);
                     } else {
                         uint32_t *ep = S.at(b);
+                        const int cs = S.stride(b);
 #pragma unroll
                         for (int k = 0; k < 8; k++)
                             if (b + k >= beg && b + k < end) {
                                 const uint32_t st =
                                     cell(sv[k], score((k & 1) ? sco : sce, k >> 1));
                                 const uint32_t r = rel0 + k < lim1 ? st : 0u;
-                                ep[k * 64] = r;
+                                ep[k * cs] = r;
                                 any |= r;
                                 if (b + k > beg) kb = max(kb, (r << 16) | (uint32_t)k);
                             }
@@ -529,6 +595,7 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
         pend = j;
         phd = hd;
         ptl = tl;
+        nzhi = max(nzhi, min(tl, j)); /* this row's nonzero stores lie in [hd, tl] */
         /* gscore (:3176-3195): the cell of the last query column is updated
          * when it was computed this row, the lane is live and tail == qlen;
          * then H_h[qlen] holds it (head <= qlen, or the lane would not be
@@ -644,7 +711,9 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                 const int16_t tz = smin16(add16(Wd::v(c >= bs ? c : bs - 1), TWO), qlen);
                 dec = tnz != tz;
             }
-            dep |= ballot(dec) & gmask;
+            /* bit of the reference's lane: read `dec` from the lane holding it */
+            const int dec_o = __builtin_amdgcn_ds_bpermute(inv * 4, dec ? 1 : 0);
+            dep |= ballot(dec_o != 0) & gmask;
         }
 
         /* tail scan result */
@@ -672,7 +741,8 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
         }
         uint32_t *fm = B.fmask + B.fmask_off[g];
         for (int c = 0; c < ncol; c++) {
-            const uint64_t nz = ballot((S.ld(c) >> 16) != 0u);
+            const int nz_o = __builtin_amdgcn_ds_bpermute(inv * 4, (S.ld(c) >> 16) != 0u ? 1 : 0);
+            const uint64_t nz = ballot(nz_o != 0);
             if (gl == 0) fm[c] = (uint32_t)(nz >> (32 * half));
         }
     }
@@ -688,10 +758,10 @@ __global__ __launch_bounds__(64) void bsw_main(KParams K, KBuffers B)
     const int half = lane / Wd::GL;
     /* LDS: state columns [0, lds_cols).  The block's global slab holds the
      * query-code words, then state columns [lds_cols, state_cols). */
-    uint32_t *slab = B.slabs + (size_t)blockIdx.x * K.slab_cols * 64 + lane;
-    StateRef S{(lds_u32 *)(lds_state + lane), (glb_u32 *)(slab + (size_t)K.query_words * 64),
-               K.lds_cols};
-    uint32_t *Q = slab;
+    uint32_t *slab = B.slabs + (size_t)blockIdx.x * K.slab_cols * 64;
+    StateRef S{(lds_u32 *)(lds_state + lane),
+               (glb_u32 *)(slab + (size_t)K.query_words * 64 + (size_t)lane * 8), K.lds_cols};
+    uint32_t *Q = slab + lane;
     if (B.abort_flag && *B.abort_flag != 0) return;
     const int ntasks = B.ntasks_dev ? *B.ntasks_dev : K.ntasks;
     for (;;) {
