@@ -108,8 +108,13 @@ struct StateRef {
     int split;
     __device__ __forceinline__ glb_u32 *gcol(int c) const
     {
-        const int d = c - split;
-        return glob + (size_t)(d >> 3) * 512 + (d & 7);
+        const uint32_t d = (uint32_t)(c - split);
+        return glob + ((d & ~7u) << 6) + (d & 7u);
+    }
+    /* the lane's 8 dwords of the aligned block at b (b >= split) */
+    __device__ __forceinline__ glb_u32 *gblk(int b) const
+    {
+        return glob + ((uint32_t)(b - split) << 6);
     }
     /* generic address of column c */
     __device__ __forceinline__ uint32_t *at(int c) const
@@ -129,7 +134,7 @@ struct StateRef {
 #pragma unroll
             for (int k = 0; k < 8; k++) v[k] = p[k * 64];
         } else if (need) {
-            const glb_u32x4 *p = (const glb_u32x4 *)gcol(b);
+            const glb_u32x4 *p = (const glb_u32x4 *)gblk(b);
             const u32x4 x = p[0], y = p[1];
             v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
             v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
@@ -146,7 +151,7 @@ struct StateRef {
 #pragma unroll
             for (int k = 0; k < 8; k++) p[k * 64] = v[k];
         } else if (need) {
-            glb_u32x4 *p = (glb_u32x4 *)gcol(b);
+            glb_u32x4 *p = (glb_u32x4 *)gblk(b);
             p[0] = (u32x4){v[0], v[1], v[2], v[3]};
             p[1] = (u32x4){v[4], v[5], v[6], v[7]};
         }
@@ -294,9 +299,10 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
         }
     }
     /* ---- query codes for the fast path ---- */
-    /* every int8 score fits the per-row byte table; the fast path only
-     * needs query bytes 0-4 */
-    bool fast = true;
+    /* Every int8 score fits the per-row byte table.  The fast path needs
+     * query bytes 0-4 and nonnegative insertion penalties (it relies on a
+     * run of zero cells keeping E at zero). */
+    bool fast = oe_ins_s >= 0 && e_ins_s >= 0;
     {
         bool ok = true;
         const int nwords = (ncol + 1 + 7) / 8 + 1;
