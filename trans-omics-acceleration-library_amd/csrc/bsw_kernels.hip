@@ -472,6 +472,12 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
             if (beg < end) {
                 const int b0 = beg & ~7;
                 uint32_t sv[8], sn[8];
+                /* A middle block's stores are issued one block late, after
+                 * the next block's loads: the wait for those loads at the end
+                 * of a block then does not also wait for its own stores. */
+                uint32_t pr[8];
+                int pb = -1;
+                bool pneed = false;
                 S.ld8(b0, sv, b0 <= tl && b0 + 7 >= lo);
                 uint32_t qw = Q[(size_t)(b0 >> 3) * 64];
                 for (int b = b0; b < end; b += 8) {
@@ -480,6 +486,10 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                     if (more) {
                         S.ld8(b + 8, sn, b + 8 <= tl && b + 15 >= lo);
                         qn = Q[(size_t)((b >> 3) + 1) * 64];
+                    }
+                    if (pb >= 0) {
+                        S.st8(pb, pr, pneed);
+                        pb = -1;
                     }
                     /* scores of columns b+0,2,4,6 (sce) and b+1,3,5,7 (sco) */
                     const uint32_t sce = __builtin_amdgcn_perm(thi, tlo, qw & 0x0F0F0F0Fu);
@@ -495,7 +505,10 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                             kb = max(kb, (r[k] << 16) | (uint32_t)k);
                             any |= r[k];
                         }
-                        S.st8(b, r, b <= sthi && b + 7 >= lo);
+#pragma unroll
+                        for (int k = 0; k < 8; k++) pr[k] = r[k];
+                        pb = b;
+                        pneed = b <= sthi && b + 7 >= lo;
                     } else {
                         uint32_t *ep = S.at(b);
                         const int cs = S.stride(b);
@@ -521,6 +534,7 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                         qw = qn;
                     }
                 }
+                if (pb >= 0) S.st8(pb, pr, pneed);
                 j = end;
             }
             h10 = (int16_t)hprev;
