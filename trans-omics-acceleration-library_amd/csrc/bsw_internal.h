@@ -31,8 +31,9 @@ struct KParams {
     int32_t last_len2; /* 8-bit padding lanes take the last pair's len2 */
     int32_t state_cols;  /* state columns per wave (max ncol + slack) */
     int32_t lds_cols;    /* state columns kept in LDS (multiple of 8) */
-    int32_t slab_cols;   /* 64-dword rows per global slab: query words, then the
-                            state past lds_cols in whole 8-column blocks */
+    int32_t slab_cols;   /* 64-dword rows per global slab: query words, the state
+                            past lds_cols in whole 8-column blocks, and one
+                            8-column scratch block */
     int32_t query_words; /* query-code words per lane, at the start of the slab */
     int16_t match, mism, ambig;
     int16_t e_del_s, oe_del_s, e_ins_s, oe_ins_s, zdrop_s;
@@ -45,7 +46,8 @@ struct KBuffers {
     const int32_t *task_groups; /* ntasks * groups-per-wave ids, -1 = none;
                                    null = identity order */
     int32_t *task_counter;      /* persistent-grid work counter */
-    uint32_t *slabs;            /* global state slabs when not in LDS */
+    uint32_t *slabs;            /* per-wave slabs: query words, state past LDS */
+    const uint32_t *zero;       /* 32 zero bytes (the read target of skipped blocks) */
     /* 16-bit stale-column bookkeeping (all null for the 8-bit path) */
     const uint32_t *stale_in; /* per group: lanes whose F[ncol] is nonzero */
     uint32_t *dep_out;        /* per group: lanes whose tail depended on it */

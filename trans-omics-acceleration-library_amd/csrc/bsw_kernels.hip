@@ -94,67 +94,44 @@ typedef __attribute__((address_space(1))) uint32_t glb_u32;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) u32x4 glb_u32x4;
 
-/* One dword of state per (column, lane).  Columns below `split` live in
- * LDS as [column][64 lanes] (lds points at this lane's dword of column 0).
- * The rest live in the block's global slab as 8-column blocks of each lane,
- * [block][64 lanes][8 columns] (glob points at this lane's 8 dwords of the
- * first block), so that a lane's block is 32 contiguous bytes: a wave reads a
- * block with two 16-byte loads per lane, and lanes that skip a block (outside
- * their band) leave its cache lines untouched.  split is a multiple of 8, so
- * an aligned 8-column block lies wholly in one of the two. */
+/* One dword of state per (column, lane), in 8-column blocks of each lane:
+ * [block][64 lanes][8 columns], so that a lane's block is 32 contiguous
+ * bytes.  Blocks below `split` live in LDS, the rest in the wave's global
+ * slab.  Every access goes through a generic address, so one load or store
+ * instruction serves both places and a lane that skips a block is pointed
+ * at a zero block (loads) or its own scratch block (stores) instead of being
+ * branched around: the memory instructions of a block are then the same for
+ * every lane, and lanes outside their band touch no state cache lines.
+ * split is a multiple of 8, so an aligned block lies wholly in one place. */
 struct StateRef {
-    lds_u32 *lds;
-    glb_u32 *glob;
+    uint32_t *lds;         /* this lane's 8 dwords of LDS block 0 */
+    uint32_t *glob;        /* this lane's 8 dwords of the global block at split */
+    const uint32_t *zero;  /* 32 zero bytes */
+    uint32_t *trash;       /* 32 bytes of this lane that nothing reads */
     int split;
-    __device__ __forceinline__ glb_u32 *gcol(int c) const
+    __device__ __forceinline__ uint32_t *blk(int b) const
     {
-        const uint32_t d = (uint32_t)(c - split);
-        return glob + ((d & ~7u) << 6) + (d & 7u);
+        return b < split ? lds + ((uint32_t)b << 6) : glob + ((uint32_t)(b - split) << 6);
     }
-    /* the lane's 8 dwords of the aligned block at b (b >= split) */
-    __device__ __forceinline__ glb_u32 *gblk(int b) const
-    {
-        return glob + ((uint32_t)(b - split) << 6);
-    }
-    /* generic address of column c */
-    __device__ __forceinline__ uint32_t *at(int c) const
-    {
-        return c < split ? (uint32_t *)(lds + c * 64) : (uint32_t *)gcol(c);
-    }
-    /* distance between consecutive columns of one block, in dwords */
-    __device__ __forceinline__ int stride(int c) const { return c < split ? 64 : 1; }
+    __device__ __forceinline__ uint32_t *at(int c) const { return blk(c & ~7) + (c & 7); }
+    __device__ __forceinline__ int stride(int) const { return 1; }
     __device__ __forceinline__ uint32_t ld(int c) const { return *at(c); }
     __device__ __forceinline__ void st(int c, uint32_t v) const { *at(c) = v; }
     /* the 8 columns of the aligned block at b; `need` false: this lane does
      * not read the block (its values are taken as zero) */
     __device__ __forceinline__ void ld8(int b, uint32_t *v, bool need = true) const
     {
-        if (b < split) {
-            const lds_u32 *p = lds + b * 64;
-#pragma unroll
-            for (int k = 0; k < 8; k++) v[k] = p[k * 64];
-        } else if (need) {
-            const glb_u32x4 *p = (const glb_u32x4 *)gblk(b);
-            const u32x4 x = p[0], y = p[1];
-            v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
-            v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
-        } else {
-#pragma unroll
-            for (int k = 0; k < 8; k++) v[k] = 0u;
-        }
+        const u32x4 *p = (const u32x4 *)(need ? (const uint32_t *)blk(b) : zero);
+        const u32x4 x = p[0], y = p[1];
+        v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+        v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
     }
     /* `need` false: the block already holds these values for this lane */
     __device__ __forceinline__ void st8(int b, const uint32_t *v, bool need = true) const
     {
-        if (b < split) {
-            lds_u32 *p = lds + b * 64;
-#pragma unroll
-            for (int k = 0; k < 8; k++) p[k * 64] = v[k];
-        } else if (need) {
-            glb_u32x4 *p = (glb_u32x4 *)gblk(b);
-            p[0] = (u32x4){v[0], v[1], v[2], v[3]};
-            p[1] = (u32x4){v[4], v[5], v[6], v[7]};
-        }
+        u32x4 *p = (u32x4 *)(need ? blk(b) : trash);
+        p[0] = (u32x4){v[0], v[1], v[2], v[3]};
+        p[1] = (u32x4){v[4], v[5], v[6], v[7]};
     }
 };
 
@@ -778,9 +755,12 @@ __global__ __launch_bounds__(64) void bsw_main(KParams K, KBuffers B)
     const int half = lane / Wd::GL;
     /* LDS: state columns [0, lds_cols).  The block's global slab holds the
      * query-code words, then state columns [lds_cols, state_cols). */
+    /* slab: query words [word][64 lanes], the global state blocks, and one
+     * scratch block (the target of skipped stores) */
     uint32_t *slab = B.slabs + (size_t)blockIdx.x * K.slab_cols * 64;
-    StateRef S{(lds_u32 *)(lds_state + lane),
-               (glb_u32 *)(slab + (size_t)K.query_words * 64 + (size_t)lane * 8), K.lds_cols};
+    StateRef S{(uint32_t *)lds_state + lane * 8,
+               slab + (size_t)K.query_words * 64 + (size_t)lane * 8, B.zero,
+               slab + (size_t)(K.slab_cols - 8) * 64 + (size_t)lane * 8, K.lds_cols};
     uint32_t *Q = slab + lane;
     if (B.abort_flag && *B.abort_flag != 0) return;
     const int ntasks = B.ntasks_dev ? *B.ntasks_dev : K.ntasks;

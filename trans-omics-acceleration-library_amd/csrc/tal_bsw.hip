@@ -105,7 +105,8 @@ struct tal_bsw {
     int waves_per_cu = 8;  /* target waves per CU for the LDS/global split */
     int order = 1;         /* task order: 0 = input order, 1 = longest groups first */
     DevBuf image; /* IMAGE_COLS u32: F[c] nonzero lanes of the 16-bit scratch */
-    DevBuf zero_image; /* the image of a freshly constructed object */
+    DevBuf zero_image; /* the image of a freshly constructed object; its first
+                          bytes double as the kernels' zero block */
     PathBufs b16, b8;
     /* dispatcher scratch */
     DevBuf flag, idx_all, idx_sel, nsel, pairs8, pairs16, cubtmp;
@@ -320,8 +321,8 @@ Sizing sizing(const tal_bsw *h, int max_ncol)
     const size_t max_cols = h->lds_per_block / 256;
     if ((size_t)lds_cols > max_cols) lds_cols = (int)(max_cols & ~(size_t)7);
     z.lds_cols = lds_cols;
-    /* global state in whole 8-column blocks */
-    z.slab_cols = z.qwords + ((std::max(0, z.state_cols - lds_cols) + 7) & ~7);
+    /* global state in whole 8-column blocks, then one scratch block */
+    z.slab_cols = z.qwords + ((std::max(0, z.state_cols - lds_cols) + 7) & ~7) + 8;
     z.lds_bytes = (size_t)lds_cols * 64 * sizeof(uint32_t);
     return z;
 }
@@ -442,6 +443,7 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
     HIPCHK(h, pb.slabs.ensure((size_t)grid * slab_cols * 64 * sizeof(uint32_t)),
            "alloc state slabs");
     B.slabs = pb.slabs.as<uint32_t>();
+    B.zero = h->zero_image.as<uint32_t>();
     if (bits == 16) {
         HIPCHK(h, pb.dep.ensure(sizeof(uint32_t) * G), "alloc dep");
         HIPCHK(h, pb.wrote.ensure(G), "alloc wrote");
@@ -628,6 +630,7 @@ int plan_width(tal_bsw *h, tal_bsw_plan *P, int bits, hipStream_t s)
     HIPCHK(h, pb.slabs.ensure((size_t)W.grid * W.z.slab_cols * 64 * sizeof(uint32_t)),
            "alloc state slabs");
     B.slabs = pb.slabs.as<uint32_t>();
+    B.zero = h->zero_image.as<uint32_t>();
     if (bits == 8) {
         B.last_len2_dev = P->ctl.as<int32_t>() + 1;
     } else {
