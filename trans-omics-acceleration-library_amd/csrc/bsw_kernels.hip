@@ -21,6 +21,8 @@
  * signed max/compare and unsigned min then behave exactly like epi8 on the
  * high byte, so one code path serves both widths.
  */
+#include <type_traits>
+
 #include "bsw_internal.h"
 
 namespace talbsw {
@@ -93,45 +95,67 @@ typedef __attribute__((address_space(3))) uint32_t lds_u32;
 typedef __attribute__((address_space(1))) uint32_t glb_u32;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) u32x4 glb_u32x4;
+typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
 
 /* One dword of state per (column, lane), in 8-column blocks of each lane:
  * [block][64 lanes][8 columns], so that a lane's block is 32 contiguous
  * bytes.  Blocks below `split` live in LDS, the rest in the wave's global
- * slab.  Every access goes through a generic address, so one load or store
- * instruction serves both places and a lane that skips a block is pointed
- * at a zero block (loads) or its own scratch block (stores) instead of being
- * branched around: the memory instructions of a block are then the same for
- * every lane, and lanes outside their band touch no state cache lines.
- * split is a multiple of 8, so an aligned block lies wholly in one place. */
+ * slab; split is a multiple of 8, so an aligned block lies wholly in one
+ * place.  The row sweep runs over the LDS blocks and then over the global
+ * blocks, each with its own kind of memory instruction.  In the global part a
+ * lane that skips a block is pointed at a zero block (loads) or its own
+ * scratch block (stores) instead of being branched around: every lane then
+ * issues the same memory instructions, so the waits for them stay exact, and
+ * lanes outside their band touch no state cache lines. */
 struct StateRef {
-    uint32_t *lds;         /* this lane's 8 dwords of LDS block 0 */
-    uint32_t *glob;        /* this lane's 8 dwords of the global block at split */
-    const uint32_t *zero;  /* 32 zero bytes */
-    uint32_t *trash;       /* 32 bytes of this lane that nothing reads */
+    lds_u32 *lds;          /* this lane's 8 dwords of LDS block 0 */
+    glb_u32 *glob;         /* this lane's 8 dwords of the global block at split */
+    const glb_u32 *zero;   /* 32 zero bytes */
+    glb_u32 *trash;        /* 32 bytes of this lane that nothing reads */
     int split;
-    __device__ __forceinline__ uint32_t *blk(int b) const
+    /* generic address of column c (single accesses outside the sweep) */
+    __device__ __forceinline__ uint32_t *at(int c) const
     {
-        return b < split ? lds + ((uint32_t)b << 6) : glob + ((uint32_t)(b - split) << 6);
+        const int b = c & ~7;
+        return b < split ? (uint32_t *)(lds + ((uint32_t)b << 6) + (c & 7))
+                         : (uint32_t *)(glob + ((uint32_t)(b - split) << 6) + (c & 7));
     }
-    __device__ __forceinline__ uint32_t *at(int c) const { return blk(c & ~7) + (c & 7); }
-    __device__ __forceinline__ int stride(int) const { return 1; }
     __device__ __forceinline__ uint32_t ld(int c) const { return *at(c); }
     __device__ __forceinline__ void st(int c, uint32_t v) const { *at(c) = v; }
-    /* the 8 columns of the aligned block at b; `need` false: this lane does
-     * not read the block (its values are taken as zero) */
-    __device__ __forceinline__ void ld8(int b, uint32_t *v, bool need = true) const
+    __device__ __forceinline__ lds_u32 *lblk(int b) const { return lds + ((uint32_t)b << 6); }
+    __device__ __forceinline__ glb_u32 *gblk(int b) const
     {
-        const u32x4 *p = (const u32x4 *)(need ? (const uint32_t *)blk(b) : zero);
-        const u32x4 x = p[0], y = p[1];
+        return glob + ((uint32_t)(b - split) << 6);
+    }
+    /* the 8 columns of the aligned block at b; G: the block is global */
+    template <bool G> __device__ __forceinline__ void ld8(int b, uint32_t *v, bool need) const
+    {
+        u32x4 x, y;
+        if (G) {
+            const glb_u32x4 *p = (const glb_u32x4 *)(need ? (const glb_u32 *)gblk(b) : zero);
+            x = p[0];
+            y = p[1];
+        } else {
+            const lds_u32x4 *p = (const lds_u32x4 *)lblk(b);
+            x = p[0];
+            y = p[1];
+        }
         v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
         v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
     }
     /* `need` false: the block already holds these values for this lane */
-    __device__ __forceinline__ void st8(int b, const uint32_t *v, bool need = true) const
+    template <bool G> __device__ __forceinline__ void st8(int b, const uint32_t *v, bool need) const
     {
-        u32x4 *p = (u32x4 *)(need ? blk(b) : trash);
-        p[0] = (u32x4){v[0], v[1], v[2], v[3]};
-        p[1] = (u32x4){v[4], v[5], v[6], v[7]};
+        const u32x4 x = {v[0], v[1], v[2], v[3]}, y = {v[4], v[5], v[6], v[7]};
+        if (G) {
+            glb_u32x4 *p = (glb_u32x4 *)(need ? gblk(b) : trash);
+            p[0] = x;
+            p[1] = y;
+        } else {
+            lds_u32x4 *p = (lds_u32x4 *)lblk(b);
+            p[0] = x;
+            p[1] = y;
+        }
     }
 };
 
@@ -446,56 +470,43 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
              * and memory above nzhi already holds the masked zeros. */
             const int lo = min(hd, pend + 1);
             const int sthi = max(tl, nzhi);
-            if (beg < end) {
-                const int b0 = beg & ~7;
-                uint32_t sv[8], sn[8];
-                /* A middle block's stores are issued one block late, after
-                 * the next block's loads: the wait for those loads at the end
-                 * of a block then does not also wait for its own stores. */
-                uint32_t pr[8];
-                int pb = -1;
-                bool pneed = false;
-                S.ld8(b0, sv, b0 <= tl && b0 + 7 >= lo);
-                uint32_t qw = Q[(size_t)(b0 >> 3) * 64];
-                for (int b = b0; b < end; b += 8) {
-                    const bool more = b + 8 < end;
-                    uint32_t qn = 0;
-                    if (more) {
-                        S.ld8(b + 8, sn, b + 8 <= tl && b + 15 >= lo);
-                        qn = Q[(size_t)((b >> 3) + 1) * 64];
-                    }
-                    if (pb >= 0) {
-                        S.st8(pb, pr, pneed);
-                        pb = -1;
-                    }
+            /* the row's blocks: [b0, split) in LDS, then the global ones */
+            uint32_t pr[8];
+            int pb = -1;
+            bool pneed = false;
+            auto sweep = [&](auto gtag, int from, int to) {
+                constexpr bool G = decltype(gtag)::value;
+                /* one block: compute and store (a middle block's stores are
+                 * left pending in pr/pb), fold its row-max key and nonzero
+                 * flag into the row's */
+                auto do_block = [&](int b, const uint32_t *sv, uint32_t qw) {
                     /* scores of columns b+0,2,4,6 (sce) and b+1,3,5,7 (sco) */
                     const uint32_t sce = __builtin_amdgcn_perm(thi, tlo, qw & 0x0F0F0F0Fu);
-                    const uint32_t sco = __builtin_amdgcn_perm(thi, tlo, (qw >> 4) & 0x0F0F0F0Fu);
+                    const uint32_t sco =
+                        __builtin_amdgcn_perm(thi, tlo, (qw >> 4) & 0x0F0F0F0Fu);
                     const uint32_t rel0 = (uint32_t)(b - hd);
                     uint32_t any = 0, kb = 0;
                     if (b > beg && b + 8 <= end) {
-                        uint32_t r[8];
 #pragma unroll
                         for (int k = 0; k < 8; k++) {
                             const uint32_t st = cell(sv[k], score((k & 1) ? sco : sce, k >> 1));
-                            r[k] = rel0 + k < lim1 ? st : 0u;
-                            kb = max(kb, (r[k] << 16) | (uint32_t)k);
-                            any |= r[k];
+                            pr[k] = rel0 + k < lim1 ? st : 0u;
+                            kb = max(kb, (pr[k] << 16) | (uint32_t)k);
+                            any |= pr[k];
                         }
-#pragma unroll
-                        for (int k = 0; k < 8; k++) pr[k] = r[k];
                         pb = b;
                         pneed = b <= sthi && b + 7 >= lo;
                     } else {
-                        uint32_t *ep = S.at(b);
-                        const int cs = S.stride(b);
 #pragma unroll
                         for (int k = 0; k < 8; k++)
                             if (b + k >= beg && b + k < end) {
                                 const uint32_t st =
                                     cell(sv[k], score((k & 1) ? sco : sce, k >> 1));
                                 const uint32_t r = rel0 + k < lim1 ? st : 0u;
-                                ep[k * cs] = r;
+                                if (G)
+                                    S.gblk(b)[k] = r;
+                                else
+                                    S.lblk(b)[k] = r;
                                 any |= r;
                                 if (b + k > beg) kb = max(kb, (r << 16) | (uint32_t)k);
                             }
@@ -505,13 +516,49 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                         bfirst = min(bfirst, b);
                         blast = b;
                     }
-                    if (more) {
-#pragma unroll
-                        for (int k = 0; k < 8; k++) sv[k] = sn[k];
-                        qw = qn;
+                };
+                /* the next block's loads, issued for every lane (a lane past
+                 * its part reads the zero block or its first LDS block) */
+                auto load = [&](int b, bool more, uint32_t *v, uint32_t &q) {
+                    S.ld8<G>(more ? b : from, v, more && b <= tl && b + 7 >= lo);
+                    q = Q[(size_t)((more ? b : from) >> 3) * 64];
+                };
+                /* the pending stores (to the scratch block when there are
+                 * none; LDS stores are issued only when pending) */
+                auto flush = [&]() {
+                    if (G)
+                        S.st8<true>(pb, pr, pb >= 0 && pneed);
+                    else if (pb >= 0)
+                        S.st8<false>(pb, pr, true);
+                    pb = -1;
+                };
+                /* Two buffers used in turn: a block's loads land straight in
+                 * the registers its computation reads, so the wait for them
+                 * falls just before that computation, a whole block after
+                 * they were issued, and never covers later stores. */
+                if (from < to) {
+                    uint32_t A[8], Bv[8];
+                    uint32_t qa = 0, qb = 0;
+                    load(from, true, A, qa);
+                    for (int b = from;;) {
+                        load(b + 8, b + 8 < to, Bv, qb);
+                        flush();
+                        do_block(b, A, qa);
+                        b += 8;
+                        if (b >= to) break;
+                        load(b + 8, b + 8 < to, A, qa);
+                        flush();
+                        do_block(b, Bv, qb);
+                        b += 8;
+                        if (b >= to) break;
                     }
                 }
-                if (pb >= 0) S.st8(pb, pr, pneed);
+                flush();
+            };
+            if (beg < end) {
+                const int b0 = beg & ~7;
+                sweep(std::false_type{}, b0, min(end, S.split));
+                sweep(std::true_type{}, max(b0, S.split), end);
                 j = end;
             }
             h10 = (int16_t)hprev;
@@ -758,9 +805,10 @@ __global__ __launch_bounds__(64) void bsw_main(KParams K, KBuffers B)
     /* slab: query words [word][64 lanes], the global state blocks, and one
      * scratch block (the target of skipped stores) */
     uint32_t *slab = B.slabs + (size_t)blockIdx.x * K.slab_cols * 64;
-    StateRef S{(uint32_t *)lds_state + lane * 8,
-               slab + (size_t)K.query_words * 64 + (size_t)lane * 8, B.zero,
-               slab + (size_t)(K.slab_cols - 8) * 64 + (size_t)lane * 8, K.lds_cols};
+    StateRef S{(lds_u32 *)lds_state + lane * 8,
+               (glb_u32 *)(slab + (size_t)K.query_words * 64 + (size_t)lane * 8),
+               (const glb_u32 *)B.zero,
+               (glb_u32 *)(slab + (size_t)(K.slab_cols - 8) * 64 + (size_t)lane * 8), K.lds_cols};
     uint32_t *Q = slab + lane;
     if (B.abort_flag && *B.abort_flag != 0) return;
     const int ntasks = B.ntasks_dev ? *B.ntasks_dev : K.ntasks;
