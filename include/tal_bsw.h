@@ -263,6 +263,53 @@ int tal_bsw_server_get_stats(tal_bsw_server *s, tal_bsw_server_stats *out);
 /* Finishes every queued request, then releases the server. */
 void tal_bsw_server_destroy(tal_bsw_server *s);
 
+/* ---------------------------------------------------------------------------
+ * Pair files.  The text format is bench-bsw's (benchmarks/bench-bsw.cpp:
+ * 117-169, loadPairs): three lines per pair -- h0 in decimal, the reference,
+ * the query -- each base an ASCII character whose code is the character
+ * minus '0' ('0'-'3' = ACGT, '4' = N).
+ *
+ * tal_bsw_pairfile_read reads the next pairs (up to max_pairs) the way
+ * loadPairs fills its buffers: pair k's reference at ref + k * ref_slot and
+ * its query at qer + k * qer_slot (bench-bsw uses slots of MAX_SEQ_LEN_REF =
+ * 256 and MAX_SEQ_LEN_QER = 128 bytes, bandedSWA.h:47-48), idr/idq those
+ * offsets, id = k, the pass-through and output fields -1.  *n_read is 0 at the
+ * end of the file; an h0 line with no reference after it also ends the file,
+ * as in bench-bsw.  A sequence longer than its slot minus two is TAL_ERANGE
+ * (bench-bsw's fgets stores the newline and a NUL in the slot, and would split
+ * a longer line); a reference with no query line after it is TAL_EINVAL.
+ * tal_bsw_pairfile_write writes pairs in the same format. */
+typedef struct tal_bsw_pairfile tal_bsw_pairfile;
+
+int tal_bsw_pairfile_open(const char *path, tal_bsw_pairfile **out);
+int tal_bsw_pairfile_read(tal_bsw_pairfile *f, tal_seqpair *pairs, uint8_t *ref,
+                          size_t ref_slot, uint8_t *qer, size_t qer_slot,
+                          int32_t max_pairs, int32_t *n_read);
+void tal_bsw_pairfile_close(tal_bsw_pairfile *f);
+int tal_bsw_pairfile_write(const char *path, const tal_seqpair *pairs,
+                           const uint8_t *ref, const uint8_t *qer, int32_t n);
+
+/* Packed batch files (no reference counterpart): a 64-byte header, the
+ * SeqPair records, the reference bytes, the query bytes.  tal_bsw_pack_map
+ * maps a file privately (copy-on-write), so outputs may be written into the
+ * records and the sequences passed straight to tal_bsw_get_scores_*. */
+typedef struct tal_bsw_pack {
+    tal_seqpair *pairs;
+    int64_t n;
+    uint8_t *ref;
+    size_t ref_len;
+    uint8_t *qer;
+    size_t qer_len;
+    void *mapping;
+    size_t mapping_len;
+} tal_bsw_pack;
+
+int tal_bsw_pack_write(const char *path, const tal_seqpair *pairs, int64_t n,
+                       const uint8_t *ref, size_t ref_len, const uint8_t *qer,
+                       size_t qer_len);
+int tal_bsw_pack_map(const char *path, tal_bsw_pack *out);
+void tal_bsw_pack_unmap(tal_bsw_pack *p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,7 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TAL_BSW_ARCH", "gfx950")
 
-HIP_SOURCES = ["bsw_kernels.hip", "tal_bsw.hip", "bsw_server.hip"]
+HIP_SOURCES = ["bsw_kernels.hip", "tal_bsw.hip", "bsw_server.hip", "bsw_pairfile.cpp"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -31,12 +31,21 @@ def build_hip(out=None, extra=()):
     out = out or os.path.join(HERE, "libtal_bsw.so")
     objs = []
     for src in HIP_SOURCES:
-        obj = os.path.join(CSRC, src.replace(".hip", ".o"))
+        obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
         _run([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", out, "-lpthread"])
     for o in objs:
         os.remove(o)
+    return out
+
+
+def build_cli(lib=None, out=None):
+    """tal-bench-bsw: bench-bsw's command line on the engine (C++ shim)."""
+    out = out or os.path.join(HERE, "tal-bench-bsw")
+    _run(["g++", "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}",
+          os.path.join(CSRC, "tal_bench_bsw.cpp"), "-o", out, f"-L{HERE}", "-ltal_bsw",
+          "-Wl,-rpath,$ORIGIN"])
     return out
 
 
@@ -50,6 +59,7 @@ def build_synth(out=None):
 def build_all():
     build_synth()
     build_hip()
+    build_cli()
 
 
 if __name__ == "__main__":

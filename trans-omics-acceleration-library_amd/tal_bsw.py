@@ -30,7 +30,10 @@ EXPORTS = ["tal_bsw_create", "tal_bsw_destroy", "tal_bsw_last_error",
            "tal_bsw_plan_create", "tal_bsw_plan_launch", "tal_bsw_plan_destroy",
            "tal_bsw_autotune", "tal_bsw_set_tuning",
            "tal_bsw_server_create", "tal_bsw_server_submit", "tal_bsw_server_wait",
-           "tal_bsw_server_get_stats", "tal_bsw_server_destroy"]
+           "tal_bsw_server_get_stats", "tal_bsw_server_destroy",
+           "tal_bsw_pairfile_open", "tal_bsw_pairfile_read", "tal_bsw_pairfile_close",
+           "tal_bsw_pairfile_write", "tal_bsw_pack_write", "tal_bsw_pack_map",
+           "tal_bsw_pack_unmap"]
 
 
 class Params(ctypes.Structure):
@@ -67,6 +70,13 @@ class ServerStats(ctypes.Structure):
     _fields_ = [("requests", ctypes.c_int64), ("pairs", ctypes.c_int64),
                 ("batches", ctypes.c_int64), ("peak_bytes", ctypes.c_int64),
                 ("busy_ms", ctypes.c_double)]
+
+
+class Pack(ctypes.Structure):
+    _fields_ = [("pairs", ctypes.c_void_p), ("n", ctypes.c_int64),
+                ("ref", ctypes.c_void_p), ("ref_len", ctypes.c_size_t),
+                ("qer", ctypes.c_void_p), ("qer_len", ctypes.c_size_t),
+                ("mapping", ctypes.c_void_p), ("mapping_len", ctypes.c_size_t)]
 
 
 class TalError(RuntimeError):
@@ -124,6 +134,18 @@ def load():
         L.tal_bsw_server_get_stats.restype = ctypes.c_int
         L.tal_bsw_server_get_stats.argtypes = [P, ctypes.POINTER(ServerStats)]
         L.tal_bsw_server_destroy.argtypes = [P]
+        L.tal_bsw_pairfile_open.restype = ctypes.c_int
+        L.tal_bsw_pairfile_open.argtypes = [ctypes.c_char_p, ctypes.POINTER(P)]
+        L.tal_bsw_pairfile_read.restype = ctypes.c_int
+        L.tal_bsw_pairfile_read.argtypes = [P, P, P, sz, P, sz, i32, ctypes.POINTER(i32)]
+        L.tal_bsw_pairfile_close.argtypes = [P]
+        L.tal_bsw_pairfile_write.restype = ctypes.c_int
+        L.tal_bsw_pairfile_write.argtypes = [ctypes.c_char_p, P, P, P, i32]
+        L.tal_bsw_pack_write.restype = ctypes.c_int
+        L.tal_bsw_pack_write.argtypes = [ctypes.c_char_p, P, ctypes.c_int64, P, sz, P, sz]
+        L.tal_bsw_pack_map.restype = ctypes.c_int
+        L.tal_bsw_pack_map.argtypes = [ctypes.c_char_p, ctypes.POINTER(Pack)]
+        L.tal_bsw_pack_unmap.argtypes = [ctypes.POINTER(Pack)]
         _lib = L
     return _lib
 
@@ -333,3 +355,86 @@ def is_8bit(pairs, w_match=1, w_mismatch=4, **_):
     l1, l2, h0 = (pairs[f].astype(np.int64) for f in ("len1", "len2", "h0"))
     return ((l1 >= 0) & (l2 >= 0) & (l1 <= 127) & (l2 <= 127) & (h0 >= 0)
             & (h0 + sm * np.minimum(l1, l2) <= 127))
+
+
+REF_SLOT, QER_SLOT, CHUNK = 256, 128, 20000   # bandedSWA.h:47-51
+
+
+def read_pairfile(path, max_pairs=CHUNK, ref_slot=REF_SLOT, qer_slot=QER_SLOT):
+    """Yields (pairs, ref, qer) chunks of a bench-bsw text pair file
+    (benchmarks/bench-bsw.cpp:117-169 layout: fixed slots per pair)."""
+    L = load()
+    f = ctypes.c_void_p()
+    rc = L.tal_bsw_pairfile_open(os.fsencode(path), ctypes.byref(f))
+    if rc != TAL_OK:
+        raise TalError(rc, f"cannot open {path}")
+    try:
+        while True:
+            pairs = np.zeros(max_pairs, SEQPAIR_DTYPE)
+            ref = np.zeros(max_pairs * ref_slot, np.uint8)
+            qer = np.zeros(max_pairs * qer_slot, np.uint8)
+            n = ctypes.c_int32()
+            rc = L.tal_bsw_pairfile_read(f, _ptr(pairs), _ptr(ref), ref_slot, _ptr(qer), qer_slot,
+                                         max_pairs, ctypes.byref(n))
+            if rc != TAL_OK:
+                raise TalError(rc, f"reading {path}")
+            if n.value == 0:
+                return
+            yield pairs[:n.value], ref[:n.value * ref_slot], qer[:n.value * qer_slot]
+    finally:
+        L.tal_bsw_pairfile_close(f)
+
+
+def write_pairfile(path, pairs, ref, qer):
+    """Writes pairs as a bench-bsw text pair file (h0, reference, query lines)."""
+    pairs = np.ascontiguousarray(pairs, SEQPAIR_DTYPE)
+    ref = np.ascontiguousarray(ref, np.uint8)
+    qer = np.ascontiguousarray(qer, np.uint8)
+    rc = load().tal_bsw_pairfile_write(os.fsencode(path), _ptr(pairs), _ptr(ref), _ptr(qer),
+                                       len(pairs))
+    if rc != TAL_OK:
+        raise TalError(rc, f"writing {path}")
+
+
+def write_pack(path, pairs, ref, qer):
+    """Writes a packed pair file (header, SeqPair records, reference bytes, query bytes)."""
+    pairs = np.ascontiguousarray(pairs, SEQPAIR_DTYPE)
+    ref = np.ascontiguousarray(ref, np.uint8)
+    qer = np.ascontiguousarray(qer, np.uint8)
+    rc = load().tal_bsw_pack_write(os.fsencode(path), _ptr(pairs), len(pairs), _ptr(ref),
+                                   ref.nbytes, _ptr(qer), qer.nbytes)
+    if rc != TAL_OK:
+        raise TalError(rc, f"writing {path}")
+
+
+class MappedPack:
+    """A packed pair file mapped copy-on-write; pairs/ref/qer are numpy views of
+    the mapping, valid until close()."""
+
+    def __init__(self, path):
+        self._p = Pack()
+        rc = load().tal_bsw_pack_map(os.fsencode(path), ctypes.byref(self._p))
+        if rc != TAL_OK:
+            raise TalError(rc, f"cannot map {path}")
+        p = self._p
+
+        def view(addr, count, dtype):
+            if count == 0:
+                return np.zeros(0, dtype)
+            buf = (ctypes.c_char * (count * np.dtype(dtype).itemsize)).from_address(addr)
+            return np.frombuffer(buf, dtype)
+
+        self.pairs = view(p.pairs, p.n, SEQPAIR_DTYPE)
+        self.ref = view(p.ref, p.ref_len, np.uint8)
+        self.qer = view(p.qer, p.qer_len, np.uint8)
+
+    def close(self):
+        if self._p.mapping:
+            self.pairs = self.ref = self.qer = None
+            load().tal_bsw_pack_unmap(ctypes.byref(self._p))
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
