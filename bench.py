@@ -83,6 +83,9 @@ def main():
                     help="streamed serving instead: Poisson arrivals of 1k-64k-pair requests, "
                          "p50/p99 latency at 50/80/95%% of the measured peak")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests per load level")
+    ap.add_argument("--sw", default="", choices=["", *sorted(synth.SW_WORKLOADS)],
+                    help="measure the full inter-task SW engine (PairWiseSW drop-in) on one of "
+                         "its synthetic workloads instead")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (for scaling the "
                          "FETCH_SIZE/WRITE_SIZE counters) and exit")
@@ -97,8 +100,112 @@ def main():
     if args.serve:
         _serve(args)
         return
+    if args.sw:
+        _sw(args)
+        return
 
     _batch(args)
+
+
+# bench-smith-waterman's default weights (bench-smith-waterman.cpp:51-54)
+SW_PARAMS = (1, -1, 1, 1)
+# VALU lane-ops per cell: packed 16-bit classes xor, neq, mad, add, 3 max for
+# H, best, H+open, 2 for E, 2 for F = 13 (two cells per 32-bit lane op, so the
+# peak is 2 x VALU_PEAK_TOPS); the 32-bit class has compare and select for the
+# score instead of neq and mad, and max3: 11, one cell per lane op
+SW_OPS = {8: 13, 16: 13, 32: 11}
+
+
+def _sw(args):
+    """Full inter-task SW, PairWiseSW_{8,16,32}::getScores drop-in: one step
+    scores the whole synthetic batch from HBM (bounds kernel, one wait for
+    the batch bounds, unit sort, DP kernel); value = sum of len1 * len2 over
+    the batch / step time (bench-smith-waterman.cpp:150,246)."""
+    import torch
+    import tal_sw
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    bits, n_def, lo, hi = synth.SW_WORKLOADS[args.sw]
+    n = args.pairs or n_def
+    t = time.time()
+    pairs, buf = synth.sw_generate(bits, n, lo, hi)
+    cells = float(tal_sw.cells(pairs).sum())
+    log(f"[sw] {args.sw}: {n} pairs len U[{lo},{hi}] class {bits}-bit in {time.time() - t:.1f}s; "
+        f"cells {cells:.3e}")
+    dev = torch.device("cuda:0")
+    d_buf = torch.from_numpy(buf).to(dev)
+    d_pairs = torch.from_numpy(pairs.view(np.uint8)).to(dev)
+    eng = tal_sw.PairWiseSW(*SW_PARAMS)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def step():
+        eng.get_scores_device(bits, d_pairs.data_ptr(), d_buf.data_ptr(), buf.nbytes, n, stream)
+
+    step()
+    torch.cuda.synchronize()
+    got = d_pairs.cpu().numpy().view(tal_sw.PAIR_DTYPE)
+    g = tal_sw.GROUP[bits]
+    m = min(n, max(g, (args.check_pairs // g) * g))
+    import oracle as O
+    want = O.sw_scores(bits, pairs[:m].copy(), buf, SW_PARAMS, nthreads=16)["score"]
+    bad = int((got["score"][:m] != want).sum())
+    if bad:
+        raise SystemExit(f"[sw] parity FAILED on {bad} of the first {m} pairs")
+    log(f"[sw] parity: first {m} pairs bit-exact against the oracle")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        kms.append(eng.stats()["ms_kernel"])
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    st = eng.stats()
+    k_ms = float(np.mean(kms))
+    peak = VALU_PEAK_TOPS * (2 if bits != 32 else 1)
+    achieved = SW_OPS[bits] * cells / (k_ms / 1e3) / 1e12
+    line = {
+        "metric": f"GCUPS (len1*len2 cells/s) of bit-exact full SW, PairWiseSW_{bits} drop-in",
+        "value": round(cells / dt / 1e9, 2), "unit": "GCUPS", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": {8: "int8 (as int16 << 8)", 16: "int16", 32: "int32"}[bits],
+        "data": f"synthetic (seeded, {args.sw})",
+        "config": {"workload": args.sw, "pairs": n, "len": f"U[{lo},{hi}]", "class": bits,
+                   "weights": SW_PARAMS, "cells": cells},
+        "pairs_per_s": round(n / dt, 1), "parity": f"bit-exact on the first {m} pairs",
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 2),
+                     "unit": "TOP/s", "frac": round(achieved / peak, 4), "traffic": None,
+                     "kernel": f"sw_main<{bits}>", "kernel_ms": round(k_ms, 3),
+                     "kernel_gcups": round(cells / (k_ms / 1e3) / 1e9, 1),
+                     "ops_per_cell": SW_OPS[bits],
+                     "padded_rows_cols": [st["max_rows"], st["max_cols"]]},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = _sw_cpu(bits, pairs, buf, args.cpu_pairs)
+    print(json.dumps(line), flush=True)
+
+
+def _sw_cpu(bits, pairs, buf, m):
+    """The reference's own getScores (AVX-512BW, 16 threads) when oracle/_ref
+    holds it and the host runs it, else the oracle; on the first m pairs."""
+    import oracle as O
+    import tal_sw
+    g = tal_sw.GROUP[bits]
+    m = min(len(pairs), max(g, (m // g) * g))
+    sub = pairs[:m].copy()
+    kind = "reference"
+    t = time.perf_counter()
+    out = O.ref_sw_scores(bits, sub, buf, SW_PARAMS, nthreads=16) if O.ref_sw_available() else None
+    if out is None:
+        kind = "port"
+        t = time.perf_counter()
+        O.sw_scores(bits, sub, buf, SW_PARAMS, nthreads=16)
+    dt = time.perf_counter() - t
+    cells = float(tal_sw.cells(sub).sum())
+    return {"value": round(cells / dt / 1e9, 3), "unit": "GCUPS", "cores": 16, "kind": kind,
+            "sample": f"first {m} pairs, {dt:.1f}s"}
 
 
 def _traffic():

@@ -151,3 +151,99 @@ def make_pairs(refs, queries, h0s):
     for f in OUT_FIELDS:
         pairs[f] = -1
     return pairs, ref, qer
+
+
+# ---------------------------------------------------------------------------
+# Full inter-task SW (sw_oracle.c): PairWiseSW_{8,16,32}::getScores.
+
+# SeqPair of PairWiseSW.h:44-49 (12 bytes)
+SW_PAIR_DTYPE = np.dtype([("id", "<i4"), ("len1", "<i2"), ("len2", "<i2"), ("score", "<i4")])
+SW_GROUP = {8: 64, 16: 32, 32: 16}        # SIMD_WIDTH of the AVX-512BW build
+SW_SLOT = {8: 128, 16: 1024, 32: 1024}    # MAX_SEQ_LEN_8/16/32
+REF_SW_PATH = os.path.join(HERE, "_ref", "libpwsw_ref.so")
+
+
+def _sw_lib():
+    L = lib()
+    if not getattr(L, "_sw_ready", False):
+        P, i32 = ctypes.c_void_p, ctypes.c_int32
+        L.sw_oracle_get_scores.restype = ctypes.c_int
+        L.sw_oracle_get_scores.argtypes = [ctypes.c_int] + [i32] * 4 + [P, P, i32, ctypes.c_int]
+        L.sw_oracle_scalar.argtypes = [i32] * 4 + [P, P, i32, i32]
+        L.sw_oracle_set_mutation.argtypes = [ctypes.c_int]
+        L._sw_ready = True
+    return L
+
+
+def sw_scores(bits, pairs, seq_buf, params=(1, -1, 1, 1), nthreads=1):
+    """Oracle getScores of the bits-wide class; scores written into pairs."""
+    assert pairs.dtype == SW_PAIR_DTYPE and pairs.flags.c_contiguous
+    seq_buf = np.ascontiguousarray(seq_buf, dtype=np.uint8)
+    rc = _sw_lib().sw_oracle_get_scores(int(bits), *[int(x) for x in params], _ptr(pairs),
+                                        _ptr(seq_buf), len(pairs), int(nthreads))
+    if rc != 0:
+        raise ValueError(f"bad width {bits}")
+    return pairs
+
+
+def sw_scalar(pairs, seq_buf, params=(1, -1, 1, 1), slot=1024):
+    """smithWatermanOnePair on each pair's own lengths."""
+    seq_buf = np.ascontiguousarray(seq_buf, dtype=np.uint8)
+    _sw_lib().sw_oracle_scalar(*[int(x) for x in params], _ptr(pairs), _ptr(seq_buf),
+                               len(pairs), int(slot))
+    return pairs
+
+
+def sw_set_mutation(m):
+    """Plants a known mistake in sw_scores (tests of the pins only)."""
+    _sw_lib().sw_oracle_set_mutation(int(m))
+
+
+_ref_sw = None
+
+
+def ref_sw_available():
+    return os.path.exists(REF_SW_PATH)
+
+
+def ref_sw_scores(bits, pairs, seq_buf, params=(1, -1, 1, 1), nthreads=1):
+    """The reference's own getScores (oracle/_ref, built from its sources by
+    `make -C oracle ref`).  Returns None when the CPU lacks AVX-512BW."""
+    global _ref_sw
+    if _ref_sw is None:
+        _ref_sw = ctypes.CDLL(REF_SW_PATH)
+        _ref_sw.ref_sw_get_scores.restype = ctypes.c_int
+        _ref_sw.ref_sw_get_scores.argtypes = [ctypes.c_int] + [ctypes.c_int32] * 4 + [
+            ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int]
+    n = len(pairs)
+    g = SW_GROUP[bits]
+    # getScores writes the padding entries of the last group
+    buf = np.zeros(((n + g - 1) // g) * g, SW_PAIR_DTYPE)
+    buf[:n] = pairs
+    seq_buf = np.ascontiguousarray(seq_buf, dtype=np.uint8).copy()
+    rc = _ref_sw.ref_sw_get_scores(int(bits), *[int(x) for x in params], _ptr(buf),
+                                   _ptr(seq_buf), n, int(nthreads))
+    if rc == -2:
+        return None
+    if rc != 0:
+        raise ValueError(f"bad width {bits}")
+    pairs["score"] = buf["score"][:n]
+    return pairs
+
+
+def sw_make_batch(seq1s, seq2s, bits):
+    """(pairs, seq_buf) in the reference's layout: pair k's sequences at
+    seq_buf + 2*k*S and + (2k+1)*S (S = MAX_SEQ_LEN of the class)."""
+    S = SW_SLOT[bits]
+    n = len(seq1s)
+    pairs = np.zeros(n, SW_PAIR_DTYPE)
+    buf = np.zeros(max(1, 2 * n * S), np.uint8)
+    for k in range(n):
+        a = np.asarray(seq1s[k], np.uint8)
+        b = np.asarray(seq2s[k], np.uint8)
+        buf[2 * k * S:2 * k * S + len(a)] = a
+        buf[(2 * k + 1) * S:(2 * k + 1) * S + len(b)] = b
+    pairs["id"] = np.arange(n)
+    pairs["len1"] = [len(a) for a in seq1s]
+    pairs["len2"] = [len(b) for b in seq2s]
+    return pairs, buf

@@ -30,6 +30,33 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(lib, name), name
 
 
+def test_sw_header_declares_binding_exports():
+    import tal_sw
+    text = open(os.path.join(ROOT, "include", "tal_sw.h")).read()
+    declared = sorted(set(re.findall(r"\b(tal_sw_\w+)\s*\(", text)))
+    assert declared == sorted(tal_sw.EXPORTS)
+    lib = ctypes.CDLL(tal_sw.LIB_PATH)
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_sw_pair_layout(tmp_path):
+    src = tmp_path / "swlayout.c"
+    src.write_text("""
+#include <stddef.h>
+#include "tal_sw.h"
+_Static_assert(sizeof(tal_sw_pair) == 12, "size");
+_Static_assert(offsetof(tal_sw_pair, len1) == 4, "len1");
+_Static_assert(offsetof(tal_sw_pair, len2) == 6, "len2");
+_Static_assert(offsetof(tal_sw_pair, score) == 8, "score");
+int main(void) { return 0; }
+""")
+    subprocess.check_call(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), "-c", str(src),
+                           "-o", str(tmp_path / "swlayout.o")])
+    import tal_sw
+    assert tal_sw.PAIR_DTYPE.itemsize == 12
+
+
 def test_seqpair_layout(tmp_path):
     src = tmp_path / "layout.c"
     src.write_text("""

@@ -14,6 +14,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TAL_BSW_ARCH", "gfx950")
 
 HIP_SOURCES = ["bsw_kernels.hip", "tal_bsw.hip", "bsw_server.hip", "bsw_pairfile.cpp"]
+SW_SOURCES = ["sw_kernels.hip", "tal_sw.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -27,10 +28,10 @@ def _run(cmd):
     subprocess.check_call(cmd)
 
 
-def build_hip(out=None, extra=()):
+def build_hip(out=None, extra=(), sources=None):
     out = out or os.path.join(HERE, "libtal_bsw.so")
     objs = []
-    for src in HIP_SOURCES:
+    for src in sources or HIP_SOURCES:
         obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
         _run([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
@@ -56,9 +57,15 @@ def build_synth(out=None):
     return out
 
 
+def build_sw(out=None):
+    """libtal_sw.so: the full inter-task SW engine (include/tal_sw.h)."""
+    return build_hip(out or os.path.join(HERE, "libtal_sw.so"), sources=SW_SOURCES)
+
+
 def build_all():
     build_synth()
     build_hip()
+    build_sw()
     build_cli()
 
 

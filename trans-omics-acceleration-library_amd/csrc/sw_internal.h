@@ -1,0 +1,50 @@
+/*
+ * sw_internal.h -- data shared by the full-SW kernels (sw_kernels.hip) and
+ * their host runtime (tal_sw.hip).
+ */
+#ifndef TAL_SW_INTERNAL_H
+#define TAL_SW_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../../include/tal_sw.h"
+
+namespace talsw {
+
+/* control words of one call */
+enum { SW_CTL_ERR = 0, SW_CTL_MAXROW = 1, SW_CTL_MAXCOL = 2, SW_CTL_COUNTER = 3, SW_CTL_WORDS = 4 };
+/* error word values (largest wins) */
+enum { SW_ERR_BOUNDS = 1, SW_ERR_RANGE = 2 };
+
+struct SwParams {
+    int32_t n;      /* pairs */
+    int32_t nunits; /* 32-pair (8/16-bit) or 16-pair (32-bit) units */
+    int32_t slot;   /* sequence slot S of the class */
+    int32_t rows_cap, cols_cap; /* slab capacity in rows / columns (multiples of 4) */
+    size_t slab_words;          /* 32-bit words per wave slab */
+    /* lane constants.  8/16-bit: two packed 16-bit halves (8-bit: value << 8)
+     * and k_delta = mismatch - match; 32-bit: plain int32 and k_delta =
+     * mismatch. */
+    uint32_t k_match, k_delta, k_open, k_ext;
+};
+
+struct SwBuffers {
+    tal_sw_pair *pairs;
+    const uint8_t *seq;
+    const int32_t *unit_dims; /* (nrow << 16) | ncol per unit */
+    const int32_t *order;     /* units in sweep order */
+    int32_t *ctl;             /* SW_CTL_* words */
+    uint32_t *slabs;          /* one slab per block of the persistent grid */
+};
+
+hipError_t launch_sw_shapes(const tal_sw_pair *pairs, int32_t n, int32_t bits, size_t seq_len,
+                            int32_t *unit_dims, uint32_t *keys, int32_t *ids, int32_t *ctl,
+                            hipStream_t s);
+hipError_t launch_sw_main(int bits, int grid, const SwParams &P, const SwBuffers &B,
+                          hipStream_t s);
+
+} // namespace talsw
+
+#endif

@@ -1,0 +1,426 @@
+/*
+ * sw_kernels.hip -- gfx950 kernels of the full inter-task Smith-Waterman
+ * engine (include/tal_sw.h), the drop-in for PairWiseSW_{8,16,32}::getScores
+ * (src/alignment/Smith-Waterman/PairWiseSW_16.cpp:136-377 and its 8/32-bit
+ * twins).
+ *
+ * Semantics (the reference's AVX-512BW build): a group is 64 / 32 / 16
+ * consecutive pairs; every pair of a group is scored over the group's padded
+ * matrix (rows = longest len1, columns = longest len2, both rounded up to a
+ * multiple of 4, in the class's integer type), with 'B' past a pair's own
+ * seq1 and 'D' past its seq2 (_16.cpp:296-345).  The recurrence of MAIN_CODE
+ * (_16.cpp:62-76), per cell, with every add wrapping at the lane width:
+ *     m = Hdiag + (s1 == s2 ? match : mismatch)
+ *     H = max(m, E, F, 0);  best = max(best, H)
+ *     t = H + open;  E' = max(E + extend, t);  F' = max(F + extend, t)
+ * with F = open and H = 0 above the first row (:170-174), E = open and
+ * Hdiag = 0 left of the first column (:183-191).
+ *
+ * Layout on the GPU:
+ *  - a unit is 32 pairs (8/16-bit) or 16 pairs (32-bit); a 16-lane quarter
+ *    of a wave scores one unit, so a wave scores four.  In the 8/16-bit
+ *    classes each lane holds two pairs of its unit (k and k+16) in the two
+ *    16-bit halves of every register and the cell update runs on packed
+ *    16-bit instructions; the 8-bit class holds value << 8 in each half, so
+ *    that 16-bit adds wrap and 16-bit signed max orders exactly as the epi8
+ *    instructions do.  The 32-bit class holds one pair per lane.
+ *  - units are sorted by their bounds (widest first), so the four units of a
+ *    wave have equal or near bounds; cells past a unit's own bounds are
+ *    computed but masked out of its maximum.
+ *  - rows are swept in strips of 16/8/4 rows held in registers (E, the
+ *    previous column's H and the seq1 codes of each row); between strips the
+ *    bottom row's H and F per column go to a per-wave slab in global memory,
+ *    4 columns per 32-byte record per lane.  The seq1/seq2 codes are staged
+ *    in the same slab, packed per lane.
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sw_internal.h"
+
+namespace talsw {
+
+namespace {
+
+typedef short s2v __attribute__((ext_vector_type(2)));
+typedef unsigned short u2v __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t pmax(uint32_t a, uint32_t b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(s2v, a),
+                                                                  __builtin_bit_cast(s2v, b)));
+}
+__device__ __forceinline__ uint32_t padd(uint32_t a, uint32_t b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u2v, a) + __builtin_bit_cast(u2v, b));
+}
+/* min(a ^ b, 1) per 16-bit half: 0 where the halves are equal, else 1.  An
+ * instruction of its own for the same reason as pmad below. */
+__device__ __forceinline__ uint32_t pneq(uint32_t a, uint32_t b)
+{
+    uint32_t r;
+    asm("v_pk_min_u16 %0, %1, 1 op_sel_hi:[1,0]" : "=v"(r) : "v"(a ^ b));
+    return r;
+}
+/* a * b + c per 16-bit half (low 16 bits).  Written as the instruction
+ * itself: with a in {0, 1} the compiler would otherwise turn it into two
+ * compares, two selects and a byte permute. */
+__device__ __forceinline__ uint32_t pmad(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t r;
+    asm("v_pk_mad_u16 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+__device__ __forceinline__ int32_t smax32(int32_t a, int32_t b) { return a > b ? a : b; }
+
+/* One cell of MAIN_CODE for the lane's pairs.  PACK: two 16-bit lanes per
+ * register (8/16-bit classes); otherwise one int32 lane. */
+template <bool PACK> struct Cell;
+
+template <> struct Cell<true> {
+    uint32_t match, delta, open, ext;
+    __device__ __forceinline__ uint32_t step(uint32_t a, uint32_t b, uint32_t d, uint32_t &e,
+                                             uint32_t &f) const
+    {
+        const uint32_t neq = pneq(a, b);
+        const uint32_t m = padd(d, pmad(neq, delta, match));
+        const uint32_t h = pmax(pmax(pmax(m, e), f), 0u);
+        const uint32_t t = padd(h, open);
+        e = pmax(padd(e, ext), t);
+        f = pmax(padd(f, ext), t);
+        return h;
+    }
+    __device__ __forceinline__ uint32_t best(uint32_t acc, uint32_t h) const { return pmax(acc, h); }
+};
+
+template <> struct Cell<false> {
+    uint32_t match, mismatch, open, ext;
+    __device__ __forceinline__ uint32_t step(uint32_t a, uint32_t b, uint32_t d, uint32_t &e,
+                                             uint32_t &f) const
+    {
+        const uint32_t m = d + (a == b ? match : mismatch);
+        const int32_t h = smax32(smax32(smax32((int32_t)m, (int32_t)e), (int32_t)f), 0);
+        const uint32_t t = (uint32_t)h + open;
+        e = (uint32_t)smax32((int32_t)(e + ext), (int32_t)t);
+        f = (uint32_t)smax32((int32_t)(f + ext), (int32_t)t);
+        return (uint32_t)h;
+    }
+    __device__ __forceinline__ uint32_t best(uint32_t acc, uint32_t h) const
+    {
+        return (uint32_t)smax32((int32_t)acc, (int32_t)h);
+    }
+};
+
+/* Slab of one wave, in 32-bit words:
+ *   codes1 [rows_cap/4][64][4]   seq1 codes of row r at [r>>2][lane][r&3]
+ *   codes2 [cols_cap/4][64][4]   seq2 codes, same shape over columns
+ *   state  [cols_cap/4][64][8]   H (4 columns) then F (4 columns) of the row
+ *                                below the last finished strip */
+struct Slab {
+    uint32_t *codes1, *codes2, *state;
+};
+
+/*
+ * One strip of H rows starting at r0, over columns [0, NC).  MASK: some
+ * lane's unit ends inside the strip or before NC, so cells past the unit's
+ * own bounds (rows >= nrow, columns >= ncol) are kept out of its maximum.
+ */
+template <bool PACK, int H, bool MASK>
+__device__ __forceinline__ uint32_t strip(const Cell<PACK> &C, const Slab &sl, int lane, int r0,
+                                          int NC, bool first, bool last, int nrow, int ncol,
+                                          uint32_t best)
+{
+    uint32_t s1[H], e[H], hp[H];
+    const u32x4 *c1 = (const u32x4 *)sl.codes1 + (size_t)(r0 >> 2) * 64 + lane;
+#pragma unroll
+    for (int k = 0; k < H / 4; k++) {
+        const u32x4 v = c1[(size_t)k * 64];
+        s1[4 * k] = v.x;
+        s1[4 * k + 1] = v.y;
+        s1[4 * k + 2] = v.z;
+        s1[4 * k + 3] = v.w;
+    }
+#pragma unroll
+    for (int r = 0; r < H; r++) {
+        e[r] = C.open;
+        hp[r] = 0;
+    }
+    const int rv = nrow - r0; /* rows of this strip inside the lane's unit */
+    const u32x4 *c2 = (const u32x4 *)sl.codes2 + lane;
+    u32x4 *st = (u32x4 *)sl.state + 2 * lane;
+    const u32x4 open4 = {C.open, C.open, C.open, C.open};
+    const u32x4 zero4 = {0u, 0u, 0u, 0u};
+    uint32_t hup = 0;
+    uint32_t acc[4] = {best, best, best, best};
+    u32x4 nb = c2[0];
+    u32x4 nh = first ? zero4 : st[0];
+    u32x4 nf = first ? open4 : st[1];
+    for (int c0 = 0; c0 < NC; c0 += 4) {
+        const int t = c0 >> 2;
+        const u32x4 b = nb, hu = nh, fu = nf;
+        if (c0 + 4 < NC) { /* prefetch the next 4-column record */
+            nb = c2[(size_t)(t + 1) * 64];
+            if (!first) {
+                nh = st[(size_t)(t + 1) * 128];
+                nf = st[(size_t)(t + 1) * 128 + 1];
+            }
+        }
+        uint32_t rm[H];
+        if (MASK) {
+            const uint32_t cm = c0 < ncol ? 0xFFFFFFFFu : 0u;
+#pragma unroll
+            for (int r = 0; r < H; r++) rm[r] = r < rv ? cm : 0u;
+        }
+        /* the tile's 16 x 4 cells in anti-diagonal order: step k visits
+         * (k - cc, cc) for the four columns, which do not depend on each
+         * other, so dependent packed instructions are four cells apart.
+         * Column cc carries its F down and the H it takes as the next row's
+         * diagonal; row r carries E right, and hp[r] hands H(r, cc) to
+         * column cc + 1. */
+        uint32_t f[4] = {fu.x, fu.y, fu.z, fu.w};
+        uint32_t d[4] = {hup, hu.x, hu.y, hu.z};
+        hup = hu.w;
+        const uint32_t bc[4] = {b.x, b.y, b.z, b.w};
+        uint32_t ho[4];
+#pragma unroll
+        for (int k = 0; k < H + 3; k++) {
+#pragma unroll
+            for (int cc = 0; cc < 4; cc++) {
+                const int r = k - cc;
+                if (r < 0 || r >= H) continue;
+                const uint32_t h = C.step(s1[r], bc[cc], d[cc], e[r], f[cc]);
+                acc[cc] = C.best(acc[cc], MASK ? (h & rm[r]) : h);
+                d[cc] = hp[r];
+                hp[r] = h;
+                if (r == H - 1) ho[cc] = h;
+            }
+        }
+        const uint32_t *fo = f;
+        if (!last) {
+            st[(size_t)t * 128] = u32x4{ho[0], ho[1], ho[2], ho[3]};
+            st[(size_t)t * 128 + 1] = u32x4{fo[0], fo[1], fo[2], fo[3]};
+        }
+    }
+    return C.best(C.best(acc[0], acc[1]), C.best(acc[2], acc[3]));
+}
+
+/* All rows in strips of 16, then 8 and 4 for the remainder (the bounds
+ * are multiples of 4).  A strip is masked only where some unit of the wave
+ * ends before the strip's last row or before the widest unit's last
+ * column. */
+template <bool PACK>
+__device__ __forceinline__ uint32_t sweep(const Cell<PACK> &C, const Slab &sl, int lane, int NR,
+                                          int NC, int nrow, int ncol, int min_nrow, int min_ncol)
+{
+    uint32_t best = 0;
+    int r0 = 0;
+    while (r0 < NR) {
+        const int rem = NR - r0;
+        const int h = rem >= 16 ? 16 : rem >= 8 ? 8 : 4;
+        const bool first = r0 == 0, last = r0 + h >= NR;
+        const bool m = r0 + h > min_nrow || min_ncol < NC;
+        if (h == 16) {
+            best = m ? strip<PACK, 16, true>(C, sl, lane, r0, NC, first, last, nrow, ncol, best)
+                     : strip<PACK, 16, false>(C, sl, lane, r0, NC, first, last, nrow, ncol, best);
+        } else if (h == 8) {
+            best = m ? strip<PACK, 8, true>(C, sl, lane, r0, NC, first, last, nrow, ncol, best)
+                     : strip<PACK, 8, false>(C, sl, lane, r0, NC, first, last, nrow, ncol, best);
+        } else {
+            best = m ? strip<PACK, 4, true>(C, sl, lane, r0, NC, first, last, nrow, ncol, best)
+                     : strip<PACK, 4, false>(C, sl, lane, r0, NC, first, last, nrow, ncol, best);
+        }
+        r0 += h;
+    }
+    return best;
+}
+
+__device__ __forceinline__ int wave_max4(int v)
+{
+    const int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+    const int c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+    return max(max(a, b), max(c, d));
+}
+__device__ __forceinline__ int wave_min4(int v)
+{
+    const int a = __builtin_amdgcn_readlane(v, 0), b = __builtin_amdgcn_readlane(v, 16);
+    const int c = __builtin_amdgcn_readlane(v, 32), d = __builtin_amdgcn_readlane(v, 48);
+    return min(min(a, b), min(c, d));
+}
+
+/* byte k of a pair's sequence, or the padding code past its length */
+__device__ __forceinline__ uint32_t code_at(const uint8_t *s, int len, int k, uint32_t pad)
+{
+    return k < len ? (uint32_t)s[k] : pad;
+}
+
+} // namespace
+
+/*
+ * Group bounds and validation, one thread per group: the padded bounds of
+ * the wrapper (_16.cpp:296-345; int8 bounds in the 8-bit class, _8.cpp:293,
+ * 308, which wrap to -128 when they round up to 128), each unit's bounds and
+ * sort key, the batch maxima and the error word.
+ */
+__global__ void sw_shapes(const tal_sw_pair *pairs, int32_t n, int32_t bits, size_t seq_len,
+                          int32_t *unit_dims, uint32_t *keys, int32_t *ids, int32_t *ctl)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int GL = bits == 8 ? 64 : bits == 16 ? 32 : 16;
+    const int UL = bits == 32 ? 16 : 32;
+    const int ngroups = (n + GL - 1) / GL;
+    if (g >= ngroups) return;
+    const int64_t S = bits == 8 ? 128 : 1024;
+    const int limit = bits == 8 ? 127 : 1024;
+    int max1 = 0, max2 = 0, err = 0;
+    for (int k = 0; k < GL; k++) {
+        const int p = g * GL + k;
+        if (p >= n) break;
+        const tal_sw_pair sp = pairs[p];
+        if (sp.len1 < 0 || sp.len2 < 0 || sp.len1 > limit || sp.len2 > limit) {
+            err = max(err, SW_ERR_RANGE);
+            continue;
+        }
+        if (sp.id < 0 || (2 * (int64_t)sp.id + 1) * S + sp.len2 > (int64_t)seq_len ||
+            2 * (int64_t)sp.id * S + sp.len1 > (int64_t)seq_len)
+            err = max(err, SW_ERR_BOUNDS);
+        max1 = max(max1, (int)sp.len1);
+        max2 = max(max2, (int)sp.len2);
+    }
+    if (err) atomicMax(&ctl[SW_CTL_ERR], err);
+    int nrow = ((max1 + 3) / 4) * 4, ncol = ((max2 + 3) / 4) * 4;
+    if (bits == 8) {
+        nrow = (int8_t)(uint8_t)(nrow & 0xFF);
+        ncol = (int8_t)(uint8_t)(ncol & 0xFF);
+    }
+    /* a bound <= 0 leaves the reference's loops empty: every score is 0 */
+    if (nrow <= 0 || ncol <= 0) nrow = ncol = 0;
+    const int upg = GL / UL;
+    for (int u = 0; u < upg; u++) {
+        const int id = g * upg + u;
+        unit_dims[id] = (nrow << 16) | ncol;
+        keys[id] = ((uint32_t)(ncol >> 2) << 10) | (uint32_t)(nrow >> 2);
+        ids[id] = id;
+    }
+    atomicMax(&ctl[SW_CTL_MAXROW], nrow);
+    atomicMax(&ctl[SW_CTL_MAXCOL], ncol);
+}
+
+/*
+ * The DP.  Persistent grid of one-wave blocks; a wave takes the next four
+ * units of the sorted order from the task counter until none is left.
+ */
+template <int BITS>
+__global__ void __launch_bounds__(64) sw_main(SwParams P, SwBuffers B)
+{
+    constexpr bool PACK = BITS != 32;
+    const int lane = threadIdx.x;
+    const int q = lane >> 4, l16 = lane & 15;
+    Slab sl;
+    {
+        uint32_t *base = B.slabs + (size_t)blockIdx.x * P.slab_words;
+        sl.codes1 = base;
+        sl.codes2 = base + (size_t)P.rows_cap * 64;
+        sl.state = sl.codes2 + (size_t)P.cols_cap * 64;
+    }
+    Cell<PACK> C;
+    if constexpr (PACK) {
+        C = Cell<true>{P.k_match, P.k_delta, P.k_open, P.k_ext};
+    } else {
+        C = Cell<false>{P.k_match, P.k_delta, P.k_open, P.k_ext};
+    }
+    const int ntasks = (P.nunits + 3) >> 2;
+    for (;;) {
+        int task = 0;
+        if (lane == 0) task = atomicAdd(&B.ctl[SW_CTL_COUNTER], 1);
+        task = __builtin_amdgcn_readfirstlane(task);
+        if (task >= ntasks) break;
+        const int ui = task * 4 + q;
+        const int unit = ui < P.nunits ? B.order[ui] : -1;
+        const int dims = unit >= 0 ? B.unit_dims[unit] : 0;
+        const int nrow = dims >> 16, ncol = dims & 0xFFFF;
+        const int NR = wave_max4(nrow), NC = wave_max4(ncol);
+        const int min_nrow = wave_min4(nrow), min_ncol = wave_min4(ncol);
+        /* the lane's pairs: k and k+16 of the unit (8/16-bit), k (32-bit) */
+        const int ul = PACK ? 32 : 16;
+        const int pa = unit >= 0 ? unit * ul + l16 : P.n;
+        const int pb = PACK ? pa + 16 : P.n;
+        const bool va = pa < P.n, vb = PACK && unit >= 0 && pb < P.n;
+        int la1 = 0, la2 = 0, lb1 = 0, lb2 = 0;
+        const uint8_t *a1 = B.seq, *a2 = B.seq, *b1 = B.seq, *b2 = B.seq;
+        if (va) {
+            const tal_sw_pair sp = B.pairs[pa];
+            la1 = sp.len1;
+            la2 = sp.len2;
+            a1 = B.seq + 2 * (int64_t)sp.id * P.slot;
+            a2 = a1 + P.slot;
+        }
+        if (vb) {
+            const tal_sw_pair sp = B.pairs[pb];
+            lb1 = sp.len1;
+            lb2 = sp.len2;
+            b1 = B.seq + 2 * (int64_t)sp.id * P.slot;
+            b2 = b1 + P.slot;
+        }
+        /* stage the codes; rows/columns past a pair's length take the
+         * padding codes 'B' / 'D' up to the unit's bounds */
+        for (int r = 0; r < NR; r += 4) {
+            u32x4 v;
+            uint32_t w[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                uint32_t c = code_at(a1, la1, r + k, 'B');
+                if (PACK) c |= code_at(b1, lb1, r + k, 'B') << 16;
+                w[k] = c;
+            }
+            v = u32x4{w[0], w[1], w[2], w[3]};
+            ((u32x4 *)sl.codes1)[(size_t)(r >> 2) * 64 + lane] = v;
+        }
+        for (int c = 0; c < NC; c += 4) {
+            uint32_t w[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                uint32_t x = code_at(a2, la2, c + k, 'D');
+                if (PACK) x |= code_at(b2, lb2, c + k, 'D') << 16;
+                w[k] = x;
+            }
+            ((u32x4 *)sl.codes2)[(size_t)(c >> 2) * 64 + lane] = u32x4{w[0], w[1], w[2], w[3]};
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+        const uint32_t best = sweep<PACK>(C, sl, lane, NR, NC, nrow, ncol, min_nrow, min_ncol);
+        if (PACK) {
+            const int16_t lo = (int16_t)(best & 0xFFFF), hi = (int16_t)(best >> 16);
+            if (va) B.pairs[pa].score = BITS == 8 ? (lo >> 8) : lo;
+            if (vb) B.pairs[pb].score = BITS == 8 ? (hi >> 8) : hi;
+        } else if (va) {
+            B.pairs[pa].score = (int32_t)best;
+        }
+        /* the slab is rewritten by the next task: finish this one's reads */
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    }
+}
+
+hipError_t launch_sw_shapes(const tal_sw_pair *pairs, int32_t n, int32_t bits, size_t seq_len,
+                            int32_t *unit_dims, uint32_t *keys, int32_t *ids, int32_t *ctl,
+                            hipStream_t s)
+{
+    const int GL = bits == 8 ? 64 : bits == 16 ? 32 : 16;
+    const int G = (n + GL - 1) / GL;
+    if (G == 0) return hipSuccess;
+    hipLaunchKernelGGL(sw_shapes, dim3((G + 127) / 128), dim3(128), 0, s, pairs, n, bits, seq_len,
+                       unit_dims, keys, ids, ctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_sw_main(int bits, int grid, const SwParams &P, const SwBuffers &B, hipStream_t s)
+{
+    if (bits == 8)
+        hipLaunchKernelGGL(sw_main<8>, dim3(grid), dim3(64), 0, s, P, B);
+    else if (bits == 16)
+        hipLaunchKernelGGL(sw_main<16>, dim3(grid), dim3(64), 0, s, P, B);
+    else
+        hipLaunchKernelGGL(sw_main<32>, dim3(grid), dim3(64), 0, s, P, B);
+    return hipGetLastError();
+}
+
+} // namespace talsw

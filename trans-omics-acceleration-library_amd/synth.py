@@ -69,3 +69,46 @@ def generate(cfg, n, first=0, seed=None, nthreads=8):
                             _p(ref), _p(qer), int(nthreads)):
         raise ValueError(f"unknown workload {cfg}")
     return pairs, ref, qer, w
+
+
+# ---------------------------------------------------------------------------
+# Full inter-task SW batches (PairWiseSW_{8,16,32} layout, bench-smith-
+# waterman's input): pair k's seq1 at seq_buf + 2kS and seq2 at (2k+1)S, as
+# ASCII bases.  seq2 is seq1 read from a random offset with 10% substitutions,
+# so that the alignments are local and the scores non-trivial.
+
+SW_PAIR_DTYPE = np.dtype([("id", "<i4"), ("len1", "<i2"), ("len2", "<i2"), ("score", "<i4")])
+SW_SLOT = {8: 128, 16: 1024, 32: 1024}
+
+# name -> (class bits, pairs, min length, max length)
+SW_WORKLOADS = {
+    "sw16": (16, 200_000, 200, 1000),
+    "sw8": (8, 1_000_000, 30, 124),
+    "sw32": (32, 50_000, 200, 1000),
+}
+
+
+def sw_generate(bits, n, lo, hi, seed=7, sub=0.10):
+    """(pairs, seq_buf) for n pairs with len1, len2 ~ U[lo, hi]."""
+    S = SW_SLOT[bits]
+    rng = np.random.default_rng(seed)
+    bases = np.frombuffer(b"ACGT", np.uint8)
+    len1 = rng.integers(lo, hi + 1, n)
+    len2 = rng.integers(lo, hi + 1, n)
+    buf = np.zeros((max(n, 1), 2, S), np.uint8)
+    src = bases[rng.integers(0, 4, (max(n, 1), S), dtype=np.uint8)]
+    buf[:, 0, :] = src
+    shift = rng.integers(0, max(1, S // 8), max(n, 1))
+    idx = (np.arange(S)[None, :] + shift[:, None]) % S
+    q = np.take_along_axis(src, idx, axis=1)
+    mut = rng.random(q.shape) < sub
+    q[mut] = bases[rng.integers(0, 4, int(mut.sum()), dtype=np.uint8)]
+    buf[:, 1, :] = q
+    pos = np.arange(S)[None, :]
+    buf[:, 0, :][pos >= len1[:, None]] = 0
+    buf[:, 1, :][pos >= len2[:, None]] = 0
+    pairs = np.zeros(n, SW_PAIR_DTYPE)
+    pairs["id"] = np.arange(n)
+    pairs["len1"] = len1
+    pairs["len2"] = len2
+    return pairs, buf.reshape(-1)
