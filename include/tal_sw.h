@@ -41,6 +41,8 @@ typedef struct tal_sw_stats {
     int32_t units;    /* 32-pair (8/16-bit) or 16-pair (32-bit) units scored */
     int32_t max_rows; /* the largest padded group bounds of the last call */
     int32_t max_cols;
+    int32_t regrouped; /* 1: the pairs were regrouped by length (padding could
+                          not change any score, see tal_sw_get_scores) */
 } tal_sw_stats;
 
 /* Sequence slot S of a class: MAX_SEQ_LEN_8 = 128, MAX_SEQ_LEN_16 =
@@ -66,7 +68,14 @@ const char *tal_sw_last_error(const tal_sw *h);
  * buffers: `seq_buf_len` bytes are copied to the device, the scores of pairs
  * [0, n) are written back.  bits = 8, 16 or 32.  TAL_ERANGE: a length is
  * negative or above the class limit; TAL_EINVAL: a pair's sequences lie
- * outside seq_buf_len or its id is negative.  Blocking. */
+ * outside seq_buf_len or its id is negative.  Blocking.
+ *
+ * When mismatch, gap open and gap extend are all <= 0 at the class width, no
+ * lane value can wrap (match * min(len1, len2) fits the class) and no seq1
+ * holds 'D' and no seq2 holds 'B' (the padding codes), a padded cell can
+ * never beat a pair's own cells, so each score is the pair's own DP: the
+ * engine then scores pairs regrouped by length and skips the padding work.
+ * The scores are the same either way. */
 int tal_sw_get_scores(tal_sw *h, int bits, tal_sw_pair *pairs, const uint8_t *seq_buf,
                       size_t seq_buf_len, int32_t n);
 

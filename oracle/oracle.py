@@ -220,7 +220,7 @@ def ref_sw_scores(bits, pairs, seq_buf, params=(1, -1, 1, 1), nthreads=1):
     # getScores writes the padding entries of the last group
     buf = np.zeros(((n + g - 1) // g) * g, SW_PAIR_DTYPE)
     buf[:n] = pairs
-    seq_buf = np.ascontiguousarray(seq_buf, dtype=np.uint8).copy()
+    seq_buf = np.ascontiguousarray(seq_buf, dtype=np.uint8)  # read only by getScores
     rc = _ref_sw.ref_sw_get_scores(int(bits), *[int(x) for x in params], _ptr(buf),
                                    _ptr(seq_buf), n, int(nthreads))
     if rc == -2:

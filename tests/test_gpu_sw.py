@@ -179,3 +179,53 @@ def test_device_path_and_stats():
     assert st["ms_kernel"] > 0 and st["units"] == (len(pairs) + 31) // 32
     assert st["max_rows"] <= 400 and st["max_cols"] <= 400
     eng.close()
+
+
+def test_regrouping_engages_only_when_exact():
+    """Penalty weights on DNA: pairs are regrouped by length (stats say so)
+    and the scores still equal the oracle's group semantics; each condition
+    that could let padding matter keeps the reference's groups."""
+    _say("[gpu] sw regrouping conditions")
+    rng = np.random.default_rng(31)
+    A = np.frombuffer(b"ACGT", np.uint8)
+    pairs, buf = _rand_batch(rng, 700, 16, 0, 400)
+    eng = T.PairWiseSW(2, -4, -6, -1)
+    _check(eng, 16, pairs, buf, "regroup")
+    assert eng.stats()["regrouped"] == 1
+    # positive gap weights: padding matters
+    eng2 = T.PairWiseSW(1, -1, 1, 1)
+    _check(eng2, 16, pairs, buf, "positive gaps")
+    assert eng2.stats()["regrouped"] == 0
+    # a seq2 holding the seq1 padding code 'B'
+    p2, b2 = pairs.copy(), buf.copy()
+    k = int(np.argmax(p2["len2"]))
+    b2[(2 * int(p2["id"][k]) + 1) * 1024 + 3] = ord("B")
+    _check(eng, 16, p2, b2, "B in seq2")
+    assert eng.stats()["regrouped"] == 0
+    # match * min(len1, len2) past the int16 lane
+    eng3 = T.PairWiseSW(40, -4, -6, -1)
+    s = [A[rng.integers(0, 4, 900)] for _ in range(40)]
+    p3, b3 = T.make_batch(s, [x.copy() for x in s], 16)
+    _check(eng3, 16, p3, b3, "int16 bound")
+    assert eng3.stats()["regrouped"] == 0
+    for e in (eng, eng2, eng3):
+        e.close()
+
+
+def test_regrouping_keeps_zero_bound_groups_8bit():
+    _say("[gpu] sw regrouping with 8-bit zero-bound groups")
+    rng = np.random.default_rng(32)
+    A = np.frombuffer(b"ACGT", np.uint8)
+    s1, s2 = [], []
+    for grp in range(20):
+        longest = 126 if grp % 3 == 0 else 100
+        for k in range(64):
+            L = longest if k == 5 else int(rng.integers(0, 100))
+            a = A[rng.integers(0, 4, L)]
+            s1.append(a)
+            s2.append(np.resize(a, int(rng.integers(0, 100))).copy())
+    pairs, buf = T.make_batch(s1, s2, 8)
+    eng = T.PairWiseSW(1, -3, -5, -2)
+    _check(eng, 8, pairs, buf, "8-bit zero groups")
+    assert eng.stats()["regrouped"] == 1
+    eng.close()

@@ -15,6 +15,11 @@ ARCH = os.environ.get("TAL_BSW_ARCH", "gfx950")
 
 HIP_SOURCES = ["bsw_kernels.hip", "tal_bsw.hip", "bsw_server.hip", "bsw_pairfile.cpp"]
 SW_SOURCES = ["sw_kernels.hip", "tal_sw.hip"]
+# per-source flags: the SW cell update is four independent packed-16-bit
+# chains per step; the ILP scheduler interleaves them, where the default one
+# keeps each chain together and pays a wait state between dependent packed
+# instructions
+SOURCE_FLAGS = {"sw_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"]}
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -33,7 +38,8 @@ def build_hip(out=None, extra=(), sources=None):
     objs = []
     for src in sources or HIP_SOURCES:
         obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
-        _run([HIPCC, *HIP_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
+        _run([HIPCC, *HIP_FLAGS, *SOURCE_FLAGS.get(src, []), *extra, "-c",
+              os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", out, "-lpthread"])
     for o in objs:

@@ -208,7 +208,7 @@ __device__ __forceinline__ uint32_t strip(const Cell<PACK> &C, const Slab &sl, i
 /* All rows in strips of 16, then 8 and 4 for the remainder (the bounds
  * are multiples of 4).  A strip is masked only where some unit of the wave
  * ends before the strip's last row or before the widest unit's last
- * column. */
+ * column; masked strips take at most 8 rows. */
 template <bool PACK>
 __device__ __forceinline__ uint32_t sweep(const Cell<PACK> &C, const Slab &sl, int lane, int NR,
                                           int NC, int nrow, int ncol, int min_nrow, int min_ncol)
@@ -217,12 +217,13 @@ __device__ __forceinline__ uint32_t sweep(const Cell<PACK> &C, const Slab &sl, i
     int r0 = 0;
     while (r0 < NR) {
         const int rem = NR - r0;
-        const int h = rem >= 16 ? 16 : rem >= 8 ? 8 : 4;
-        const bool first = r0 == 0, last = r0 + h >= NR;
+        int h = rem >= 16 ? 16 : rem >= 8 ? 8 : 4;
         const bool m = r0 + h > min_nrow || min_ncol < NC;
+        /* masked strips are at most 8 rows: the row masks take registers */
+        if (m && h == 16) h = 8;
+        const bool first = r0 == 0, last = r0 + h >= NR;
         if (h == 16) {
-            best = m ? strip<PACK, 16, true>(C, sl, lane, r0, NC, first, last, nrow, ncol, best)
-                     : strip<PACK, 16, false>(C, sl, lane, r0, NC, first, last, nrow, ncol, best);
+            best = strip<PACK, 16, false>(C, sl, lane, r0, NC, first, last, nrow, ncol, best);
         } else if (h == 8) {
             best = m ? strip<PACK, 8, true>(C, sl, lane, r0, NC, first, last, nrow, ncol, best)
                      : strip<PACK, 8, false>(C, sl, lane, r0, NC, first, last, nrow, ncol, best);
@@ -248,10 +249,24 @@ __device__ __forceinline__ int wave_min4(int v)
     return min(min(a, b), min(c, d));
 }
 
-/* byte k of a pair's sequence, or the padding code past its length */
-__device__ __forceinline__ uint32_t code_at(const uint8_t *s, int len, int k, uint32_t pad)
+/* Bytes k..k+3 of a sequence as one little-endian word, the padding code in
+ * the bytes at or past len.  Whole words are read as one aligned load when
+ * the sequences are 4-byte aligned (their slots are multiples of 128 bytes,
+ * so that is the buffer's own alignment); otherwise and for the last partial
+ * word, byte loads at clamped offsets, which every lane issues without
+ * branching.  `s` must be readable at offset 0 even when len is 0. */
+__device__ __forceinline__ uint32_t word_at(const uint8_t *s, int len, int k, uint32_t pad,
+                                            bool aligned)
 {
-    return k < len ? (uint32_t)s[k] : pad;
+    if (aligned && k + 4 <= len) return *(const uint32_t *)(s + k);
+    const int last = max(len - 1, 0);
+    uint32_t w = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const uint32_t c = s[min(k + i, last)];
+        w |= (k + i < len ? c : pad) << (8 * i);
+    }
+    return w;
 }
 
 } // namespace
@@ -262,8 +277,19 @@ __device__ __forceinline__ uint32_t code_at(const uint8_t *s, int len, int k, ui
  * 308, which wrap to -128 when they round up to 128), each unit's bounds and
  * sort key, the batch maxima and the error word.
  */
+__device__ __forceinline__ uint32_t spread_bits(uint32_t x)
+{
+    x &= 0xFFFF;
+    x = (x | (x << 8)) & 0x00FF00FFu;
+    x = (x | (x << 4)) & 0x0F0F0F0Fu;
+    x = (x | (x << 2)) & 0x33333333u;
+    x = (x | (x << 1)) & 0x55555555u;
+    return x;
+}
+
 __global__ void sw_shapes(const tal_sw_pair *pairs, int32_t n, int32_t bits, size_t seq_len,
-                          int32_t *unit_dims, uint32_t *keys, int32_t *ids, int32_t *ctl)
+                          int32_t *unit_dims, uint32_t *keys, int32_t *ids, int32_t *ctl,
+                          SwInert I)
 {
     const int g = blockIdx.x * blockDim.x + threadIdx.x;
     const int GL = bits == 8 ? 64 : bits == 16 ? 32 : 16;
@@ -304,6 +330,80 @@ __global__ void sw_shapes(const tal_sw_pair *pairs, int32_t n, int32_t bits, siz
     }
     atomicMax(&ctl[SW_CTL_MAXROW], nrow);
     atomicMax(&ctl[SW_CTL_MAXCOL], ncol);
+    if (!I.on) return;
+    /* Regrouping keys: pairs of a zero-bound group keep score 0 and sort
+     * last (key 0); the others sort by the Morton order of (len1, len2), so
+     * that runs of the sorted order have near lengths in both dimensions. */
+    int elig = 0, bad = 0;
+    for (int k = 0; k < GL; k++) {
+        const int p = g * GL + k;
+        if (p >= n) break;
+        const tal_sw_pair sp = pairs[p];
+        uint32_t key = 0;
+        if (nrow > 0 && sp.len1 >= 0 && sp.len2 >= 0) {
+            key = (spread_bits(sp.len1) | (spread_bits(sp.len2) << 1)) + 1;
+            elig++;
+            /* the best local score is at most match * min(len1, len2) */
+            if ((int64_t)max(I.match, 0) * min((int)sp.len1, (int)sp.len2) > I.lane_max) bad = 1;
+        }
+        I.pair_keys[p] = key;
+        I.pair_ids[p] = p;
+    }
+    if (elig) atomicAdd(&ctl[SW_CTL_ELIGIBLE], elig);
+    if (bad) atomicMax(&ctl[SW_CTL_NOT_INERT], 1);
+}
+
+/* One wave per pair: does seq1 hold the seq2 padding code 'D' or seq2 the
+ * seq1 padding code 'B'?  Then a padded cell could score a match and the
+ * reference's groups must be kept.  Pairs that fail validation are skipped
+ * (the bounds kernel reports them). */
+__global__ void sw_scan_codes(const tal_sw_pair *pairs, int32_t n, int32_t bits,
+                              const uint8_t *seq, size_t seq_len, int32_t *ctl)
+{
+    const int p = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (p >= n) return;
+    const tal_sw_pair sp = pairs[p];
+    const int64_t S = bits == 8 ? 128 : 1024;
+    const int limit = bits == 8 ? 127 : 1024;
+    if (sp.len1 < 0 || sp.len2 < 0 || sp.len1 > limit || sp.len2 > limit || sp.id < 0 ||
+        (2 * (int64_t)sp.id + 1) * S + sp.len2 > (int64_t)seq_len ||
+        2 * (int64_t)sp.id * S + sp.len1 > (int64_t)seq_len)
+        return;
+    const uint8_t *a = seq + 2 * (int64_t)sp.id * S, *b = a + S;
+    bool hit = false;
+    for (int k = lane; k < sp.len1; k += 64) hit |= a[k] == 'D';
+    for (int k = lane; k < sp.len2; k += 64) hit |= b[k] == 'B';
+    if (hit) ctl[SW_CTL_NOT_INERT] = 1;
+}
+
+/* Bounds and sort key of each regrouped unit (UL consecutive entries of the
+ * sorted pairs), in the layout of the bounds kernel. */
+__global__ void sw_unit_dims(const tal_sw_pair *pairs, const int32_t *perm, int32_t m,
+                             int32_t bits, int32_t *unit_dims, uint32_t *keys, int32_t *ids)
+{
+    const int UL = bits == 32 ? 16 : 32;
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= (m + UL - 1) / UL) return;
+    int max1 = 0, max2 = 0;
+    for (int k = 0; k < UL; k++) {
+        const int i = u * UL + k;
+        if (i >= m) break;
+        const tal_sw_pair sp = pairs[perm[i]];
+        max1 = max(max1, (int)sp.len1);
+        max2 = max(max2, (int)sp.len2);
+    }
+    int nrow = ((max1 + 3) / 4) * 4, ncol = ((max2 + 3) / 4) * 4;
+    if (nrow <= 0 || ncol <= 0) nrow = ncol = 0;
+    unit_dims[u] = (nrow << 16) | ncol;
+    keys[u] = ((uint32_t)(ncol >> 2) << 10) | (uint32_t)(nrow >> 2);
+    ids[u] = u;
+}
+
+__global__ void sw_zero_scores(tal_sw_pair *pairs, const int32_t *idx, int32_t count)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) pairs[idx[k]].score = 0;
 }
 
 /*
@@ -343,48 +443,65 @@ __global__ void __launch_bounds__(64) sw_main(SwParams P, SwBuffers B)
         const int min_nrow = wave_min4(nrow), min_ncol = wave_min4(ncol);
         /* the lane's pairs: k and k+16 of the unit (8/16-bit), k (32-bit) */
         const int ul = PACK ? 32 : 16;
-        const int pa = unit >= 0 ? unit * ul + l16 : P.n;
-        const int pb = PACK ? pa + 16 : P.n;
+        int pa, pb;
+        if (B.perm) { /* regrouped units */
+            const int i = unit * ul + l16;
+            pa = unit >= 0 && i < P.m ? B.perm[i] : P.n;
+            pb = PACK && unit >= 0 && i + 16 < P.m ? B.perm[i + 16] : P.n;
+        } else {
+            pa = unit >= 0 ? unit * ul + l16 : P.n;
+            pb = PACK ? pa + 16 : P.n;
+        }
         const bool va = pa < P.n, vb = PACK && unit >= 0 && pb < P.n;
         int la1 = 0, la2 = 0, lb1 = 0, lb2 = 0;
-        const uint8_t *a1 = B.seq, *a2 = B.seq, *b1 = B.seq, *b2 = B.seq;
+        /* an always-readable address for empty or absent sequences */
+        const uint8_t *safe = (const uint8_t *)B.pairs;
+        const uint8_t *a1 = safe, *a2 = safe, *b1 = safe, *b2 = safe;
         if (va) {
             const tal_sw_pair sp = B.pairs[pa];
             la1 = sp.len1;
             la2 = sp.len2;
-            a1 = B.seq + 2 * (int64_t)sp.id * P.slot;
-            a2 = a1 + P.slot;
+            if (la1 > 0) a1 = B.seq + 2 * (int64_t)sp.id * P.slot;
+            if (la2 > 0) a2 = B.seq + (2 * (int64_t)sp.id + 1) * P.slot;
         }
         if (vb) {
             const tal_sw_pair sp = B.pairs[pb];
             lb1 = sp.len1;
             lb2 = sp.len2;
-            b1 = B.seq + 2 * (int64_t)sp.id * P.slot;
-            b2 = b1 + P.slot;
+            if (lb1 > 0) b1 = B.seq + 2 * (int64_t)sp.id * P.slot;
+            if (lb2 > 0) b2 = B.seq + (2 * (int64_t)sp.id + 1) * P.slot;
         }
         /* stage the codes; rows/columns past a pair's length take the
-         * padding codes 'B' / 'D' up to the unit's bounds */
+         * padding codes 'B' / 'D' up to the unit's bounds.  Packed: row k
+         * of pair a in the low half and of pair b in the high half. */
+        const bool al = P.aligned4 != 0;
         for (int r = 0; r < NR; r += 4) {
+            const uint32_t wa = word_at(a1, la1, r, 'B', al);
             u32x4 v;
-            uint32_t w[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                uint32_t c = code_at(a1, la1, r + k, 'B');
-                if (PACK) c |= code_at(b1, lb1, r + k, 'B') << 16;
-                w[k] = c;
+            if (PACK) {
+                const uint32_t wb = word_at(b1, lb1, r, 'B', al);
+                v = u32x4{__builtin_amdgcn_perm(wb, wa, 0x0C040C00u),
+                          __builtin_amdgcn_perm(wb, wa, 0x0C050C01u),
+                          __builtin_amdgcn_perm(wb, wa, 0x0C060C02u),
+                          __builtin_amdgcn_perm(wb, wa, 0x0C070C03u)};
+            } else {
+                v = u32x4{wa & 0xFF, (wa >> 8) & 0xFF, (wa >> 16) & 0xFF, wa >> 24};
             }
-            v = u32x4{w[0], w[1], w[2], w[3]};
             ((u32x4 *)sl.codes1)[(size_t)(r >> 2) * 64 + lane] = v;
         }
         for (int c = 0; c < NC; c += 4) {
-            uint32_t w[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                uint32_t x = code_at(a2, la2, c + k, 'D');
-                if (PACK) x |= code_at(b2, lb2, c + k, 'D') << 16;
-                w[k] = x;
+            const uint32_t wa = word_at(a2, la2, c, 'D', al);
+            u32x4 v;
+            if (PACK) {
+                const uint32_t wb = word_at(b2, lb2, c, 'D', al);
+                v = u32x4{__builtin_amdgcn_perm(wb, wa, 0x0C040C00u),
+                          __builtin_amdgcn_perm(wb, wa, 0x0C050C01u),
+                          __builtin_amdgcn_perm(wb, wa, 0x0C060C02u),
+                          __builtin_amdgcn_perm(wb, wa, 0x0C070C03u)};
+            } else {
+                v = u32x4{wa & 0xFF, (wa >> 8) & 0xFF, (wa >> 16) & 0xFF, wa >> 24};
             }
-            ((u32x4 *)sl.codes2)[(size_t)(c >> 2) * 64 + lane] = u32x4{w[0], w[1], w[2], w[3]};
+            ((u32x4 *)sl.codes2)[(size_t)(c >> 2) * 64 + lane] = v;
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
         const uint32_t best = sweep<PACK>(C, sl, lane, NR, NC, nrow, ncol, min_nrow, min_ncol);
@@ -402,13 +519,43 @@ __global__ void __launch_bounds__(64) sw_main(SwParams P, SwBuffers B)
 
 hipError_t launch_sw_shapes(const tal_sw_pair *pairs, int32_t n, int32_t bits, size_t seq_len,
                             int32_t *unit_dims, uint32_t *keys, int32_t *ids, int32_t *ctl,
-                            hipStream_t s)
+                            const SwInert &I, hipStream_t s)
 {
     const int GL = bits == 8 ? 64 : bits == 16 ? 32 : 16;
     const int G = (n + GL - 1) / GL;
     if (G == 0) return hipSuccess;
     hipLaunchKernelGGL(sw_shapes, dim3((G + 127) / 128), dim3(128), 0, s, pairs, n, bits, seq_len,
-                       unit_dims, keys, ids, ctl);
+                       unit_dims, keys, ids, ctl, I);
+    return hipGetLastError();
+}
+
+hipError_t launch_sw_scan_codes(const tal_sw_pair *pairs, int32_t n, int32_t bits,
+                                const uint8_t *seq, size_t seq_len, int32_t *ctl, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(sw_scan_codes, dim3((n + 3) / 4), dim3(256), 0, s, pairs, n, bits, seq,
+                       seq_len, ctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_sw_unit_dims(const tal_sw_pair *pairs, const int32_t *perm, int32_t m,
+                               int32_t bits, int32_t *unit_dims, uint32_t *keys, int32_t *ids,
+                               hipStream_t s)
+{
+    const int UL = bits == 32 ? 16 : 32;
+    const int U = (m + UL - 1) / UL;
+    if (U == 0) return hipSuccess;
+    hipLaunchKernelGGL(sw_unit_dims, dim3((U + 127) / 128), dim3(128), 0, s, pairs, perm, m, bits,
+                       unit_dims, keys, ids);
+    return hipGetLastError();
+}
+
+hipError_t launch_sw_zero_scores(tal_sw_pair *pairs, const int32_t *idx, int32_t count,
+                                 hipStream_t s)
+{
+    if (count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(sw_zero_scores, dim3((count + 255) / 256), dim3(256), 0, s, pairs, idx,
+                       count);
     return hipGetLastError();
 }
 

@@ -64,8 +64,12 @@ struct tal_sw {
     int device = 0;
     hipStream_t stream = nullptr;
     int cu_count = 0;
-    int waves_per_cu = 8; /* persistent waves per CU (TAL_SW_WAVES_PER_CU) */
+    /* persistent waves per CU (TAL_SW_WAVES_PER_CU): the DP kernel's 168
+     * VGPRs leave room for three waves per SIMD */
+    int waves_per_cu = 12;
     Buf dims, keys, keys2, ids, order, ctl, slabs, cubtmp, d_pairs, d_seq;
+    Buf pkeys, pkeys2, pids, perm; /* regrouping: per-pair keys and order */
+    bool regroup = true; /* TAL_SW_REGROUP=0 keeps the reference's groups */
     int32_t *ctl_host = nullptr; /* pinned copy of the control words */
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     bool timed = false;
@@ -121,12 +125,52 @@ SwParams lane_constants(const tal_sw *h, int bits)
     return P;
 }
 
+/* Lane value of a weight in the class (the reference's set1 truncation). */
+int64_t lane_value(int bits, int32_t w)
+{
+    return bits == 8 ? (int8_t)w : bits == 16 ? (int16_t)w : w;
+}
+
+/*
+ * Can padding change a score?  Every pair of a reference group is scored
+ * over the group's padded matrix; a padded cell holds a padding code on at
+ * least one side.  With mismatch, gap open and gap extend all <= 0 at the
+ * class width, a padded cell whose codes differ scores at most the best of
+ * the cells above and to its left, so by induction no padded cell exceeds
+ * the pair's own best, and cells inside the pair's own lengths never depend
+ * on padded ones.  The codes differ unless seq1 holds 'D' or seq2 holds 'B'
+ * (scanned on the device), and no lane value wraps when match * min(len1,
+ * len2) fits the lane and open + extend does not underflow (checked per
+ * pair).  When all of this holds a pair's score is its own DP's, whatever
+ * its group, and pairs may be regrouped by length: this removes the work on
+ * padded cells.  Zero-bound groups of the 8-bit class still score 0.
+ */
+bool weights_allow_regroup(const tal_sw *h, int bits)
+{
+    const int64_t mis = lane_value(bits, h->w_mismatch), open = lane_value(bits, h->w_open);
+    const int64_t ext = lane_value(bits, h->w_extend);
+    const int64_t lane_min = bits == 8 ? -128 : bits == 16 ? -32768 : INT32_MIN;
+    return mis <= 0 && open <= 0 && ext <= 0 && open + ext >= lane_min;
+}
+
 int run(tal_sw *h, int bits, tal_sw_pair *d_pairs, const uint8_t *d_seq, size_t seq_len,
         int32_t n, hipStream_t s)
 {
     const int GL = group_lanes(bits), UL = unit_lanes(bits);
     const int32_t G = (n + GL - 1) / GL;
-    const int32_t U = G * (GL / UL);
+    int32_t U = G * (GL / UL);
+    SwInert I{};
+    I.on = h->regroup && weights_allow_regroup(h, bits);
+    I.match = (int32_t)lane_value(bits, h->w_match);
+    I.lane_max = bits == 8 ? 127 : bits == 16 ? 32767 : INT32_MAX;
+    if (I.on) {
+        SWCHK(h, h->pkeys.ensure(sizeof(uint32_t) * n), "alloc pair keys");
+        SWCHK(h, h->pkeys2.ensure(sizeof(uint32_t) * n), "alloc pair keys");
+        SWCHK(h, h->pids.ensure(sizeof(int32_t) * n), "alloc pair ids");
+        SWCHK(h, h->perm.ensure(sizeof(int32_t) * n), "alloc pair order");
+        I.pair_keys = h->pkeys.as<uint32_t>();
+        I.pair_ids = h->pids.as<int32_t>();
+    }
     SWCHK(h, h->dims.ensure(sizeof(int32_t) * U), "alloc dims");
     SWCHK(h, h->keys.ensure(sizeof(uint32_t) * U), "alloc keys");
     SWCHK(h, h->keys2.ensure(sizeof(uint32_t) * U), "alloc keys");
@@ -137,8 +181,9 @@ int run(tal_sw *h, int bits, tal_sw_pair *d_pairs, const uint8_t *d_seq, size_t 
     SWCHK(h, hipEventRecord(h->ev[0], s), "event");
     SWCHK(h, hipMemsetAsync(ctl, 0, sizeof(int32_t) * SW_CTL_WORDS, s), "clear ctl");
     SWCHK(h, launch_sw_shapes(d_pairs, n, bits, seq_len, h->dims.as<int32_t>(),
-                              h->keys.as<uint32_t>(), h->ids.as<int32_t>(), ctl, s),
+                              h->keys.as<uint32_t>(), h->ids.as<int32_t>(), ctl, I, s),
           "shapes");
+    if (I.on) SWCHK(h, launch_sw_scan_codes(d_pairs, n, bits, d_seq, seq_len, ctl, s), "scan");
     SWCHK(h, hipMemcpyAsync(h->ctl_host, ctl, sizeof(int32_t) * SW_CTL_WORDS,
                             hipMemcpyDeviceToHost, s),
           "read ctl");
@@ -151,6 +196,29 @@ int run(tal_sw *h, int bits, tal_sw_pair *d_pairs, const uint8_t *d_seq, size_t 
         return fail(h, TAL_EINVAL, "a pair's id is negative or its sequences lie outside the "
                                    "%zu-byte sequence buffer", seq_len);
     const int max_rows = h->ctl_host[SW_CTL_MAXROW], max_cols = h->ctl_host[SW_CTL_MAXCOL];
+    const bool regrouped = I.on && h->ctl_host[SW_CTL_NOT_INERT] == 0;
+    int32_t m = 0;
+    if (regrouped) {
+        /* pairs by Morton key (zero-bound groups last, score 0), then units of
+         * UL consecutive sorted pairs */
+        m = h->ctl_host[SW_CTL_ELIGIBLE];
+        size_t ptmp = 0;
+        SWCHK(h, hipcub::DeviceRadixSort::SortPairsDescending(
+                     nullptr, ptmp, h->pkeys.as<uint32_t>(), h->pkeys2.as<uint32_t>(),
+                     h->pids.as<int32_t>(), h->perm.as<int32_t>(), n, 0, 24, s),
+              "sort size");
+        SWCHK(h, h->cubtmp.ensure(ptmp + 16), "alloc sort tmp");
+        SWCHK(h, hipcub::DeviceRadixSort::SortPairsDescending(
+                     h->cubtmp.p, ptmp, h->pkeys.as<uint32_t>(), h->pkeys2.as<uint32_t>(),
+                     h->pids.as<int32_t>(), h->perm.as<int32_t>(), n, 0, 24, s),
+              "sort pairs");
+        SWCHK(h, launch_sw_zero_scores(d_pairs, h->perm.as<int32_t>() + m, n - m, s), "zero");
+        SWCHK(h, launch_sw_unit_dims(d_pairs, h->perm.as<int32_t>(), m, bits,
+                                     h->dims.as<int32_t>(), h->keys.as<uint32_t>(),
+                                     h->ids.as<int32_t>(), s),
+              "unit bounds");
+        U = (m + UL - 1) / UL;
+    }
 
     size_t tmp = 0;
     SWCHK(h, hipcub::DeviceRadixSort::SortPairsDescending(
@@ -158,10 +226,11 @@ int run(tal_sw *h, int bits, tal_sw_pair *d_pairs, const uint8_t *d_seq, size_t 
                  h->ids.as<int32_t>(), h->order.as<int32_t>(), U, 0, 20, s),
           "sort size");
     SWCHK(h, h->cubtmp.ensure(tmp + 16), "alloc sort tmp");
-    SWCHK(h, hipcub::DeviceRadixSort::SortPairsDescending(
-                 h->cubtmp.p, tmp, h->keys.as<uint32_t>(), h->keys2.as<uint32_t>(),
-                 h->ids.as<int32_t>(), h->order.as<int32_t>(), U, 0, 20, s),
-          "sort units");
+    if (U > 0)
+        SWCHK(h, hipcub::DeviceRadixSort::SortPairsDescending(
+                     h->cubtmp.p, tmp, h->keys.as<uint32_t>(), h->keys2.as<uint32_t>(),
+                     h->ids.as<int32_t>(), h->order.as<int32_t>(), U, 0, 20, s),
+              "sort units");
 
     SwParams P = lane_constants(h, bits);
     P.n = n;
@@ -169,8 +238,13 @@ int run(tal_sw *h, int bits, tal_sw_pair *d_pairs, const uint8_t *d_seq, size_t 
     P.slot = bits == 8 ? 128 : 1024;
     P.rows_cap = std::max(4, max_rows);
     P.cols_cap = std::max(4, max_cols);
+    P.aligned4 = ((uintptr_t)d_seq & 3) == 0;
+    P.m = m;
     P.slab_words = (size_t)64 * (P.rows_cap + 3 * (size_t)P.cols_cap);
     const int ntasks = (U + 3) / 4;
+    h->st.regrouped = regrouped ? 1 : 0;
+    h->st.units = U;
+    if (ntasks == 0) return TAL_OK; /* every pair in a zero-bound group */
     const int grid = std::max(1, std::min(ntasks, h->cu_count * h->waves_per_cu));
     SWCHK(h, h->slabs.ensure(sizeof(uint32_t) * P.slab_words * grid), "alloc slabs");
     SwBuffers B{};
@@ -178,13 +252,13 @@ int run(tal_sw *h, int bits, tal_sw_pair *d_pairs, const uint8_t *d_seq, size_t 
     B.seq = d_seq;
     B.unit_dims = h->dims.as<int32_t>();
     B.order = h->order.as<int32_t>();
+    B.perm = regrouped ? h->perm.as<int32_t>() : nullptr;
     B.ctl = ctl;
     B.slabs = h->slabs.as<uint32_t>();
     SWCHK(h, hipEventRecord(h->ev[1], s), "event");
     SWCHK(h, launch_sw_main(bits, grid, P, B, s), "launch");
     SWCHK(h, hipEventRecord(h->ev[2], s), "event");
     h->timed = true;
-    h->st.units = U;
     h->st.max_rows = max_rows;
     h->st.max_cols = max_cols;
     return TAL_OK;
@@ -223,6 +297,7 @@ int tal_sw_create(int32_t w_match, int32_t w_mismatch, int32_t w_open, int32_t w
     h->w_open = w_open;
     h->w_extend = w_extend;
     h->device = device;
+    if (const char *v = getenv("TAL_SW_REGROUP")) h->regroup = atoi(v) != 0;
     if (const char *v = getenv("TAL_SW_WAVES_PER_CU")) {
         const int x = atoi(v);
         if (x >= 1 && x <= 32) h->waves_per_cu = x;
@@ -259,7 +334,7 @@ void tal_sw_destroy(tal_sw *h)
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     for (Buf *b : {&h->dims, &h->keys, &h->keys2, &h->ids, &h->order, &h->ctl, &h->slabs,
-                   &h->cubtmp, &h->d_pairs, &h->d_seq})
+                   &h->cubtmp, &h->d_pairs, &h->d_seq, &h->pkeys, &h->pkeys2, &h->pids, &h->perm})
         b->release();
     if (h->ctl_host) (void)hipHostFree(h->ctl_host);
     for (auto e : h->ev)
