@@ -43,6 +43,7 @@ typedef struct tal_sw_stats {
     int32_t max_cols;
     int32_t regrouped; /* 1: the pairs were regrouped by length (padding could
                           not change any score, see tal_sw_get_scores) */
+    int32_t split;     /* wave quarters per unit of the DP kernel (1, 2 or 4) */
 } tal_sw_stats;
 
 /* Sequence slot S of a class: MAX_SEQ_LEN_8 = 128, MAX_SEQ_LEN_16 =

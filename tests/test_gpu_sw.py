@@ -229,3 +229,21 @@ def test_regrouping_keeps_zero_bound_groups_8bit():
     _check(eng, 8, pairs, buf, "8-bit zero groups")
     assert eng.stats()["regrouped"] == 1
     eng.close()
+
+
+@pytest.mark.parametrize("split", [1, 2, 4])
+def test_every_unit_split(split, monkeypatch):
+    """The DP kernel's unit split (one unit per quarter, or split over two or
+    four quarters) gives the same scores, in both the reference's groups and
+    regrouped order, for ragged and uniform bounds."""
+    _say(f"[gpu] sw split={split}")
+    monkeypatch.setenv("TAL_SW_SPLIT", str(split))
+    rng = np.random.default_rng(40 + split)
+    for bits, lo, hi, n in [(16, 0, 300, 901), (8, 0, 124, 1200), (32, 0, 300, 450),
+                            (16, 700, 1024, 90)]:
+        pairs, buf = _rand_batch(rng, n, bits, lo, hi)
+        for params in [(1, -1, 1, 1), (2, -4, -6, -1)]:
+            eng = T.PairWiseSW(*params)
+            _check(eng, bits, pairs, buf, f"split={split} bits={bits} {params}")
+            assert eng.stats()["split"] == split
+            eng.close()

@@ -71,7 +71,7 @@ hipError_t launch_sw_unit_dims(const tal_sw_pair *pairs, const int32_t *perm, in
                                hipStream_t s);
 hipError_t launch_sw_zero_scores(tal_sw_pair *pairs, const int32_t *idx, int32_t count,
                                  hipStream_t s);
-hipError_t launch_sw_main(int bits, int grid, const SwParams &P, const SwBuffers &B,
+hipError_t launch_sw_main(int bits, int split, int grid, const SwParams &P, const SwBuffers &B,
                           hipStream_t s);
 
 } // namespace talsw

@@ -122,6 +122,46 @@ struct Slab {
 };
 
 /*
+ * One 4-column tile of an H-row strip, the cells in anti-diagonal order:
+ * step k visits (k - cc, cc) for the four columns, which do not depend on
+ * each other, so dependent packed instructions are four cells apart.
+ * Column cc carries its F down and the H it takes as the next row's
+ * diagonal; row r carries E right (e[r]), and hp[r] hands H(r, cc) to
+ * column cc + 1.  hu/fu: H and F of the row above the strip for the tile's
+ * columns; hup: H above the column left of the tile; b: the tile's seq2
+ * codes.  Outputs the strip's bottom-row H (ho) and the F below it (fo).
+ * MASK: cells in rows >= rv or in a tile with cm == 0 stay out of the
+ * maxima.
+ */
+template <bool PACK, int H, bool MASK>
+__device__ __forceinline__ void tile_cells(const Cell<PACK> &C, const uint32_t (&s1)[H],
+                                           uint32_t (&e)[H], uint32_t (&hp)[H],
+                                           uint32_t (&acc)[4], uint32_t &hup, const u32x4 &hu,
+                                           const u32x4 &fu, const u32x4 &b, int rv, uint32_t cm,
+                                           uint32_t (&ho)[4], uint32_t (&fo)[4])
+{
+    uint32_t f[4] = {fu.x, fu.y, fu.z, fu.w};
+    uint32_t d[4] = {hup, hu.x, hu.y, hu.z};
+    hup = hu.w;
+    const uint32_t bc[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+    for (int k = 0; k < H + 3; k++) {
+#pragma unroll
+        for (int cc = 0; cc < 4; cc++) {
+            const int r = k - cc;
+            if (r < 0 || r >= H) continue;
+            const uint32_t h = C.step(s1[r], bc[cc], d[cc], e[r], f[cc]);
+            acc[cc] = C.best(acc[cc], MASK ? (r < rv ? (h & cm) : 0u) : h);
+            d[cc] = hp[r];
+            hp[r] = h;
+            if (r == H - 1) ho[cc] = h;
+        }
+    }
+#pragma unroll
+    for (int cc = 0; cc < 4; cc++) fo[cc] = f[cc];
+}
+
+/*
  * One strip of H rows starting at r0, over columns [0, NC).  MASK: some
  * lane's unit ends inside the strip or before NC, so cells past the unit's
  * own bounds (rows >= nrow, columns >= ncol) are kept out of its maximum.
@@ -166,41 +206,121 @@ __device__ __forceinline__ uint32_t strip(const Cell<PACK> &C, const Slab &sl, i
                 nf = st[(size_t)(t + 1) * 128 + 1];
             }
         }
-        uint32_t rm[H];
-        if (MASK) {
-            const uint32_t cm = c0 < ncol ? 0xFFFFFFFFu : 0u;
-#pragma unroll
-            for (int r = 0; r < H; r++) rm[r] = r < rv ? cm : 0u;
-        }
-        /* the tile's 16 x 4 cells in anti-diagonal order: step k visits
-         * (k - cc, cc) for the four columns, which do not depend on each
-         * other, so dependent packed instructions are four cells apart.
-         * Column cc carries its F down and the H it takes as the next row's
-         * diagonal; row r carries E right, and hp[r] hands H(r, cc) to
-         * column cc + 1. */
-        uint32_t f[4] = {fu.x, fu.y, fu.z, fu.w};
-        uint32_t d[4] = {hup, hu.x, hu.y, hu.z};
-        hup = hu.w;
-        const uint32_t bc[4] = {b.x, b.y, b.z, b.w};
-        uint32_t ho[4];
-#pragma unroll
-        for (int k = 0; k < H + 3; k++) {
-#pragma unroll
-            for (int cc = 0; cc < 4; cc++) {
-                const int r = k - cc;
-                if (r < 0 || r >= H) continue;
-                const uint32_t h = C.step(s1[r], bc[cc], d[cc], e[r], f[cc]);
-                acc[cc] = C.best(acc[cc], MASK ? (h & rm[r]) : h);
-                d[cc] = hp[r];
-                hp[r] = h;
-                if (r == H - 1) ho[cc] = h;
-            }
-        }
-        const uint32_t *fo = f;
+        uint32_t ho[4], fo[4];
+        tile_cells<PACK, H, MASK>(C, s1, e, hp, acc, hup, hu, fu, b, rv,
+                                  c0 < ncol ? 0xFFFFFFFFu : 0u, ho, fo);
         if (!last) {
             st[(size_t)t * 128] = u32x4{ho[0], ho[1], ho[2], ho[3]};
             st[(size_t)t * 128 + 1] = u32x4{fo[0], fo[1], fo[2], fo[3]};
         }
+    }
+    return C.best(C.best(acc[0], acc[1]), C.best(acc[2], acc[3]));
+}
+
+__device__ __forceinline__ uint32_t from_lane(int src, uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_ds_bpermute(src << 2, (int)v);
+}
+
+/*
+ * One round of a unit split over K quarters of the wave (K = 2 or 4).
+ * Strips of 16 rows go round-robin to the unit's quarters: in round j,
+ * quarter s sweeps strip jK + s, one 4-column tile behind quarter s - 1,
+ * whose strip lies directly above.  After each tile the bottom row of every
+ * quarter's tile (H and F of four columns) is handed to the quarter below
+ * across lanes, so only quarter 0 reads the state slab (the bottom row of the
+ * previous round's last strip) and only quarter K - 1 writes it.
+ * act: this quarter has a strip in this round; rv: rows of it inside the
+ * lane's unit.  `ul` indexes the unit's lanes in the slab's state rows.
+ */
+template <bool PACK, int K, bool MASK>
+__device__ __forceinline__ void round_split(const Cell<PACK> &C, const Slab &sl, int lane, int s,
+                                            int ul, int r0, bool act, int rv, int NC, int ncol,
+                                            bool first, bool last, uint32_t (&acc)[4])
+{
+    constexpr int H = 16;
+    uint32_t s1[H], e[H], hp[H];
+    const u32x4 *c1 = (const u32x4 *)sl.codes1 + (size_t)(r0 >> 2) * 64 + lane;
+#pragma unroll
+    for (int k = 0; k < H / 4; k++) {
+        const u32x4 v = c1[(size_t)k * 64];
+        s1[4 * k] = v.x;
+        s1[4 * k + 1] = v.y;
+        s1[4 * k + 2] = v.z;
+        s1[4 * k + 3] = v.w;
+    }
+#pragma unroll
+    for (int r = 0; r < H; r++) {
+        e[r] = C.open;
+        hp[r] = 0;
+    }
+    const int ntiles = NC >> 2;
+    const u32x4 *c2 = (const u32x4 *)sl.codes2 + lane;
+    u32x4 *st = (u32x4 *)sl.state + 2 * ul;
+    const u32x4 open4 = {C.open, C.open, C.open, C.open};
+    const u32x4 zero4 = {0u, 0u, 0u, 0u};
+    const int src = s > 0 ? lane - 16 : lane;
+    uint32_t hup = 0;
+    u32x4 nb = c2[0];
+    u32x4 nh = zero4, nf = open4; /* quarter 0: the slab row above, prefetched */
+    if (s == 0 && !first) {
+        nh = st[0];
+        nf = st[1];
+    }
+    u32x4 xh = zero4, xf = open4; /* quarters > 0: handed down by quarter s - 1 */
+    for (int tau = 0; tau < ntiles + K - 1; tau++) {
+        const int c = tau - s;
+        const bool inr = act && c >= 0 && c < ntiles;
+        const u32x4 b = nb;
+        const u32x4 hu = s == 0 ? nh : xh, fu = s == 0 ? nf : xf;
+        const int cn = c + 1;
+        if (cn >= 0 && cn < ntiles) { /* prefetch the tile of the next step */
+            nb = c2[(size_t)cn * 64];
+            if (s == 0 && !first) {
+                nh = st[(size_t)cn * 128];
+                nf = st[(size_t)cn * 128 + 1];
+            }
+        }
+        uint32_t ho[4] = {0u, 0u, 0u, 0u}, fo[4] = {0u, 0u, 0u, 0u};
+        if (inr) {
+            tile_cells<PACK, H, MASK>(C, s1, e, hp, acc, hup, hu, fu, b, rv,
+                                      4 * c < ncol ? 0xFFFFFFFFu : 0u, ho, fo);
+            if (s == K - 1 && !last) {
+                st[(size_t)c * 128] = u32x4{ho[0], ho[1], ho[2], ho[3]};
+                st[(size_t)c * 128 + 1] = u32x4{fo[0], fo[1], fo[2], fo[3]};
+            }
+        }
+        xh = u32x4{from_lane(src, ho[0]), from_lane(src, ho[1]), from_lane(src, ho[2]),
+                   from_lane(src, ho[3])};
+        xf = u32x4{from_lane(src, fo[0]), from_lane(src, fo[1]), from_lane(src, fo[2]),
+                   from_lane(src, fo[3])};
+    }
+}
+
+/* All rounds of a split unit; returns the lane's best over its strips. */
+template <bool PACK, int K>
+__device__ __forceinline__ uint32_t sweep_split(const Cell<PACK> &C, const Slab &sl, int lane,
+                                                int s, int ul, int NR, int NC, int nrow, int ncol,
+                                                int min_nrow, int min_ncol)
+{
+    uint32_t acc[4] = {0u, 0u, 0u, 0u};
+    const int nstrips = (NR + 15) >> 4;
+    const int nrounds = (nstrips + K - 1) / K;
+    for (int j = 0; j < nrounds; j++) {
+        const int si = j * K + s;
+        const bool act = si < nstrips;
+        const int r0 = (act ? si : nstrips - 1) * 16; /* inactive quarters idle */
+        const bool first = j == 0, last = j == nrounds - 1;
+        /* masked where a unit ends inside this round's rows or before NC */
+        const bool m = (j + 1) * K * 16 > min_nrow || min_ncol < NC;
+        if (m)
+            round_split<PACK, K, true>(C, sl, lane, s, ul, r0, act, nrow - r0, NC, ncol, first,
+                                       last, acc);
+        else
+            round_split<PACK, K, false>(C, sl, lane, s, ul, r0, act, nrow - r0, NC, ncol, first,
+                                        last, acc);
+        /* the next round's quarter 0 reads what quarter K - 1 stored */
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     }
     return C.best(C.best(acc[0], acc[1]), C.best(acc[2], acc[3]));
 }
@@ -407,15 +527,19 @@ __global__ void sw_zero_scores(tal_sw_pair *pairs, const int32_t *idx, int32_t c
 }
 
 /*
- * The DP.  Persistent grid of one-wave blocks; a wave takes the next four
- * units of the sorted order from the task counter until none is left.
+ * The DP.  Persistent grid of one-wave blocks; a wave takes the next 4 / K
+ * units of the sorted order from the task counter until none is left.  K = 1:
+ * each quarter sweeps a unit of its own; K = 2 or 4: each unit is split over
+ * K quarters (round_split), for batches with too few units to fill the GPU.
  */
-template <int BITS>
-__global__ void __launch_bounds__(64) sw_main(SwParams P, SwBuffers B)
+template <int BITS, int K>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) sw_main(SwParams P, SwBuffers B)
 {
     constexpr bool PACK = BITS != 32;
+    constexpr int UPW = 4 / K; /* units per wave */
     const int lane = threadIdx.x;
     const int q = lane >> 4, l16 = lane & 15;
+    const int j = q / K, s = q % K; /* unit slot and the quarter's part of it */
     Slab sl;
     {
         uint32_t *base = B.slabs + (size_t)blockIdx.x * P.slab_words;
@@ -429,13 +553,13 @@ __global__ void __launch_bounds__(64) sw_main(SwParams P, SwBuffers B)
     } else {
         C = Cell<false>{P.k_match, P.k_delta, P.k_open, P.k_ext};
     }
-    const int ntasks = (P.nunits + 3) >> 2;
+    const int ntasks = (P.nunits + UPW - 1) / UPW;
     for (;;) {
         int task = 0;
         if (lane == 0) task = atomicAdd(&B.ctl[SW_CTL_COUNTER], 1);
         task = __builtin_amdgcn_readfirstlane(task);
         if (task >= ntasks) break;
-        const int ui = task * 4 + q;
+        const int ui = task * UPW + j;
         const int unit = ui < P.nunits ? B.order[ui] : -1;
         const int dims = unit >= 0 ? B.unit_dims[unit] : 0;
         const int nrow = dims >> 16, ncol = dims & 0xFFFF;
@@ -475,7 +599,9 @@ __global__ void __launch_bounds__(64) sw_main(SwParams P, SwBuffers B)
          * padding codes 'B' / 'D' up to the unit's bounds.  Packed: row k
          * of pair a in the low half and of pair b in the high half. */
         const bool al = P.aligned4 != 0;
-        for (int r = 0; r < NR; r += 4) {
+        /* split units sweep whole 16-row strips: stage rows up to that */
+        const int NRs = K > 1 ? (NR + 15) & ~15 : NR;
+        for (int r = 0; r < NRs; r += 4) {
             const uint32_t wa = word_at(a1, la1, r, 'B', al);
             u32x4 v;
             if (PACK) {
@@ -504,8 +630,19 @@ __global__ void __launch_bounds__(64) sw_main(SwParams P, SwBuffers B)
             ((u32x4 *)sl.codes2)[(size_t)(c >> 2) * 64 + lane] = v;
         }
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-        const uint32_t best = sweep<PACK>(C, sl, lane, NR, NC, nrow, ncol, min_nrow, min_ncol);
-        if (PACK) {
+        uint32_t best;
+        if constexpr (K == 1) {
+            best = sweep<PACK>(C, sl, lane, NR, NC, nrow, ncol, min_nrow, min_ncol);
+        } else {
+            best = sweep_split<PACK, K>(C, sl, lane, s, j * 16 + l16, NR, NC, nrow, ncol,
+                                        min_nrow, min_ncol);
+            /* the unit's best over its K quarters */
+            best = C.best(best, from_lane(lane ^ 16, best));
+            if (K == 4) best = C.best(best, from_lane(lane ^ 32, best));
+        }
+        if (s != 0) {
+            /* scores are written by the unit's first quarter */
+        } else if (PACK) {
             const int16_t lo = (int16_t)(best & 0xFFFF), hi = (int16_t)(best >> 16);
             if (va) B.pairs[pa].score = BITS == 8 ? (lo >> 8) : lo;
             if (vb) B.pairs[pb].score = BITS == 8 ? (hi >> 8) : hi;
@@ -559,14 +696,26 @@ hipError_t launch_sw_zero_scores(tal_sw_pair *pairs, const int32_t *idx, int32_t
     return hipGetLastError();
 }
 
-hipError_t launch_sw_main(int bits, int grid, const SwParams &P, const SwBuffers &B, hipStream_t s)
+template <int BITS>
+void launch_width(int split, int grid, const SwParams &P, const SwBuffers &B, hipStream_t s)
+{
+    if (split == 4)
+        hipLaunchKernelGGL((sw_main<BITS, 4>), dim3(grid), dim3(64), 0, s, P, B);
+    else if (split == 2)
+        hipLaunchKernelGGL((sw_main<BITS, 2>), dim3(grid), dim3(64), 0, s, P, B);
+    else
+        hipLaunchKernelGGL((sw_main<BITS, 1>), dim3(grid), dim3(64), 0, s, P, B);
+}
+
+hipError_t launch_sw_main(int bits, int split, int grid, const SwParams &P, const SwBuffers &B,
+                          hipStream_t s)
 {
     if (bits == 8)
-        hipLaunchKernelGGL(sw_main<8>, dim3(grid), dim3(64), 0, s, P, B);
+        launch_width<8>(split, grid, P, B, s);
     else if (bits == 16)
-        hipLaunchKernelGGL(sw_main<16>, dim3(grid), dim3(64), 0, s, P, B);
+        launch_width<16>(split, grid, P, B, s);
     else
-        hipLaunchKernelGGL(sw_main<32>, dim3(grid), dim3(64), 0, s, P, B);
+        launch_width<32>(split, grid, P, B, s);
     return hipGetLastError();
 }
 

@@ -70,6 +70,7 @@ struct tal_sw {
     Buf dims, keys, keys2, ids, order, ctl, slabs, cubtmp, d_pairs, d_seq;
     Buf pkeys, pkeys2, pids, perm; /* regrouping: per-pair keys and order */
     bool regroup = true; /* TAL_SW_REGROUP=0 keeps the reference's groups */
+    int split_force = 0; /* TAL_SW_SPLIT=1/2/4 fixes the unit split */
     int32_t *ctl_host = nullptr; /* pinned copy of the control words */
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     bool timed = false;
@@ -236,16 +237,24 @@ int run(tal_sw *h, int bits, tal_sw_pair *d_pairs, const uint8_t *d_seq, size_t 
     P.n = n;
     P.nunits = U;
     P.slot = bits == 8 ? 128 : 1024;
-    P.rows_cap = std::max(4, max_rows);
+    /* split units sweep 16-row strips, staged up to a multiple of 16 */
+    P.rows_cap = std::max(16, (max_rows + 15) & ~15);
     P.cols_cap = std::max(4, max_cols);
     P.aligned4 = ((uintptr_t)d_seq & 3) == 0;
     P.m = m;
     P.slab_words = (size_t)64 * (P.rows_cap + 3 * (size_t)P.cols_cap);
-    const int ntasks = (U + 3) / 4;
+    /* split each unit over K quarters of a wave when the batch has too few
+     * units to give every wave slot two tasks */
+    const int slots = h->cu_count * h->waves_per_cu;
+    int split = 1;
+    while (split < 4 && (int64_t)U * split < 8LL * slots) split *= 2;
+    if (h->split_force) split = h->split_force;
+    const int ntasks = (U + 4 / split - 1) / (4 / split);
     h->st.regrouped = regrouped ? 1 : 0;
     h->st.units = U;
+    h->st.split = split;
     if (ntasks == 0) return TAL_OK; /* every pair in a zero-bound group */
-    const int grid = std::max(1, std::min(ntasks, h->cu_count * h->waves_per_cu));
+    const int grid = std::max(1, std::min(ntasks, slots));
     SWCHK(h, h->slabs.ensure(sizeof(uint32_t) * P.slab_words * grid), "alloc slabs");
     SwBuffers B{};
     B.pairs = d_pairs;
@@ -256,7 +265,7 @@ int run(tal_sw *h, int bits, tal_sw_pair *d_pairs, const uint8_t *d_seq, size_t 
     B.ctl = ctl;
     B.slabs = h->slabs.as<uint32_t>();
     SWCHK(h, hipEventRecord(h->ev[1], s), "event");
-    SWCHK(h, launch_sw_main(bits, grid, P, B, s), "launch");
+    SWCHK(h, launch_sw_main(bits, split, grid, P, B, s), "launch");
     SWCHK(h, hipEventRecord(h->ev[2], s), "event");
     h->timed = true;
     h->st.max_rows = max_rows;
@@ -298,6 +307,10 @@ int tal_sw_create(int32_t w_match, int32_t w_mismatch, int32_t w_open, int32_t w
     h->w_extend = w_extend;
     h->device = device;
     if (const char *v = getenv("TAL_SW_REGROUP")) h->regroup = atoi(v) != 0;
+    if (const char *v = getenv("TAL_SW_SPLIT")) {
+        const int x = atoi(v);
+        if (x == 1 || x == 2 || x == 4) h->split_force = x;
+    }
     if (const char *v = getenv("TAL_SW_WAVES_PER_CU")) {
         const int x = atoi(v);
         if (x >= 1 && x <= 32) h->waves_per_cu = x;

@@ -32,7 +32,8 @@ EXPORTS = ["tal_sw_slot", "tal_sw_create", "tal_sw_destroy", "tal_sw_last_error"
 class Stats(ctypes.Structure):
     _fields_ = [("ms_kernel", ctypes.c_float), ("ms_total", ctypes.c_float),
                 ("units", ctypes.c_int32), ("max_rows", ctypes.c_int32),
-                ("max_cols", ctypes.c_int32), ("regrouped", ctypes.c_int32)]
+                ("max_cols", ctypes.c_int32), ("regrouped", ctypes.c_int32),
+                ("split", ctypes.c_int32)]
 
 
 class TalError(RuntimeError):
