@@ -53,7 +53,7 @@ typedef struct tal_sw_stats {
 int32_t tal_sw_slot(int bits);
 
 /* PairWiseSW::PairWiseSW(w_match, w_mismatch, w_open, w_extend),
- * PairWiseSW.cpp:40-56.  The weights are used as given (the reference adds
+ * PairWiseSW.cpp:43-57.  The weights are used as given (the reference adds
  * w_mismatch, w_open and w_extend, so penalties are negative numbers), each
  * truncated to the class width as _mm512_set1_epi8/16/32 does. */
 int tal_sw_create(int32_t w_match, int32_t w_mismatch, int32_t w_open, int32_t w_extend,

@@ -81,6 +81,41 @@ def test_cpp_shim_compiles(tmp_path):
                            str(src)])
 
 
+def test_pairwisesw_shim_compiles_reference_call_pattern(tmp_path):
+    # the calls bench-smith-waterman.cpp:203-223 makes, unchanged
+    src = tmp_path / "pwsw.cpp"
+    src.write_text("""
+#include "PairWiseSW_tal.hpp"
+int run(SeqPair *seqPairArray, uint8_t *seqBuf, int64_t numPairs, int32_t numThreads) {
+    CycleBreakup ticks;
+    PairWiseSW_32 *pwsw32 = new PairWiseSW_32(1, -1, 1, 1);
+    pwsw32->getScores(seqPairArray, seqBuf, numPairs, numThreads);
+    ticks = pwsw32->getTicks();
+    PairWiseSW_16 *pwsw16 = new PairWiseSW_16(1, -1, 1, 1);
+    pwsw16->getScores(seqPairArray, seqBuf, numPairs, numThreads);
+    PairWiseSW_8 *pwsw8 = new PairWiseSW_8(1, -1, 1, 1);
+    pwsw8->getScores(seqPairArray, seqBuf, numPairs, numThreads);
+    delete pwsw32; delete pwsw16; delete pwsw8;
+    return (int)(ticks.swTicks + MAX_SEQ_LEN_8 + MAX_SEQ_LEN_16 + MAX_SEQ_LEN_32);
+}
+static_assert(sizeof(SeqPair) == 12, "SeqPair of PairWiseSW.h:44-49");
+int main() { return 0; }
+""")
+    subprocess.check_call(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                           str(src)])
+
+
+def test_sw_arguments_without_gpu():
+    import tal_sw
+    L = tal_sw.load()
+    assert [L.tal_sw_slot(b) for b in (8, 16, 32, 12)] == [128, 1024, 1024, 0]
+    assert L.tal_sw_get_scores(None, 16, None, None, 0, 0) == tal_sw.TAL_EINVAL
+    assert L.tal_sw_last_error(None) == b"null handle"
+    cli = os.path.join(os.path.dirname(tal_sw.LIB_PATH), "tal-bench-sw")
+    r = subprocess.run([cli], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "pairFileName not specified" in r.stdout
+
+
 def test_create_rejects_bad_params():
     L = T.load()
     h = ctypes.c_void_p()

@@ -63,6 +63,15 @@ def build_synth(out=None):
     return out
 
 
+def build_sw_cli(out=None):
+    """tal-bench-sw: bench-smith-waterman's command line on the engine."""
+    out = out or os.path.join(HERE, "tal-bench-sw")
+    _run(["g++", "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}",
+          os.path.join(CSRC, "tal_bench_sw.cpp"), "-o", out, f"-L{HERE}", "-ltal_sw",
+          "-Wl,-rpath,$ORIGIN"])
+    return out
+
+
 def build_sw(out=None):
     """libtal_sw.so: the full inter-task SW engine (include/tal_sw.h)."""
     return build_hip(out or os.path.join(HERE, "libtal_sw.so"), sources=SW_SOURCES)
@@ -73,6 +82,7 @@ def build_all():
     build_hip()
     build_sw()
     build_cli()
+    build_sw_cli()
 
 
 if __name__ == "__main__":
