@@ -243,10 +243,13 @@ int run(tal_sw *h, int bits, tal_sw_pair *d_pairs, const uint8_t *d_seq, size_t 
     P.aligned4 = ((uintptr_t)d_seq & 3) == 0;
     P.m = m;
     P.slab_words = (size_t)64 * (P.rows_cap + 3 * (size_t)P.cols_cap);
-    /* split each unit over K quarters of a wave when the batch has too few
-     * units to give every wave slot two tasks */
+    /* Split each unit over K quarters of a wave.  A split pays K - 1 pipeline
+     * steps per round of K strips (a few percent once a row has more than 60
+     * tiles) and saves slab traffic and task length; measured on MI355X, K = 4
+     * wins from about 250 columns, K = 2 from about 64.  Batches with too few
+     * units to give every wave slot two tasks split further. */
     const int slots = h->cu_count * h->waves_per_cu;
-    int split = 1;
+    int split = max_cols >= 256 ? 4 : max_cols >= 64 ? 2 : 1;
     while (split < 4 && (int64_t)U * split < 8LL * slots) split *= 2;
     if (h->split_force) split = h->split_force;
     const int ntasks = (U + 4 / split - 1) / (4 / split);
