@@ -86,6 +86,9 @@ def main():
     ap.add_argument("--sw", default="", choices=["", *sorted(synth.SW_WORKLOADS)],
                     help="measure the full inter-task SW engine (PairWiseSW drop-in) on one of "
                          "its synthetic workloads instead")
+    ap.add_argument("--chain", action="store_true",
+                    help="measure the DP chaining engine (dp_chain::mm_dp_vectorized drop-in) "
+                         "on its synthetic anchor sets instead")
     ap.add_argument("--sw-weights", default="1,-1,1,1", metavar="M,X,O,E",
                     help="full-SW weights (match, mismatch, gap open, gap extend as the reference "
                          "adds them); default: bench-smith-waterman's")
@@ -105,6 +108,9 @@ def main():
         return
     if args.sw:
         _sw(args)
+        return
+    if args.chain:
+        _chain(args)
         return
 
     _batch(args)
@@ -224,6 +230,116 @@ def _sw_cpu(bits, pairs, buf, m, min_s=10.0):
     cells = float(tal_sw.cells(sub).sum()) * passes
     return {"value": round(cells / dt / 1e9, 3), "unit": "GCUPS", "cores": 16, "kind": kind,
             "sample": f"{passes} passes over the first {m} pairs, {dt:.1f}s"}
+
+
+# VALU lane-ops per predecessor examined: differences and |dd| 5, the five
+# window tests and their conjunction 9, overlap 2, ilog2 2, float gap 4, the
+# score 3, the (score, j) key and its max 5
+CHAIN_OPS = 30
+
+
+def _chain(args):
+    """DP chaining, dp_chain::mm_dp_vectorized over many anchor sets: one
+    step chains the whole synthetic batch from HBM; value = anchors / s."""
+    import torch
+    import tal_chain
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    nsets, lo, hi = synth.CHAIN_WORKLOADS["chain"]
+    t = time.time()
+    a, off = synth.chain_generate(nsets, lo, hi)
+    n = len(a)
+    log(f"[chain] {nsets} sets of U[{lo},{hi}] anchors ({n} anchors) in {time.time() - t:.1f}s")
+    dev = torch.device("cuda:0")
+    d_a = torch.from_numpy(a.view(np.uint8)).to(dev)
+    d_off = torch.from_numpy(off).to(dev)
+    d_f = torch.zeros(n, dtype=torch.int32, device=dev)
+    d_p = torch.zeros(n, dtype=torch.int32, device=dev)
+    d_v = torch.zeros(n, dtype=torch.int32, device=dev)
+    eng = tal_chain.DpChain()
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def step():
+        eng.dp_device(d_a.data_ptr(), d_off.data_ptr(), nsets, d_f.data_ptr(), d_p.data_ptr(),
+                      d_v.data_ptr(), stream=stream)
+
+    step()
+    torch.cuda.synchronize()
+    m = min(nsets, 200)
+    want = O.chain_dp_sets(a[:off[m]], off[:m + 1], nthreads=16)
+    got = (d_f.cpu().numpy().view(np.uint32)[:off[m]], d_p.cpu().numpy()[:off[m]],
+           d_v.cpu().numpy()[:off[m]])
+    bad = sum(int((x != y).sum()) for x, y in zip(got, want))
+    if bad:
+        raise SystemExit(f"[chain] parity FAILED: {bad} outputs differ in the first {m} sets")
+    log(f"[chain] parity: first {m} sets bit-exact against the oracle")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        kms.append(eng.stats()["ms_kernel"])
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    st = eng.stats()
+    k_ms = float(np.mean(kms))
+    pred = float(st["predecessors"])
+    achieved = CHAIN_OPS * pred / (k_ms / 1e3) / 1e12
+    line = {
+        "metric": "anchors/s of bit-exact DP chaining, dp_chain::mm_dp_vectorized drop-in",
+        "value": round(n / dt, 1), "unit": "anchors/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32 (+ f32/f64 gap cost)",
+        "data": "synthetic (seeded anchor sets)",
+        "config": {"workload": "chain", "sets": nsets, "anchors_per_set": f"U[{lo},{hi}]",
+                   "anchors": n, "bw": 500, "max_iter": 5000, "dr": 5000, "dq": 5000},
+        "predecessors_per_s": round(pred / dt, 1),
+        "parity": f"bit-exact on the first {m} sets",
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3),
+                     "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
+                     "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
+                     "kernel": "chain_main", "kernel_ms": round(k_ms, 3),
+                     "ops_per_predecessor": CHAIN_OPS, "lds_ring": st["lds_ring"]},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = _chain_cpu(a, off)
+    print(json.dumps(line), flush=True)
+
+
+def _chain_cpu(a, off, min_s=10.0):
+    """The reference's own mm_dp_vectorized (AVX-512 build in oracle/_ref) on
+    16 threads, sets dealt out to the threads, else the oracle on 16 threads;
+    whole passes over the batch until min_s seconds."""
+    import oracle as O
+    from concurrent.futures import ThreadPoolExecutor
+    nsets = len(off) - 1
+    kind = "reference"
+    if not O.ref_chain_available() or O.ref_chain_dp_sets(a[:off[1]], off[:2]) is None:
+        kind = "port"
+    chunks = [(k, min(nsets, k + 50)) for k in range(0, nsets, 50)]
+
+    def part(c):
+        s0, s1 = c
+        sub = a[off[s0]:off[s1]]
+        so = off[s0:s1 + 1] - off[s0]
+        if kind == "reference":
+            O.ref_chain_dp_sets(sub, so)
+        else:
+            O.chain_dp_sets(sub, so, nthreads=1)
+
+    passes = 0
+    t = time.perf_counter()
+    with ThreadPoolExecutor(16) as ex:
+        while True:
+            list(ex.map(part, chunks))
+            passes += 1
+            dt = time.perf_counter() - t
+            if dt >= min_s:
+                break
+    return {"value": round(len(a) * passes / dt, 1), "unit": "anchors/s", "cores": 16,
+            "kind": kind, "sample": f"{passes} passes over all {nsets} sets, {dt:.1f}s"}
 
 
 def _traffic():

@@ -247,3 +247,76 @@ def sw_make_batch(seq1s, seq2s, bits):
     pairs["len1"] = [len(a) for a in seq1s]
     pairs["len2"] = [len(b) for b in seq2s]
     return pairs, buf
+
+
+# ---------------------------------------------------------------------------
+# DP chaining (chain_oracle.c): dp_chain::mm_dp_vectorized.
+
+# anchor_t of parallel_chaining_32_bit.h:56-65 (16 bytes)
+ANCHOR_DTYPE = np.dtype([("r", "<u8"), ("q", "<i4"), ("l", "<i4")])
+REF_CHAIN_PATH = os.path.join(HERE, "_ref", "libchain_ref.so")
+# bench-dp-chaining's object and call (bench-dp-chaining.cpp:69,101)
+CHAIN_DEFAULTS = dict(bw=500, max_iter=5000, dr=5000, dq=5000)
+
+
+def _chain_lib():
+    L = lib()
+    if not getattr(L, "_chain_ready", False):
+        P, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+        L.chain_oracle_dp.argtypes = [P, i64, i32, i32, i32, i32, P, P, P]
+        L.chain_oracle_dp_sets.argtypes = [P, P, i32, i32, i32, i32, i32, P, P, P, ctypes.c_int]
+        L.chain_oracle_set_mutation.argtypes = [ctypes.c_int]
+        L._chain_ready = True
+    return L
+
+
+def chain_dp_sets(anchors, offsets, bw=500, max_iter=5000, dr=5000, dq=5000, nthreads=1):
+    """Oracle chaining of every set; returns (f uint32, p int32, v int32)."""
+    anchors = np.ascontiguousarray(anchors, ANCHOR_DTYPE)
+    offsets = np.ascontiguousarray(offsets, np.int64)
+    n = len(anchors)
+    f = np.zeros(max(n, 1), np.uint32)
+    p = np.zeros(max(n, 1), np.int32)
+    v = np.zeros(max(n, 1), np.int32)
+    _chain_lib().chain_oracle_dp_sets(_ptr(anchors), _ptr(offsets), len(offsets) - 1, bw,
+                                      max_iter, dr, dq, _ptr(f), _ptr(p), _ptr(v), nthreads)
+    return f[:n], p[:n], v[:n]
+
+
+def chain_set_mutation(m):
+    _chain_lib().chain_oracle_set_mutation(int(m))
+
+
+_ref_chain = None
+
+
+def ref_chain_available():
+    return os.path.exists(REF_CHAIN_PATH)
+
+
+def ref_chain_dp_sets(anchors, offsets, bw=500, max_iter=5000, dr=5000, dq=5000):
+    """The reference's own mm_dp_vectorized (oracle/_ref, built from its
+    header by `make -C oracle ref`), set by set.  None when the CPU lacks
+    AVX-512BW/CD."""
+    global _ref_chain
+    if _ref_chain is None:
+        _ref_chain = ctypes.CDLL(REF_CHAIN_PATH)
+        P, i32 = ctypes.c_void_p, ctypes.c_int32
+        _ref_chain.ref_chain_dp.restype = ctypes.c_int
+        _ref_chain.ref_chain_dp.argtypes = [P, P, P, ctypes.c_int64, i32, i32, i32, i32, P, P, P]
+    n = len(anchors)
+    r = np.ascontiguousarray(anchors["r"], np.uint64)
+    q = np.ascontiguousarray(anchors["q"], np.int32)
+    ln = np.ascontiguousarray(anchors["l"], np.int32)
+    f = np.zeros(max(n, 1), np.uint32)
+    p = np.zeros(max(n, 1), np.int32)
+    v = np.zeros(max(n, 1), np.int32)
+    for k in range(len(offsets) - 1):
+        a, b = int(offsets[k]), int(offsets[k + 1])
+        if b == a:
+            continue
+        rc = _ref_chain.ref_chain_dp(_ptr(r[a:b]), _ptr(q[a:b]), _ptr(ln[a:b]), b - a, bw,
+                                     max_iter, dr, dq, _ptr(f[a:b]), _ptr(p[a:b]), _ptr(v[a:b]))
+        if rc == -2:
+            return None
+    return f[:n], p[:n], v[:n]

@@ -116,6 +116,42 @@ def test_sw_arguments_without_gpu():
     assert r.returncode == 2 and "pairFileName not specified" in r.stdout
 
 
+def test_chain_header_and_shim():
+    import tal_chain
+    text = open(os.path.join(ROOT, "include", "tal_chain.h")).read()
+    declared = sorted(set(re.findall(r"\b(tal_chain_\w+)\s*\(", text)))
+    assert declared == sorted(tal_chain.EXPORTS)
+    lib = ctypes.CDLL(tal_chain.LIB_PATH)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert tal_chain.ANCHOR_DTYPE.itemsize == 16
+    L = tal_chain.load()
+    assert L.tal_chain_dp(None, None, None, 0, 0, 0, None, None, None) == tal_chain.TAL_EINVAL
+
+
+def test_chain_shim_compiles_reference_call_pattern(tmp_path):
+    # the calls bench-dp-chaining.cpp:69-101 makes, unchanged
+    src = tmp_path / "chain.cpp"
+    src.write_text("""
+#include <vector>
+#include "parallel_chaining_tal.hpp"
+int run(std::vector<anchor_t> &anc) {
+    dp_chain obj(5000,5000,500,25,5000,1.000000,0,1);
+    int32_t n = anc.size();
+    num_bits_t *f = (num_bits_t*)malloc(sizeof(num_bits_t)*(n+16)); f = f + 16;
+    int32_t *p = (int32_t*)malloc(sizeof(num_bits_t)*(n+16)); p = p + 16;
+    int32_t *v = (int32_t*)malloc(sizeof(num_bits_t)*(n+16)); v = v + 16;
+    num_bits_t *anchor_r, *anchor_q, *anchor_l;
+    create_SoA_Anchors_32_bit(&anc[0], anc.size(), anchor_r, anchor_q, anchor_l);
+    obj.mm_dp_vectorized(anc.size(), &anc[0], anchor_r, anchor_q, anchor_l, f, p, v, 5000, 5000, NULL, NULL);
+    return (int)f[0] + p[0] + v[0];
+}
+int main() { return 0; }
+""")
+    subprocess.check_call(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                           str(src)])
+
+
 def test_create_rejects_bad_params():
     L = T.load()
     h = ctypes.c_void_p()

@@ -1,0 +1,111 @@
+"""GPU parity of the DP chaining engine (libtal_chain.so) against the oracle
+(bit-exact f, p, v), through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle as O
+import synth as S
+import tal_chain as T
+from helpers import DeviceArrays
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _check(eng, a, off, label, **kw):
+    params = dict(bw=eng.params.bw, max_iter=eng.params.max_iter)
+    want = O.chain_dp_sets(a, off, nthreads=8, **params, **kw)
+    got = eng.dp(a, off, **kw)
+    for name, x, y in zip("fpv", got, want):
+        bad = np.where(x != y)[0]
+        assert len(bad) == 0, f"{label}: {name} differs at {len(bad)} anchors, first {bad[:5]}"
+
+
+@pytest.mark.parametrize("nsets,lo,hi", [(300, 1, 60), (200, 1, 3000), (20, 4000, 9000)])
+def test_synthetic_sets(nsets, lo, hi):
+    _say(f"[gpu] chain sets={nsets} size=[{lo},{hi}]")
+    a, off = S.chain_generate(nsets, lo, hi, seed=nsets + hi)
+    eng = T.DpChain()
+    _check(eng, a, off, "default")
+    st = eng.stats()
+    assert st["sets"] == nsets and st["predecessors"] > 0
+    eng.close()
+
+
+@pytest.mark.parametrize("kw,ctor", [(dict(dr=300, dq=200), dict(bw=20, max_iter=50)),
+                                     (dict(dr=5000, dq=5000), dict(bw=5000, max_iter=8)),
+                                     (dict(dr=100000, dq=100000), dict(bw=0, max_iter=5000)),
+                                     (dict(dr=50, dq=5000), dict(bw=500, max_iter=3)),
+                                     (dict(dr=5000, dq=5000), dict(bw=500, max_iter=100))])
+def test_parameters_and_ring_wrap(kw, ctor):
+    """Small max_iter makes the LDS ring shorter than the sets, so its slots
+    are reused (max_iter = 100: a 128-entry ring under 3000-anchor sets)."""
+    _say(f"[gpu] chain params {kw} {ctor}")
+    a, off = S.chain_generate(60, 1, 3000, seed=17)
+    eng = T.DpChain(**ctor)
+    _check(eng, a, off, f"{kw} {ctor}", **kw)
+    if ctor["max_iter"] == 100:
+        assert eng.stats()["lds_ring"] == 128
+    eng.close()
+
+
+def test_rounding_ties_and_edges():
+    _say("[gpu] chain rounding, ties, edges")
+    rng = np.random.default_rng(3)
+    eng = T.DpChain()
+    # the gap-rounding set: float and double gap costs differ at dd = 29, 58 ...
+    m = 300
+    a = np.zeros(m, T.ANCHOR_DTYPE)
+    a["r"] = 5000 + 40 * np.arange(m)
+    a["q"] = 40 * np.arange(m) + rng.choice([0, 29, 58, 116, 5], m)
+    a["l"] = 100
+    _check(eng, a, np.array([0, m]), "rounding")
+    # duplicates (ties), empty sets, single anchors, zero-length seeds
+    parts, offs = [], [0]
+    for size in (0, 1, 2, 0, 7, 6, 65, 64, 130, 1):
+        r = np.sort(rng.integers(0, 3000, size)).astype(np.uint64)
+        if size > 3:
+            r[1::3] = r[0::3][:len(r[1::3])]
+        part = np.zeros(size, T.ANCHOR_DTYPE)
+        part["r"] = r
+        part["q"] = (r.astype(np.int64) // 2 + rng.integers(0, 3, size)).astype(np.int32)
+        part["l"] = rng.choice([0, 15, 15, 20], size)
+        parts.append(part)
+        offs.append(offs[-1] + size)
+    _check(eng, np.concatenate(parts), np.array(offs), "edges")
+    # r beyond 32 bits (the window uses 64-bit r, the scores its low 32 bits)
+    a2, off2 = S.chain_generate(10, 100, 500, seed=8)
+    a2["r"] += np.uint64(1 << 33)
+    _check(eng, a2, off2, "64-bit r")
+    eng.close()
+
+
+def test_errors_and_device_path():
+    _say("[gpu] chain errors and device path")
+    a, off = S.chain_generate(50, 10, 800, seed=5)
+    eng = T.DpChain()
+    bad = a.copy()
+    bad["l"][7] = -1
+    with pytest.raises(T.TalError) as e:
+        eng.dp(bad, off)
+    assert e.value.code == T.TAL_ERANGE
+    boff = off.copy()
+    boff[3], boff[4] = boff[4], boff[3]
+    with pytest.raises(T.TalError) as e:
+        eng.dp(a, boff)
+    assert e.value.code == T.TAL_EINVAL
+    want = O.chain_dp_sets(a, off, nthreads=8)
+    n = len(a)
+    d = DeviceArrays(a, off, np.zeros(n, np.uint32), np.zeros(n, np.int32), np.zeros(n, np.int32))
+    try:
+        eng.dp_device(d.ptr(0), d.ptr(1), len(off) - 1, d.ptr(2), d.ptr(3), d.ptr(4))
+        got = [d.download(2, np.zeros(n, np.uint32)), d.download(3, np.zeros(n, np.int32)),
+               d.download(4, np.zeros(n, np.int32))]
+    finally:
+        d.close()
+    for x, y in zip(got, want):
+        assert np.array_equal(x, y)
+    eng.close()

@@ -19,7 +19,12 @@ SW_SOURCES = ["sw_kernels.hip", "tal_sw.hip"]
 # chains per step; the ILP scheduler interleaves them, where the default one
 # keeps each chain together and pays a wait state between dependent packed
 # instructions
-SOURCE_FLAGS = {"sw_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"]}
+SOURCE_FLAGS = {"sw_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"],
+                # the chaining gap cost divides and multiplies as IEEE float and
+                # double, as the reference's source is written
+                "chain_kernels.hip": ["-fhip-fp32-correctly-rounded-divide-sqrt",
+                                      "-ffp-contract=off"]}
+CHAIN_SOURCES = ["chain_kernels.hip", "tal_chain.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -77,10 +82,16 @@ def build_sw(out=None):
     return build_hip(out or os.path.join(HERE, "libtal_sw.so"), sources=SW_SOURCES)
 
 
+def build_chain(out=None):
+    """libtal_chain.so: the DP chaining engine (include/tal_chain.h)."""
+    return build_hip(out or os.path.join(HERE, "libtal_chain.so"), sources=CHAIN_SOURCES)
+
+
 def build_all():
     build_synth()
     build_hip()
     build_sw()
+    build_chain()
     build_cli()
     build_sw_cli()
 

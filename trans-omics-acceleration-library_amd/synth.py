@@ -112,3 +112,50 @@ def sw_generate(bits, n, lo, hi, seed=7, sub=0.10):
     pairs["len1"] = len1
     pairs["len2"] = len2
     return pairs, buf.reshape(-1)
+
+
+# ---------------------------------------------------------------------------
+# DP chaining inputs (bench-dp-chaining's anchor sets): per set, anchors of a
+# few true chains (near one diagonal, r and q advancing together with small
+# drift) and random anchors, sorted by r; l is the seed length.
+
+ANCHOR_DTYPE = np.dtype([("r", "<u8"), ("q", "<i4"), ("l", "<i4")])
+
+# name -> (sets, min anchors, max anchors)
+CHAIN_WORKLOADS = {
+    "chain": (4000, 200, 4000),
+}
+
+
+def chain_generate(nsets, lo, hi, seed=11, noise=0.3, kmer=(15, 28)):
+    """(anchors, offsets): nsets sets of U[lo, hi] anchors each."""
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(lo, hi + 1, nsets)
+    offsets = np.zeros(nsets + 1, np.int64)
+    offsets[1:] = np.cumsum(sizes)
+    n = int(offsets[-1])
+    anchors = np.zeros(n, ANCHOR_DTYPE)
+    for k in range(nsets):
+        m = int(sizes[k])
+        base = int(rng.integers(0, 3_000_000_000))
+        nchains = int(rng.integers(1, 4))
+        is_noise = rng.random(m) < noise
+        chain_id = rng.integers(0, nchains, m)
+        starts_r = base + rng.integers(0, 200_000, nchains)
+        starts_q = rng.integers(0, 5_000, nchains)
+        step = rng.integers(1, 40, m)
+        pos = np.zeros(m, np.int64)
+        for c in range(nchains):
+            sel = (chain_id == c) & ~is_noise
+            pos[sel] = np.cumsum(step[sel])
+        drift = rng.integers(-4, 5, m)
+        r = starts_r[chain_id] + pos
+        q = starts_q[chain_id] + pos + drift
+        r[is_noise] = base + rng.integers(0, 220_000, int(is_noise.sum()))
+        q[is_noise] = rng.integers(0, 60_000, int(is_noise.sum()))
+        order = np.argsort(r, kind="stable")
+        a, b = int(offsets[k]), int(offsets[k + 1])
+        anchors["r"][a:b] = r[order]
+        anchors["q"][a:b] = np.maximum(q[order], 0)
+        anchors["l"][a:b] = rng.integers(kmer[0], kmer[1] + 1, m)
+    return anchors, offsets
