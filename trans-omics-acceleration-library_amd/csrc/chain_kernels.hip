@@ -94,6 +94,12 @@ __global__ void __launch_bounds__(64) chain_main(ChainParams P, ChainBuffers B)
     int32_t *fr = lds, *vr = lds + P.ring;
     const int lane = threadIdx.x;
     const int mask = P.ring - 1;
+    /* The task loop keeps every lane on one path: no continue, and the one
+     * lane-0-only update (the predecessor count) after the loop.  A
+     * lane-divergent branch at the end of the body let the compiler's loop
+     * structure send lanes 1..63 back to the loop head without lane 0, where
+     * readfirstlane then read their task word, not lane 0's. */
+    uint64_t examined = 0;
     for (;;) {
         int task = 0;
         if (lane == 0) task = atomicAdd(&B.ctl[CHAIN_CTL_COUNTER], 1);
@@ -101,8 +107,7 @@ __global__ void __launch_bounds__(64) chain_main(ChainParams P, ChainBuffers B)
         if (task >= P.nsets) break;
         const int set = B.order[task];
         const int64_t a0 = B.off[set];
-        const int32_t n = (int32_t)(B.off[set + 1] - a0);
-        if (n <= 0) continue;
+        const int32_t n = max((int32_t)(B.off[set + 1] - a0), 0);
         const tal_chain_anchor *A = B.a + a0;
         /* avg_qspan = (float)sum(l) / n (:220-222) */
         uint64_t sum = 0;
@@ -112,7 +117,6 @@ __global__ void __launch_bounds__(64) chain_main(ChainParams P, ChainBuffers B)
         const float avg_val = (float)(0.01 * (double)avg_qspan);
         const double avg_d = (double)avg_qspan;
         int32_t st = 0;
-        uint64_t examined = 0;
         uint32_t of = 0;
         int32_t op = 0, ov = 0;
         for (int32_t i = 0; i < n; i++) {
@@ -196,8 +200,8 @@ __global__ void __launch_bounds__(64) chain_main(ChainParams P, ChainBuffers B)
                 }
             }
         }
-        if (lane == 0) atomicAdd(B.examined, (unsigned long long)examined);
     }
+    if (lane == 0) atomicAdd(B.examined, (unsigned long long)examined);
 }
 
 hipError_t launch_chain_check(const tal_chain_anchor *a, const int64_t *off, int32_t nsets,
