@@ -48,8 +48,9 @@ typedef struct tal_chain_stats {
     float ms_kernel;         /* the DP kernel of the last call */
     float ms_total;          /* the whole last call, device work only */
     int32_t sets;            /* anchor sets of the last call */
-    int32_t lds_ring;        /* f/v ring entries per wave in LDS (0: global memory) */
+    int32_t lds_ring;        /* anchors in the fast path's LDS ring per wave */
     int64_t predecessors;    /* candidate predecessors examined (sum of i - st) */
+    int32_t overflow_sets;   /* sets redone by the overflow pass (window > ring) */
 } tal_chain_stats;
 
 /* dp_chain::dp_chain(...) */

@@ -41,14 +41,28 @@ def test_synthetic_sets(nsets, lo, hi):
                                      (dict(dr=50, dq=5000), dict(bw=500, max_iter=3)),
                                      (dict(dr=5000, dq=5000), dict(bw=500, max_iter=100))])
 def test_parameters_and_ring_wrap(kw, ctor):
-    """Small max_iter makes the LDS ring shorter than the sets, so its slots
-    are reused (max_iter = 100: a 128-entry ring under 3000-anchor sets)."""
+    """Small max_iter keeps the predecessor windows inside the fast path's
+    ring, whose slots are reused under 3000-anchor sets; large windows go to
+    the overflow pass."""
     _say(f"[gpu] chain params {kw} {ctor}")
     a, off = S.chain_generate(60, 1, 3000, seed=17)
     eng = T.DpChain(**ctor)
     _check(eng, a, off, f"{kw} {ctor}", **kw)
-    if ctor["max_iter"] == 100:
-        assert eng.stats()["lds_ring"] == 128
+    eng.close()
+
+
+@pytest.mark.parametrize("ring", [64, 512, 4096])
+def test_ring_sizes_and_overflow_pass(ring, monkeypatch):
+    """A 64-anchor ring sends most sets to the overflow pass, 4096 none."""
+    _say(f"[gpu] chain ring={ring}")
+    monkeypatch.setenv("TAL_CHAIN_RING", str(ring))
+    a, off = S.chain_generate(80, 1, 2500, seed=23)
+    eng = T.DpChain()
+    _check(eng, a, off, f"ring={ring}")
+    st = eng.stats()
+    assert st["lds_ring"] == ring
+    if ring == 64:
+        assert st["overflow_sets"] > 0
     eng.close()
 
 

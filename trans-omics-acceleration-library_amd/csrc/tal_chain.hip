@@ -7,8 +7,9 @@
  * A call: one kernel validates the sets and finds the largest (one wait on
  * the stream to read that and the error word), the sets are sorted longest
  * first, and the persistent DP kernel runs one set per wave with an LDS ring
- * of f and v sized max(largest set, max_iter + 1) rounded up to a power of
- * two (chain_kernels.hip).
+ * of the latest anchors; sets whose predecessor window outgrows the ring are
+ * redone in the same stream by an overflow pass whose ring holds max(largest
+ * set, max_iter + 1) anchors (chain_kernels.hip).
  */
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -16,6 +17,7 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <new>
 #include <string>
 
@@ -71,7 +73,11 @@ struct tal_chain {
     hipStream_t stream = nullptr;
     int cu_count = 0;
     size_t lds_per_block = 0, lds_per_cu = 0;
-    Buf keys, keys2, ids, order, sum, cubtmp, d_a, d_off, d_f, d_p, d_v;
+    Buf keys, keys2, ids, order, over, sum, cubtmp, d_a, d_off, d_f, d_p, d_v;
+    /* anchors in the fast path's LDS ring (TAL_CHAIN_RING): 20 bytes each, so
+     * 1024 holds eight waves per CU; measured on MI355X, smaller rings send
+     * more sets to the slow overflow pass than their occupancy wins back */
+    int small_ring = 1024;
     Summary *host = nullptr; /* pinned */
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     bool timed = false;
@@ -122,14 +128,18 @@ int run(tal_chain *h, const tal_chain_anchor *a, const int64_t *off, int32_t nse
     if (h->host->ctl[CHAIN_CTL_ERR] == CHAIN_ERR_OFFSETS)
         return fail(h, TAL_EINVAL, "set offsets must be non-negative and non-decreasing");
     const uint64_t maxn = h->host->maxn;
-    /* the ring holds every predecessor an anchor can reach */
+    /* Two passes.  The ring pass keeps the latest small_ring anchors in LDS
+     * and hands every set whose predecessor window outgrows it to the
+     * overflow pass, whose f/v ring holds every predecessor an anchor can
+     * reach: max_iter + 1 or the largest set. */
     const uint64_t need = std::min<uint64_t>(maxn, (uint64_t)h->prm.max_iter + 1);
-    uint64_t ring = 64;
-    while (ring < need) ring <<= 1;
-    const size_t lds = 2 * sizeof(int32_t) * ring;
-    if (lds > h->lds_per_block)
+    uint64_t big = 64;
+    while (big < need) big <<= 1;
+    const bool overflow_possible = need + 1 > (uint64_t)h->small_ring;
+    if (overflow_possible && chain_lds_bytes(false, (int)big) > h->lds_per_block)
         return fail(h, TAL_ERANGE, "max_iter %d needs %zu bytes of LDS per wave (at most %zu)",
-                    h->prm.max_iter, lds, h->lds_per_block);
+                    h->prm.max_iter, chain_lds_bytes(false, (int)big), h->lds_per_block);
+    CHCHK(h, h->over.ensure(sizeof(int32_t) * nsets), "alloc overflow list");
 
     size_t tmp = 0;
     CHCHK(h, hipcub::DeviceRadixSort::SortPairsDescending(
@@ -147,7 +157,7 @@ int run(tal_chain *h, const tal_chain_anchor *a, const int64_t *off, int32_t nse
     P.max_iter = h->prm.max_iter;
     P.dr = dr;
     P.dq = dq;
-    P.ring = (int32_t)ring;
+    P.ring = h->small_ring;
     ChainBuffers B{};
     B.a = a;
     B.off = off;
@@ -155,17 +165,28 @@ int run(tal_chain *h, const tal_chain_anchor *a, const int64_t *off, int32_t nse
     B.p = p;
     B.v = v;
     B.order = h->order.as<int32_t>();
+    B.over = h->over.as<int32_t>();
     B.ctl = ds->ctl;
     B.examined = &ds->examined;
     /* one-wave blocks: as many as the LDS of the CUs holds, at most 16 per CU */
-    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(16, h->lds_per_cu / lds));
-    const int grid = std::max(1, std::min(nsets, h->cu_count * per_cu));
+    auto grid_for = [&](size_t lds, int tasks) {
+        const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(16, h->lds_per_cu / lds));
+        return std::max(1, std::min(tasks, h->cu_count * per_cu));
+    };
     CHCHK(h, hipEventRecord(h->ev[1], s), "event");
-    CHCHK(h, launch_chain_main(grid, P, B, s), "launch");
+    CHCHK(h, launch_chain_main(true, grid_for(chain_lds_bytes(true, P.ring), nsets), P, B, s),
+          "launch");
+    if (overflow_possible) {
+        /* the overflow pass takes its task count from the device */
+        P.ring = (int32_t)big;
+        CHCHK(h, launch_chain_main(false, grid_for(chain_lds_bytes(false, P.ring), nsets), P, B,
+                                   s),
+              "launch overflow pass");
+    }
     CHCHK(h, hipEventRecord(h->ev[2], s), "event");
     h->timed = true;
     h->st.sets = nsets;
-    h->st.lds_ring = (int32_t)ring;
+    h->st.lds_ring = h->small_ring;
     return TAL_OK;
 }
 
@@ -192,6 +213,10 @@ int tal_chain_create(const tal_chain_params *params, int device, tal_chain **out
     if (!h) return TAL_ENOMEM;
     h->prm = *params;
     h->device = device;
+    if (const char *e = getenv("TAL_CHAIN_RING")) {
+        const int x = atoi(e);
+        if (x >= 64 && x <= 4096 && (x & (x - 1)) == 0) h->small_ring = x;
+    }
     int rc = TAL_OK;
     do {
         if (hipSetDevice(device) != hipSuccess) { rc = TAL_EDEVICE; break; }
@@ -227,8 +252,8 @@ void tal_chain_destroy(tal_chain *h)
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    for (Buf *b : {&h->keys, &h->keys2, &h->ids, &h->order, &h->sum, &h->cubtmp, &h->d_a, &h->d_off,
-                   &h->d_f, &h->d_p, &h->d_v})
+    for (Buf *b : {&h->keys, &h->keys2, &h->ids, &h->order, &h->over, &h->sum, &h->cubtmp, &h->d_a,
+                   &h->d_off, &h->d_f, &h->d_p, &h->d_v})
         b->release();
     if (h->host) (void)hipHostFree(h->host);
     for (auto e : h->ev)
@@ -305,10 +330,10 @@ int tal_chain_get_stats(const tal_chain *hc, tal_chain_stats *out)
         CHCHK(h, hipEventSynchronize(h->ev[2]), "event sync");
         CHCHK(h, hipEventElapsedTime(&h->st.ms_kernel, h->ev[1], h->ev[2]), "event time");
         CHCHK(h, hipEventElapsedTime(&h->st.ms_total, h->ev[0], h->ev[2]), "event time");
-        unsigned long long ex = 0;
-        CHCHK(h, hipMemcpy(&ex, &h->sum.as<Summary>()->examined, sizeof ex, hipMemcpyDeviceToHost),
-              "read count");
-        h->st.predecessors = (int64_t)ex;
+        Summary sm;
+        CHCHK(h, hipMemcpy(&sm, h->sum.p, sizeof sm, hipMemcpyDeviceToHost), "read counts");
+        h->st.predecessors = (int64_t)sm.examined;
+        h->st.overflow_sets = sm.ctl[CHAIN_CTL_NOVER];
     }
     *out = h->st;
     return TAL_OK;
