@@ -302,7 +302,7 @@ def _chain(args):
                      "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
                      "kernel": "chain_main", "kernel_ms": round(k_ms, 3),
                      "ops_per_predecessor": CHAIN_OPS, "lds_ring": st["lds_ring"],
-                     "overflow_sets": st["overflow_sets"]},
+                     "far_anchors": st["far_anchors"]},
     }
     if not args.no_cpu:
         line["cpu_baseline"] = _chain_cpu(a, off)

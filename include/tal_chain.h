@@ -48,9 +48,10 @@ typedef struct tal_chain_stats {
     float ms_kernel;         /* the DP kernel of the last call */
     float ms_total;          /* the whole last call, device work only */
     int32_t sets;            /* anchor sets of the last call */
-    int32_t lds_ring;        /* anchors in the fast path's LDS ring per wave */
+    int32_t lds_ring;        /* anchors in the LDS ring per wave */
     int64_t predecessors;    /* candidate predecessors examined (sum of i - st) */
-    int32_t overflow_sets;   /* sets redone by the overflow pass (window > ring) */
+    int64_t far_anchors;     /* anchors whose window reaches past the ring
+                                (the older predecessors are read from HBM) */
 } tal_chain_stats;
 
 /* dp_chain::dp_chain(...) */

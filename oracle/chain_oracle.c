@@ -19,6 +19,8 @@
  *   otherwise: its 16-lane vector loop, dd converted to float times
  *     (float)(0.01 * avg_qspan), rounded toward -inf (:181-199), plus
  *     (31 - lzcnt(dd)) >> 1 for dd != 0.
+ * Both conversions are x86's: a value out of int32 range converts to
+ * INT32_MIN (cvttsd2si in the scalar loop, vcvtps2dq in the vector one).
  * avg_qspan = (float)(sum of l) / n in float (:222).  Candidates the vector
  * loop masks out score 0 there and so never beat l_i >= 0; lengths above
  * INT32_MAX (where l_i would read as negative) are outside this restatement.
@@ -28,11 +30,25 @@
 #include "chain_oracle.h"
 
 #include <math.h>
+#include <stdint.h>
 #include <stdlib.h>
 
 static int g_mutation = 0;
 
 void chain_oracle_set_mutation(int m) { g_mutation = m; }
+
+/* x86's float/double to int32 conversions: INT32_MIN out of range */
+static int32_t x86_cvt_f32(float x)
+{
+    if (g_mutation == 4 && x >= 2147483648.0f) return INT32_MAX; /* planted: saturate */
+    return x >= -2147483648.0f && x < 2147483648.0f ? (int32_t)x : INT32_MIN;
+}
+
+static int32_t x86_cvt_f64(double x)
+{
+    if (g_mutation == 4 && x >= 2147483648.0) return INT32_MAX;
+    return x > -2147483649.0 && x < 2147483648.0 ? (int32_t)x : INT32_MIN;
+}
 
 static int32_t ilog2_u32(uint32_t v)
 {
@@ -75,9 +91,10 @@ void chain_oracle_dp(const chain_oracle_anchor *a, int64_t n, int32_t bw, int32_
             const int32_t log_dd = dd ? ilog2_u32((uint32_t)dd) : 0;
             if (vector_path && g_mutation != 2) {
                 const float prod = (float)dd * avg_val;
-                gap = (int32_t)floorf(prod) + (log_dd >> 1);
+                gap = (int32_t)((uint32_t)x86_cvt_f32(floorf(prod)) + (uint32_t)(log_dd >> 1));
             } else {
-                gap = (int32_t)((double)dd * .01 * (double)avg_qspan) + (log_dd >> 1);
+                gap = (int32_t)((uint32_t)x86_cvt_f64((double)dd * .01 * (double)avg_qspan) +
+                                (uint32_t)(log_dd >> 1));
             }
             const int32_t sc = (int32_t)((uint32_t)f[j] + (uint32_t)oc - (uint32_t)gap);
             if (sc > max_f || (g_mutation == 3 && sc == max_f)) {

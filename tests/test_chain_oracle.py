@@ -36,7 +36,9 @@ def test_oracle_equals_reference_build(seed, nsets, lo, hi):
 @pytest.mark.parametrize("kw", [dict(bw=20, max_iter=50, dr=300, dq=200),
                                 dict(bw=5000, max_iter=8, dr=5000, dq=5000),
                                 dict(bw=0, max_iter=5000, dr=100000, dq=100000),
-                                dict(bw=500, max_iter=3, dr=50, dq=5000)])
+                                dict(bw=500, max_iter=3, dr=50, dq=5000),
+                                dict(bw=500, max_iter=700, dr=-1, dq=5000),
+                                dict(bw=500, max_iter=5000, dr=5000, dq=-3)])
 def test_oracle_equals_reference_other_parameters(kw):
     a, off = S.chain_generate(30, 1, 2500, seed=7)
     assert _same(O.chain_dp_sets(a, off, nthreads=8, **kw), _ref_or_skip(a, off, **kw))
@@ -79,6 +81,47 @@ def _rounding_set(m=200, seed=4):
     return a, np.array([0, m], np.int64)
 
 
+def _overflow_set(m=2000, seed=6):
+    """Half the seeds have lengths near 2^31 (and q far away, so that they
+    chain with nothing), so avg_qspan is about 1e9 and dd * 0.01 * avg_qspan
+    passes 2^31 from dd of about 200: those gap costs leave int32 range, where
+    the reference's x86 conversions give INT32_MIN (the GPU's own conversion
+    saturates, so the engine handles this case by hand).  Such a cost only
+    shows in a score when f[j] + oc is negative, oc = ddr < 0, where the score
+    wraps to a large value; so some neighbouring r of the short seeds are
+    swapped (r out of order is within what the reference accepts).  Short r
+    steps give both the scalar (<= 6 candidates) and the vector path."""
+    rng = np.random.default_rng(seed)
+    a = np.zeros(m, O.ANCHOR_DTYPE)
+    steps = rng.choice([1, 2, 30, 400, 4000], m)
+    r = 10_000 + np.cumsum(steps)
+    q = (r - 10_000 + rng.integers(-400, 401, m)).clip(0)
+    for k in rng.choice(m - 1, m // 5, replace=False):
+        r[k], r[k + 1] = r[k + 1], r[k]
+    big = rng.random(m) < 0.5
+    ln = rng.integers(15, 29, m)
+    ln[big] = (1 << 31) - 1 - rng.integers(0, 1000, int(big.sum()))
+    q[big] = 50_000_000 + rng.integers(0, 1000, int(big.sum()))
+    a["r"], a["q"], a["l"] = r, q, ln
+    return a, np.array([0, m], np.int64)
+
+
+@ref
+def test_oracle_equals_reference_gap_overflow():
+    a, off = _overflow_set()
+    assert _same(O.chain_dp_sets(a, off), _ref_or_skip(a, off))
+
+
+def test_overflow_set_reaches_the_limit():
+    # some candidate of the overflow set has a gap cost at or past 2^31 on
+    # each path, so the set pins the out-of-range conversions
+    a, _ = _overflow_set()
+    avg = np.float32(np.float32(a["l"].astype(np.uint64).sum()) / np.float32(len(a)))
+    dd = np.arange(0, 501)
+    f32 = dd.astype(np.float32) * np.float32(0.01 * np.float64(avg))
+    assert (f32 >= 2.0 ** 31).any() and (f32 < 2.0 ** 31).any()
+
+
 @ref
 def test_oracle_equals_reference_gap_rounding():
     a, off = _rounding_set()
@@ -113,7 +156,7 @@ def test_closed_forms():
     assert list(v) == list(f)
 
 
-@pytest.mark.parametrize("mutation", [1, 2, 3])
+@pytest.mark.parametrize("mutation", [1, 2, 3, 4])
 def test_planted_mistakes_are_caught(mutation):
     a, off = S.chain_generate(20, 1, 1500, seed=9)
     caught = []
@@ -133,10 +176,10 @@ def test_planted_mistakes_are_caught(mutation):
         caught.append(list(f) != [15 + 10 * i for i in range(m)] or
                       list(p) != [-1] + list(range(m - 1)))
         if O.ref_chain_available():
-            ra, roff = _rounding_set()
-            want = O.ref_chain_dp_sets(ra, roff)
-            if want is not None:
-                caught.append(not _same(O.chain_dp_sets(ra, roff), want))
+            for ra, roff in (_rounding_set(), _overflow_set()):
+                want = O.ref_chain_dp_sets(ra, roff)
+                if want is not None:
+                    caught.append(not _same(O.chain_dp_sets(ra, roff), want))
     finally:
         O.chain_set_mutation(0)
     assert any(caught)

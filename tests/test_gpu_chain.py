@@ -39,11 +39,13 @@ def test_synthetic_sets(nsets, lo, hi):
                                      (dict(dr=5000, dq=5000), dict(bw=5000, max_iter=8)),
                                      (dict(dr=100000, dq=100000), dict(bw=0, max_iter=5000)),
                                      (dict(dr=50, dq=5000), dict(bw=500, max_iter=3)),
-                                     (dict(dr=5000, dq=5000), dict(bw=500, max_iter=100))])
+                                     (dict(dr=5000, dq=5000), dict(bw=500, max_iter=100)),
+                                     (dict(dr=-1, dq=5000), dict(bw=500, max_iter=700)),
+                                     (dict(dr=5000, dq=-3), dict(bw=500, max_iter=5000))])
 def test_parameters_and_ring_wrap(kw, ctor):
-    """Small max_iter keeps the predecessor windows inside the fast path's
-    ring, whose slots are reused under 3000-anchor sets; large windows go to
-    the overflow pass."""
+    """Small max_iter keeps the predecessor windows inside the ring, whose
+    slots are reused under 3000-anchor sets; dr = -1 (every r within reach)
+    makes windows of max_iter."""
     _say(f"[gpu] chain params {kw} {ctor}")
     a, off = S.chain_generate(60, 1, 3000, seed=17)
     eng = T.DpChain(**ctor)
@@ -51,18 +53,27 @@ def test_parameters_and_ring_wrap(kw, ctor):
     eng.close()
 
 
-@pytest.mark.parametrize("ring", [64, 512, 4096])
-def test_ring_sizes_and_overflow_pass(ring, monkeypatch):
-    """A 64-anchor ring sends most sets to the overflow pass, 4096 none."""
-    _say(f"[gpu] chain ring={ring}")
+@pytest.mark.parametrize("ring,wpb,unroll,table", [(128, 1, 1, 1), (128, 4, 4, 0), (512, 2, 2, 1),
+                                                   (4096, 1, 4, 1), (1024, 4, 1, 0),
+                                                   (1024, 1, 2, 0), (256, 2, 4, 1)])
+def test_ring_sizes_and_far_predecessors(ring, wpb, unroll, table, monkeypatch):
+    """A 128-slot ring (64 in reach) sends most windows partly to HBM; 4096
+    none.  Blocks of 1, 2 and 4 waves; 1, 2 and 4 older candidates at a time;
+    gap costs from the table or computed."""
+    _say(f"[gpu] chain ring={ring} waves/block={wpb} unroll={unroll} table={table}")
     monkeypatch.setenv("TAL_CHAIN_RING", str(ring))
+    monkeypatch.setenv("TAL_CHAIN_WPB", str(wpb))
+    monkeypatch.setenv("TAL_CHAIN_UNROLL", str(unroll))
+    monkeypatch.setenv("TAL_CHAIN_TABLE", str(table))
     a, off = S.chain_generate(80, 1, 2500, seed=23)
     eng = T.DpChain()
-    _check(eng, a, off, f"ring={ring}")
+    _check(eng, a, off, f"ring={ring} wpb={wpb} unroll={unroll} table={table}")
     st = eng.stats()
     assert st["lds_ring"] == ring
-    if ring == 64:
-        assert st["overflow_sets"] > 0
+    if ring == 128:
+        assert st["far_anchors"] > 0
+    if ring == 4096:
+        assert st["far_anchors"] == 0
     eng.close()
 
 
@@ -94,6 +105,49 @@ def test_rounding_ties_and_edges():
     a2, off2 = S.chain_generate(10, 100, 500, seed=8)
     a2["r"] += np.uint64(1 << 33)
     _check(eng, a2, off2, "64-bit r")
+    # sets whose r span 2^31 or more, across the 2^32 boundary (the window
+    # test needs all 64 bits)
+    a3, off3 = S.chain_generate(12, 200, 900, seed=9)
+    for k in range(12):
+        b, e = off3[k], off3[k + 1]
+        h = (b + e) // 2
+        a3["r"][b:h] += np.uint64((1 << 32) - 3000)
+        a3["r"][h:e] += np.uint64((1 << 32) + (1 << 31))
+    _check(eng, a3, off3, "wide sets")
+    eng.close()
+
+
+@pytest.mark.parametrize("swaps", [0.02, 0.3])
+def test_unsorted_sets(swaps):
+    """r out of order within a set (the reference accepts it; its st scan
+    then stops at the first r within reach, not at a sorted bound): the
+    in-order st search instead of the binary search."""
+    _say(f"[gpu] chain unsorted swaps={swaps}")
+    rng = np.random.default_rng(31)
+    a, off = S.chain_generate(120, 1, 2500, seed=29)
+    for k in range(0, 120, 2):
+        b, e = int(off[k]), int(off[k + 1])
+        if e - b < 2:
+            continue
+        idx = rng.choice(e - b - 1, max(1, int((e - b) * swaps)), replace=False) + b
+        r = a["r"].copy()
+        r[idx], r[idx + 1] = a["r"][idx + 1], a["r"][idx]
+        a["r"] = r
+    eng = T.DpChain()
+    _check(eng, a, off, f"unsorted {swaps}")
+    eng.close()
+
+
+def test_gap_costs_out_of_int32_range():
+    """Gap costs at or past 2^31 convert to INT32_MIN as the reference's x86
+    conversions do (the set of tests/test_chain_oracle.py, pinned against the
+    reference build there)."""
+    _say("[gpu] chain gap overflow")
+    from test_chain_oracle import _overflow_set
+    eng = T.DpChain()
+    for seed in (6, 7):
+        a, off = _overflow_set(2000, seed)
+        _check(eng, a, off, f"overflow seed={seed}")
     eng.close()
 
 

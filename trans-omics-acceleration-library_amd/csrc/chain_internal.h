@@ -15,9 +15,7 @@ namespace talchain {
 
 enum {
     CHAIN_CTL_ERR = 0,
-    CHAIN_CTL_COUNTER = 1,  /* task counter of the ring pass */
-    CHAIN_CTL_NOVER = 2,    /* sets on the overflow list */
-    CHAIN_CTL_COUNTER2 = 3, /* task counter of the overflow pass */
+    CHAIN_CTL_COUNTER = 1, /* task counter of the DP kernel */
     CHAIN_CTL_WORDS = 4
 };
 enum { CHAIN_ERR_OFFSETS = 1, CHAIN_ERR_RANGE = 2 };
@@ -25,8 +23,15 @@ enum { CHAIN_ERR_OFFSETS = 1, CHAIN_ERR_RANGE = 2 };
 struct ChainParams {
     int32_t nsets;
     int32_t bw, max_iter, dr, dq;
-    int32_t ring; /* ring entries per wave in LDS (power of two) */
+    int32_t ring;       /* LDS ring slots per wave (power of two, >= 128) */
+    int32_t table;      /* gap-cost table entries per wave (0: none; else >= bw + 2) */
 };
+
+/* LDS bytes of one wave: the ring (12 bytes a slot) and the gap table */
+__host__ __device__ inline size_t chain_wave_lds(int ring, int table)
+{
+    return 12 * (size_t)ring + 4 * (size_t)table;
+}
 
 struct ChainBuffers {
     const tal_chain_anchor *a;
@@ -34,18 +39,21 @@ struct ChainBuffers {
     uint32_t *f;
     int32_t *p, *v;
     const int32_t *order; /* sets, longest first */
-    int32_t *over;        /* the overflow list (nsets entries of room) */
     int32_t *ctl;
-    unsigned long long *examined; /* sum over anchors of i - st */
+    unsigned long long *examined;    /* sum over anchors of i - st */
+    unsigned long long *far_anchors; /* anchors with predecessors older than the ring */
 };
 
 hipError_t launch_chain_check(const tal_chain_anchor *a, const int64_t *off, int32_t nsets,
                               uint32_t *keys, int32_t *ids, int32_t *ctl,
                               unsigned long long *maxn, hipStream_t s);
 hipError_t set_chain_lds_limit(size_t bytes);
-size_t chain_lds_bytes(bool small, int ring);
-hipError_t launch_chain_main(bool small, int grid, const ChainParams &P, const ChainBuffers &B,
-                             hipStream_t s);
+/* LDS bytes of one block of the DP kernel */
+size_t chain_lds_bytes(int ring, int table, int waves_per_block);
+/* waves_per_block: 1, 2 or 4; unroll (older candidates scored together in
+ * phase A): 1, 2 or 4; the gap table is used when P.table > 0 */
+hipError_t launch_chain_main(int waves_per_block, int unroll, int grid, const ChainParams &P,
+                             const ChainBuffers &B, hipStream_t s);
 
 } // namespace talchain
 

@@ -4,12 +4,10 @@
  * (src/chaining/parallel_chaining_32_bit.h:216-412), over many anchor sets
  * per call.
  *
- * A call: one kernel validates the sets and finds the largest (one wait on
- * the stream to read that and the error word), the sets are sorted longest
- * first, and the persistent DP kernel runs one set per wave with an LDS ring
- * of the latest anchors; sets whose predecessor window outgrows the ring are
- * redone in the same stream by an overflow pass whose ring holds max(largest
- * set, max_iter + 1) anchors (chain_kernels.hip).
+ * A call: one kernel validates the sets (one wait on the stream to read the
+ * error word), the sets are sorted longest first, and the persistent DP
+ * kernel runs one set per wave with an LDS ring of the latest anchors
+ * (chain_kernels.hip).
  */
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
@@ -63,6 +61,7 @@ struct Summary {
     int32_t ctl[CHAIN_CTL_WORDS];
     unsigned long long maxn;
     unsigned long long examined;
+    unsigned long long far_anchors;
 };
 
 } // namespace
@@ -73,11 +72,22 @@ struct tal_chain {
     hipStream_t stream = nullptr;
     int cu_count = 0;
     size_t lds_per_block = 0, lds_per_cu = 0;
-    Buf keys, keys2, ids, order, over, sum, cubtmp, d_a, d_off, d_f, d_p, d_v;
-    /* anchors in the fast path's LDS ring (TAL_CHAIN_RING): 20 bytes each, so
-     * 1024 holds eight waves per CU; measured on MI355X, smaller rings send
-     * more sets to the slow overflow pass than their occupancy wins back */
-    int small_ring = 1024;
+    Buf keys, keys2, ids, order, sum, cubtmp, d_a, d_off, d_f, d_p, d_v;
+    /* LDS ring slots per wave (TAL_CHAIN_RING, a power of two in 128..4096),
+     * 12 bytes each; waves per block (TAL_CHAIN_WPB: 1, 2 or 4) and waves per
+     * CU (TAL_CHAIN_WAVES_PER_CU, at most what the LDS holds).  Measured on
+     * MI355X: a 1024-slot ring holds the windows of minimap2-like sets (a
+     * shorter one sends windows to the slow HBM reads), and with it 8 waves
+     * per CU (the LDS limit) run as fast as more. */
+    int ring = 1024, wpb = 1, waves_per_cu = 8;
+    /* older candidates scored together in phase A (TAL_CHAIN_UNROLL: 1, 2
+     * or 4), so that their LDS reads overlap */
+    int unroll = 4;
+    /* gap-cost table entries per wave: bw + 2, rounded up to 64, when at
+     * most 2048 (8 KB), else (or TAL_CHAIN_TABLE=0) none: costs computed per
+     * candidate */
+    int table = 0;
+
     Summary *host = nullptr; /* pinned */
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     bool timed = false;
@@ -127,19 +137,6 @@ int run(tal_chain *h, const tal_chain_anchor *a, const int64_t *off, int32_t nse
                                    "anchors");
     if (h->host->ctl[CHAIN_CTL_ERR] == CHAIN_ERR_OFFSETS)
         return fail(h, TAL_EINVAL, "set offsets must be non-negative and non-decreasing");
-    const uint64_t maxn = h->host->maxn;
-    /* Two passes.  The ring pass keeps the latest small_ring anchors in LDS
-     * and hands every set whose predecessor window outgrows it to the
-     * overflow pass, whose f/v ring holds every predecessor an anchor can
-     * reach: max_iter + 1 or the largest set. */
-    const uint64_t need = std::min<uint64_t>(maxn, (uint64_t)h->prm.max_iter + 1);
-    uint64_t big = 64;
-    while (big < need) big <<= 1;
-    const bool overflow_possible = need + 1 > (uint64_t)h->small_ring;
-    if (overflow_possible && chain_lds_bytes(false, (int)big) > h->lds_per_block)
-        return fail(h, TAL_ERANGE, "max_iter %d needs %zu bytes of LDS per wave (at most %zu)",
-                    h->prm.max_iter, chain_lds_bytes(false, (int)big), h->lds_per_block);
-    CHCHK(h, h->over.ensure(sizeof(int32_t) * nsets), "alloc overflow list");
 
     size_t tmp = 0;
     CHCHK(h, hipcub::DeviceRadixSort::SortPairsDescending(
@@ -157,7 +154,8 @@ int run(tal_chain *h, const tal_chain_anchor *a, const int64_t *off, int32_t nse
     P.max_iter = h->prm.max_iter;
     P.dr = dr;
     P.dq = dq;
-    P.ring = h->small_ring;
+    P.ring = h->ring;
+    P.table = h->table;
     ChainBuffers B{};
     B.a = a;
     B.off = off;
@@ -165,28 +163,19 @@ int run(tal_chain *h, const tal_chain_anchor *a, const int64_t *off, int32_t nse
     B.p = p;
     B.v = v;
     B.order = h->order.as<int32_t>();
-    B.over = h->over.as<int32_t>();
     B.ctl = ds->ctl;
     B.examined = &ds->examined;
-    /* one-wave blocks: as many as the LDS of the CUs holds, at most 16 per CU */
-    auto grid_for = [&](size_t lds, int tasks) {
-        const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(16, h->lds_per_cu / lds));
-        return std::max(1, std::min(tasks, h->cu_count * per_cu));
-    };
+    B.far_anchors = &ds->far_anchors;
+    /* a persistent grid: waves_per_cu waves on every CU, fewer blocks when
+     * there are fewer sets */
+    const int waves = std::min(nsets, h->cu_count * h->waves_per_cu);
+    const int grid = std::max(1, (waves + h->wpb - 1) / h->wpb);
     CHCHK(h, hipEventRecord(h->ev[1], s), "event");
-    CHCHK(h, launch_chain_main(true, grid_for(chain_lds_bytes(true, P.ring), nsets), P, B, s),
-          "launch");
-    if (overflow_possible) {
-        /* the overflow pass takes its task count from the device */
-        P.ring = (int32_t)big;
-        CHCHK(h, launch_chain_main(false, grid_for(chain_lds_bytes(false, P.ring), nsets), P, B,
-                                   s),
-              "launch overflow pass");
-    }
+    CHCHK(h, launch_chain_main(h->wpb, h->unroll, grid, P, B, s), "launch");
     CHCHK(h, hipEventRecord(h->ev[2], s), "event");
     h->timed = true;
     h->st.sets = nsets;
-    h->st.lds_ring = h->small_ring;
+    h->st.lds_ring = h->ring;
     return TAL_OK;
 }
 
@@ -215,8 +204,23 @@ int tal_chain_create(const tal_chain_params *params, int device, tal_chain **out
     h->device = device;
     if (const char *e = getenv("TAL_CHAIN_RING")) {
         const int x = atoi(e);
-        if (x >= 64 && x <= 4096 && (x & (x - 1)) == 0) h->small_ring = x;
+        if (x >= 128 && x <= 4096 && (x & (x - 1)) == 0) h->ring = x;
     }
+    if (const char *e = getenv("TAL_CHAIN_WPB")) {
+        const int x = atoi(e);
+        if (x == 1 || x == 2 || x == 4) h->wpb = x;
+    }
+    if (const char *e = getenv("TAL_CHAIN_WAVES_PER_CU")) {
+        const int x = atoi(e);
+        if (x >= 1 && x <= 32) h->waves_per_cu = x;
+    }
+    if (const char *e = getenv("TAL_CHAIN_UNROLL")) {
+        const int x = atoi(e);
+        if (x == 1 || x == 2 || x == 4) h->unroll = x;
+    }
+    if ((int64_t)params->bw + 2 <= 2048) h->table = (params->bw + 2 + 63) & ~63;
+    if (const char *e = getenv("TAL_CHAIN_TABLE"))
+        if (atoi(e) == 0) h->table = 0;
     int rc = TAL_OK;
     do {
         if (hipSetDevice(device) != hipSuccess) { rc = TAL_EDEVICE; break; }
@@ -226,6 +230,13 @@ int tal_chain_create(const tal_chain_params *params, int device, tal_chain **out
         h->lds_per_block = prop.sharedMemPerBlock;
         h->lds_per_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
                                                               : prop.sharedMemPerBlock;
+        /* the ring's LDS bounds the waves per CU and the ring per block */
+        while (chain_lds_bytes(h->ring, h->table, h->wpb) > h->lds_per_block && h->wpb > 1)
+            h->wpb >>= 1;
+        while (chain_lds_bytes(h->ring, h->table, h->wpb) > h->lds_per_block && h->ring > 128)
+            h->ring >>= 1;
+        const int fit = (int)(h->lds_per_cu / chain_lds_bytes(h->ring, h->table, 1));
+        h->waves_per_cu = std::max(h->wpb, std::min(h->waves_per_cu, fit) / h->wpb * h->wpb);
         if (set_chain_lds_limit(h->lds_per_block) != hipSuccess) { rc = TAL_EDEVICE; break; }
         if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
             rc = TAL_EDEVICE;
@@ -252,7 +263,7 @@ void tal_chain_destroy(tal_chain *h)
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    for (Buf *b : {&h->keys, &h->keys2, &h->ids, &h->order, &h->over, &h->sum, &h->cubtmp, &h->d_a,
+    for (Buf *b : {&h->keys, &h->keys2, &h->ids, &h->order, &h->sum, &h->cubtmp, &h->d_a,
                    &h->d_off, &h->d_f, &h->d_p, &h->d_v})
         b->release();
     if (h->host) (void)hipHostFree(h->host);
@@ -333,7 +344,7 @@ int tal_chain_get_stats(const tal_chain *hc, tal_chain_stats *out)
         Summary sm;
         CHCHK(h, hipMemcpy(&sm, h->sum.p, sizeof sm, hipMemcpyDeviceToHost), "read counts");
         h->st.predecessors = (int64_t)sm.examined;
-        h->st.overflow_sets = sm.ctl[CHAIN_CTL_NOVER];
+        h->st.far_anchors = (int64_t)sm.far_anchors;
     }
     *out = h->st;
     return TAL_OK;

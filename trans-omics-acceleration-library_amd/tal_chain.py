@@ -31,7 +31,7 @@ class Params(ctypes.Structure):
 class Stats(ctypes.Structure):
     _fields_ = [("ms_kernel", ctypes.c_float), ("ms_total", ctypes.c_float),
                 ("sets", ctypes.c_int32), ("lds_ring", ctypes.c_int32),
-                ("predecessors", ctypes.c_int64), ("overflow_sets", ctypes.c_int32)]
+                ("predecessors", ctypes.c_int64), ("far_anchors", ctypes.c_int64)]
 
 
 class TalError(RuntimeError):
