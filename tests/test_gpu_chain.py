@@ -41,11 +41,14 @@ def test_synthetic_sets(nsets, lo, hi):
                                      (dict(dr=50, dq=5000), dict(bw=500, max_iter=3)),
                                      (dict(dr=5000, dq=5000), dict(bw=500, max_iter=100)),
                                      (dict(dr=-1, dq=5000), dict(bw=500, max_iter=700)),
-                                     (dict(dr=5000, dq=-3), dict(bw=500, max_iter=5000))])
+                                     (dict(dr=5000, dq=-3), dict(bw=500, max_iter=5000)),
+                                     (dict(dr=2147483000, dq=2147483000),
+                                      dict(bw=1000, max_iter=900))])
 def test_parameters_and_ring_wrap(kw, ctor):
     """Small max_iter keeps the predecessor windows inside the ring, whose
     slots are reused under 3000-anchor sets; dr = -1 (every r within reach)
-    makes windows of max_iter."""
+    makes windows of max_iter; bw + min(dq, dr) >= 2^31 turns the gap table
+    off."""
     _say(f"[gpu] chain params {kw} {ctor}")
     a, off = S.chain_generate(60, 1, 3000, seed=17)
     eng = T.DpChain(**ctor)
@@ -53,21 +56,20 @@ def test_parameters_and_ring_wrap(kw, ctor):
     eng.close()
 
 
-@pytest.mark.parametrize("ring,wpb,unroll,table", [(128, 1, 1, 1), (128, 4, 4, 0), (512, 2, 2, 1),
-                                                   (4096, 1, 4, 1), (1024, 4, 1, 0),
-                                                   (1024, 1, 2, 0), (256, 2, 4, 1)])
-def test_ring_sizes_and_far_predecessors(ring, wpb, unroll, table, monkeypatch):
+@pytest.mark.parametrize("ring,wpb,table", [(128, 1, 1), (128, 4, 0), (512, 2, 1), (4096, 1, 1),
+                                            (1024, 4, 0), (1024, 1, 0), (256, 2, 1),
+                                            (2048, 4, 1)])
+def test_ring_sizes_and_far_predecessors(ring, wpb, table, monkeypatch):
     """A 128-slot ring (64 in reach) sends most windows partly to HBM; 4096
-    none.  Blocks of 1, 2 and 4 waves; 1, 2 and 4 older candidates at a time;
-    gap costs from the table or computed."""
-    _say(f"[gpu] chain ring={ring} waves/block={wpb} unroll={unroll} table={table}")
+    none.  Blocks of 1, 2 and 4 waves sharing a set; gap costs from the
+    table or computed."""
+    _say(f"[gpu] chain ring={ring} waves/block={wpb} table={table}")
     monkeypatch.setenv("TAL_CHAIN_RING", str(ring))
     monkeypatch.setenv("TAL_CHAIN_WPB", str(wpb))
-    monkeypatch.setenv("TAL_CHAIN_UNROLL", str(unroll))
     monkeypatch.setenv("TAL_CHAIN_TABLE", str(table))
     a, off = S.chain_generate(80, 1, 2500, seed=23)
     eng = T.DpChain()
-    _check(eng, a, off, f"ring={ring} wpb={wpb} unroll={unroll} table={table}")
+    _check(eng, a, off, f"ring={ring} wpb={wpb} table={table}")
     st = eng.stats()
     assert st["lds_ring"] == ring
     if ring == 128:

@@ -24,13 +24,14 @@ struct ChainParams {
     int32_t nsets;
     int32_t bw, max_iter, dr, dq;
     int32_t ring;       /* LDS ring slots per wave (power of two, >= 128) */
-    int32_t table;      /* gap-cost table entries per wave (0: none; else >= bw + 2) */
+    int32_t table;      /* gap-cost table entries per block (0: none; else >= bw + 2) */
 };
 
-/* LDS bytes of one wave: the ring (12 bytes a slot) and the gap table */
-__host__ __device__ inline size_t chain_wave_lds(int ring, int table)
+/* LDS bytes of one block: the ring (12 bytes a slot), the gap table, the
+ * waves' phase-A bests (8 bytes a lane) and the task word */
+__host__ __device__ inline size_t chain_block_lds(int ring, int table, int waves_per_block)
 {
-    return 12 * (size_t)ring + 4 * (size_t)table;
+    return 12 * (size_t)ring + 4 * (size_t)table + 512 * (size_t)waves_per_block + 16;
 }
 
 struct ChainBuffers {
@@ -50,9 +51,9 @@ hipError_t launch_chain_check(const tal_chain_anchor *a, const int64_t *off, int
 hipError_t set_chain_lds_limit(size_t bytes);
 /* LDS bytes of one block of the DP kernel */
 size_t chain_lds_bytes(int ring, int table, int waves_per_block);
-/* waves_per_block: 1, 2 or 4; unroll (older candidates scored together in
- * phase A): 1, 2 or 4; the gap table is used when P.table > 0 */
-hipError_t launch_chain_main(int waves_per_block, int unroll, int grid, const ChainParams &P,
+/* waves_per_block: 1, 2 or 4; the gap table (table mode) is used when
+ * P.table > 0, which needs bw + max(min(dq, dr), 0) < 2^31 */
+hipError_t launch_chain_main(int waves_per_block, int grid, const ChainParams &P,
                              const ChainBuffers &B, hipStream_t s);
 
 } // namespace talchain

@@ -128,152 +128,254 @@ struct SetConst {
     uint32_t qlim;    /* max(min(dq, dr), 0): 0 < ddq <= min(dq, dr) as one test */
     float avg_val;    /* (float)(0.01 * avg_qspan) */
     double avg_d;     /* avg_qspan */
-    const int32_t *gt; /* the vector loop's gap cost by dd (LDS), or null */
+    const int32_t *gt; /* table mode: 1 - gap cost of the vector loop by dd (LDS) */
 };
 
-/* How a candidate's gap is priced: the reference's scalar loop (windows of
- * at most 6 candidates), its vector loop from the set's table, or its vector
- * loop computed (bw too large for a table). */
-enum GapMode { GAP_SCALAR, GAP_TABLE, GAP_VEC };
+/* How a candidate is scored: the reference's scalar loop (windows of at
+ * most 6 candidates), its vector loop computed (bw too large for a table,
+ * or bw + min(dq, dr) >= 2^31), or its vector loop from the set's table. */
+enum GapMode { GAP_SCALAR, GAP_VEC, GAP_TABLE };
+
+/* Anchor i as a candidate's score needs it.  In table mode the offsets are
+ * r_i - rbase - 1, q_i - 1 and l_i - 1 (see score()). */
+struct Target {
+    uint32_t r;
+    int32_t q, l;
+};
+
+template <int MODE> __device__ __forceinline__ Target target(uint32_t r, int32_t q, int32_t l)
+{
+    return MODE == GAP_TABLE ? Target{r - 1u, (int32_t)((uint32_t)q - 1u), l - 1} : Target{r, q, l};
+}
 
 /*
- * Candidate j (rj = r_j - rbase, qj, f[j]) of anchor i (ri = r_i - rbase, qi,
- * li): the reference's filters and score (:262-293 for the vector loop,
- * :386-420 for the scalar one): |ddr - ddq| <= bw, ddr != 0, 0 < ddq <=
- * min(dq, dr), score f[j] + min(ddr, ddq, l_i) - gap in int32 wrap.  Returns
- * whether it passes; the score in sc.
+ * Candidate j (rj = r_j - rbase, qj, f[j]) of anchor i: the reference's
+ * filters and score (:262-293 for the vector loop, :386-420 for the scalar
+ * one): |ddr - ddq| <= bw, ddr != 0, 0 < ddq <= min(dq, dr), score f[j] +
+ * min(ddr, ddq, l_i) - gap in int32 wrap.  Returns whether it passes; the
+ * score in sc.
+ *
+ * Table mode works on a = ddr - 1 and u = ddq - 1 (the target holds r_i - 1
+ * and q_i - 1): ddr != 0 is a != -1, 0 < ddq <= qlim is u < qlim unsigned,
+ * ddr - ddq = a - u, and min(ddr, ddq, l_i) = min(a, u, l_i - 1) + 1 with the
+ * + 1 in the table (1 - gap).  That min is exact unless ddr = INT32_MIN (a
+ * wraps); such a candidate has |ddr - ddq| >= 2^31 - qlim > bw when bw +
+ * qlim < 2^31, which the host requires of table mode, so it fails anyway.
  */
 template <int MODE>
-__device__ __forceinline__ bool score(const SetConst &C, uint32_t ri, int32_t qi, int32_t li,
-                                      uint32_t rj, int32_t qj, int32_t fj, int32_t &sc)
+__device__ __forceinline__ bool score(const SetConst &C, const Target &t, uint32_t rj, int32_t qj,
+                                      int32_t fj, int32_t &sc)
 {
-    const int32_t ddr = (int32_t)(ri - rj);
-    const int32_t ddq = (int32_t)((uint32_t)qi - (uint32_t)qj);
+    if (MODE == GAP_TABLE) {
+        const int32_t a = (int32_t)(t.r - rj);
+        const int32_t u = (int32_t)((uint32_t)t.q - (uint32_t)qj);
+        const int32_t diff = (int32_t)((uint32_t)a - (uint32_t)u);
+        const int32_t dd = diff < 0 ? (int32_t)(0u - (uint32_t)diff) : diff;
+        /* dd in [0, bw] at its own entry; INT32_MIN (and dd > bw) at bw + 1 */
+        const int32_t g = C.gt[min((uint32_t)dd, (uint32_t)C.bw + 1u)];
+        sc = (int32_t)((uint32_t)min(min(a, u), t.l) + (uint32_t)fj + (uint32_t)g);
+        /* (bitwise: one mask, no branches) */
+        return !(dd > C.bw) & (a != -1) & ((uint32_t)u < C.qlim);
+    }
+    const int32_t ddr = (int32_t)(t.r - rj);
+    const int32_t ddq = (int32_t)((uint32_t)t.q - (uint32_t)qj);
     const int32_t diff = (int32_t)((uint32_t)ddr - (uint32_t)ddq);
     const int32_t dd = diff < 0 ? (int32_t)(0u - (uint32_t)diff) : diff;
-    const int32_t oc = min(min(ddr, ddq), li);
-    int32_t gap;
-    if (MODE == GAP_TABLE) /* dd in [0, bw] at its own entry; INT32_MIN (and dd > bw) at bw + 1 */
-        gap = C.gt[min((uint32_t)dd, (uint32_t)C.bw + 1u)];
-    else if (MODE == GAP_VEC)
-        gap = gap_vec(dd, C.avg_val);
-    else
-        gap = gap_scalar(dd, C.avg_d);
+    const int32_t oc = min(min(ddr, ddq), t.l);
+    const int32_t gap = MODE == GAP_VEC ? gap_vec(dd, C.avg_val) : gap_scalar(dd, C.avg_d);
     sc = (int32_t)((uint32_t)fj + (uint32_t)oc - (uint32_t)gap);
-    /* (bitwise: one mask, no branches) */
     return !(dd > C.bw) & (ddr != 0) & ((uint32_t)ddq - 1u < C.qlim);
 }
 
+/* One phase-A candidate j: keep it if it passes, is in the lane's window and
+ * beats the lane's best (strict: j decreases, ties stay on the higher j). */
+template <int MODE>
+__device__ __forceinline__ void take_a(const SetConst &C, const Target &t, int32_t st, int32_t j,
+                                       uint32_t rj, int32_t qj, int32_t fj, int32_t &bsc,
+                                       int32_t &bj)
+{
+    int32_t sc;
+    const bool ok = score<MODE>(C, t, rj, qj, fj, sc);
+    if (ok & (j >= st) & (sc > bsc)) {
+        bsc = sc;
+        bj = j;
+    }
+}
+
 /*
- * Phase A of a chunk: lane k takes anchor c0 + k (ri, qi, li; its window
+ * Phase A of a chunk: lane k takes anchor c0 + k (target t; its window
  * starts at st, INT32_MAX for lanes past the set) and scores the candidates
  * j in [max(st, jlo), jhi], all older than the chunk, j decreasing: every
- * lane reads the same j (one broadcast read), so the loop has no divergence
- * and its iterations are independent but for the running best.  Strict >
- * keeps the highest j of a lane's best score.  Candidates come from the
- * ring (RING) or from the anchor array and the stored f outputs.
+ * lane reads the same j (broadcast reads), so the loop has no divergence
+ * and its iterations are independent but for the running best.  From the
+ * ring (RING) four slots at a time where they are contiguous (j = 3 mod 4:
+ * two 16-byte reads of rq and one of fr), else from the anchor array and the
+ * stored f outputs.
  */
-template <int MODE, int UNROLL, bool RING>
-__device__ __forceinline__ void phase_a(const SetConst &C, uint32_t ri, int32_t qi, int32_t li,
-                                        int32_t st, int32_t jhi, int32_t jlo, int mask,
-                                        const uint2 *rq, const int32_t *fr,
-                                        const tal_chain_anchor *A, const uint32_t *F,
-                                        uint32_t rbase, int32_t &bsc, int32_t &bj)
+template <int MODE, bool RING>
+__device__ __forceinline__ void phase_a(const SetConst &C, const Target &t, int32_t st,
+                                        int32_t jhi, int32_t jlo, int mask, const uint2 *rq,
+                                        const int32_t *fr, const tal_chain_anchor *A,
+                                        const uint32_t *F, uint32_t rbase, int32_t &bsc,
+                                        int32_t &bj)
 {
     int32_t j = jhi;
-    for (; j - (UNROLL - 1) >= jlo; j -= UNROLL) {
-        uint2 e[UNROLL];
-        int32_t g[UNROLL];
+    if (RING) {
+        for (; j >= jlo && (j & 3) != 3; j--) {
+            const uint2 e = rq[j & mask];
+            take_a<MODE>(C, t, st, j, e.x, (int32_t)e.y, fr[j & mask], bsc, bj);
+        }
+        for (; j - 3 >= jlo; j -= 4) {
+            const int s0 = (j & mask) - 3; /* R is a multiple of 4: no wrap inside */
+            const uint4 x0 = *(const uint4 *)(rq + s0);
+            const uint4 x1 = *(const uint4 *)(rq + s0 + 2);
+            const int4 f4 = *(const int4 *)(fr + s0);
+            take_a<MODE>(C, t, st, j, x1.z, (int32_t)x1.w, f4.w, bsc, bj);
+            take_a<MODE>(C, t, st, j - 1, x1.x, (int32_t)x1.y, f4.z, bsc, bj);
+            take_a<MODE>(C, t, st, j - 2, x0.z, (int32_t)x0.w, f4.y, bsc, bj);
+            take_a<MODE>(C, t, st, j - 3, x0.x, (int32_t)x0.y, f4.x, bsc, bj);
+        }
+        for (; j >= jlo; j--) {
+            const uint2 e = rq[j & mask];
+            take_a<MODE>(C, t, st, j, e.x, (int32_t)e.y, fr[j & mask], bsc, bj);
+        }
+    } else {
+        for (; j - 3 >= jlo; j -= 4) {
+            tal_chain_anchor x[4];
+            int32_t g[4];
 #pragma unroll
-        for (int u = 0; u < UNROLL; u++) {
-            if (RING) {
-                e[u] = rq[(j - u) & mask];
-                g[u] = fr[(j - u) & mask];
-            } else {
-                const tal_chain_anchor x = A[j - u];
-                e[u] = make_uint2((uint32_t)x.r - rbase, (uint32_t)x.q);
+            for (int u = 0; u < 4; u++) {
+                x[u] = A[j - u];
                 g[u] = (int32_t)F[j - u];
             }
-        }
 #pragma unroll
-        for (int u = 0; u < UNROLL; u++) {
-            int32_t sc;
-            const bool ok = score<MODE>(C, ri, qi, li, e[u].x, (int32_t)e[u].y, g[u], sc);
-            if (ok & (j - u >= st) & (sc > bsc)) {
-                bsc = sc;
-                bj = j - u;
-            }
+            for (int u = 0; u < 4; u++)
+                take_a<MODE>(C, t, st, j - u, (uint32_t)x[u].r - rbase, x[u].q, g[u], bsc, bj);
         }
-    }
-    for (; j >= jlo; j--) {
-        uint2 e;
-        int32_t g;
-        if (RING) {
-            e = rq[j & mask];
-            g = fr[j & mask];
-        } else {
+        for (; j >= jlo; j--) {
             const tal_chain_anchor x = A[j];
-            e = make_uint2((uint32_t)x.r - rbase, (uint32_t)x.q);
-            g = (int32_t)F[j];
-        }
-        int32_t sc;
-        const bool ok = score<MODE>(C, ri, qi, li, e.x, (int32_t)e.y, g, sc);
-        if (ok & (j >= st) & (sc > bsc)) {
-            bsc = sc;
-            bj = j;
+            take_a<MODE>(C, t, st, j, (uint32_t)x.r - rbase, x.q, (int32_t)F[j], bsc, bj);
         }
     }
 }
 
 /*
- * The DP: one set per wave at a time, WPB independent waves per block (each
- * with its own slice of LDS, no barriers), 64 anchors (a chunk) at a time:
- *
- *  1. st of each anchor of the chunk (:232-235).  Sets whose r are sorted
- *     and span less than 2^32 (minimap2's anchors are sorted by r): lane k
- *     binary-searches the first p >= st_prev with r_p >= r_i - dr in the
- *     ring (the first p within dr, which is where the reference's forward
- *     scan stops), then applies the max_iter clamp; for sorted r the
- *     reference's running st is max(that, i - max_iter).  Other sets (and
- *     windows past the ring): in order, from a window of 64 r values held
- *     one per lane (reloaded when st passes it), on 64-bit r.
- *  2. Phase A, lanes = anchors: the best candidate older than the chunk;
- *     then v of it, read from the v outputs.
- *  3. Phase B, in order: anchor c0 + k scores the candidates inside the
- *     chunk (lanes m < k, whose f and v are final), then takes the best of
- *     those (their j are higher, so they win ties) and of its phase-A best:
- *     f, p and v (:304-408).
- *  4. f goes to the LDS ring, the outputs to HBM.
- *
- * LDS ring of `ring` (R) slots per wave: rq[slot] = {r - rbase (low 32 bits),
- * q}, written when a chunk's anchors are loaded, and fr[slot] = f, written
- * when the chunk ends; at phase A of chunk c0 the ring holds every j >= c0 +
- * 64 - R.  Older candidates (windows longer than the ring), and v, are read
- * from the anchor array and from the outputs, which this wave stored when
- * their chunk ended.  The candidates use the low 32 bits of r, as the
- * reference's uint32 SoA does.
+ * Phase B of a chunk: the anchors c0 .. c0 + kend - 1 in order.  Lane k
+ * keeps the best candidate inside the chunk seen so far for anchor c0 + k
+ * (isc, ij, iv).  Anchor m is finished from its lane's intra-chunk best and
+ * its phase-A best (osc, oj, ov; the intra-chunk j are higher, so they win
+ * ties): f, p and v (:304-408).  Then its f is final, and every later lane
+ * k > m scores candidate m for its own anchor; m increases, so >= keeps
+ * the highest j of equal scores.  No wave reduction is needed.  SCAL: some
+ * anchor of the chunk (scal) takes the reference's scalar loop and its
+ * double gap.
  */
-template <int WPB, int UNROLL, bool TABLE>
+template <int MODE, bool SCAL>
+__device__ __forceinline__ void phase_b(const SetConst &C, int32_t c0, int32_t kend, int lane,
+                                        uint32_t rr, int32_t qq, int32_t ll, int32_t st_lane,
+                                        bool scal, int32_t osc, int32_t oj, int32_t ov,
+                                        int32_t &fk, int32_t &pk, int32_t &vk)
+{
+    const Target t = target<MODE>(rr, qq, ll);
+    const Target ts = target<GAP_SCALAR>(rr, qq, ll);
+    int32_t isc = INT32_MIN, ij = -1, iv = 0;
+    for (int32_t m = 0; m < kend; m++) {
+        /* candidate m's own data comes first: its score terms do not wait
+         * for its f */
+        const uint32_t rj = __builtin_amdgcn_readlane(rr, m);
+        const int32_t qj = __builtin_amdgcn_readlane(qq, m);
+        const int32_t li = __builtin_amdgcn_readlane(ll, m);
+        const int32_t t_isc = __builtin_amdgcn_readlane(isc, m);
+        const int32_t t_osc = __builtin_amdgcn_readlane(osc, m);
+        const int32_t t_ij = __builtin_amdgcn_readlane(ij, m);
+        const int32_t t_oj = __builtin_amdgcn_readlane(oj, m);
+        const int32_t t_iv = __builtin_amdgcn_readlane(iv, m);
+        const int32_t t_ov = __builtin_amdgcn_readlane(ov, m);
+        const bool intra = t_isc >= t_osc;
+        const int32_t top = intra ? t_isc : t_osc;
+        const bool take = top > li; /* top > l_i >= 0: a candidate, not a sentinel */
+        const int32_t vp = intra ? t_iv : t_ov;
+        const int32_t f = take ? top : li;
+        const int32_t pj = take ? (intra ? t_ij : t_oj) : -1;
+        const int32_t v = take && vp > f ? vp : f;
+        fk = lane == m ? f : fk;
+        pk = lane == m ? pj : pk;
+        vk = lane == m ? v : vk;
+        /* candidate c0 + m for the later anchors of the chunk */
+        int32_t sc;
+        bool ok = score<MODE>(C, t, rj, qj, f, sc);
+        if (SCAL) {
+            int32_t sc2;
+            const bool ok2 = score<GAP_SCALAR>(C, ts, rj, qj, f, sc2);
+            sc = scal ? sc2 : sc;
+            ok = scal ? ok2 : ok;
+        }
+        if (ok & (lane > m) & (c0 + m >= st_lane) & (sc >= isc)) {
+            isc = sc;
+            ij = c0 + m;
+            iv = v;
+        }
+    }
+}
+
+/*
+ * The DP: one set per block at a time (persistent grid, sets longest
+ * first); the WPB waves of a block share the set's LDS ring and split its
+ * phase A.  64 anchors (a chunk) at a time:
+ *
+ *  1. st of each anchor of the chunk (:232-235), by every wave (the same
+ *     result).  Sets whose r are sorted and span less than 2^32
+ *     (minimap2's anchors are sorted by r): lane k binary-searches the first
+ *     p >= st_prev with r_p >= r_i - dr in the ring (the first p within dr,
+ *     which is where the reference's forward scan stops), then applies the
+ *     max_iter clamp; for sorted r the reference's running st is max(that,
+ *     i - max_iter).  Other sets (and windows past the ring): in order, from
+ *     a window of 64 r values held one per lane (reloaded when st passes
+ *     it), on 64-bit r.
+ *  2. Phase A, lanes = anchors: the best candidate older than the chunk,
+ *     the candidates split in WPB contiguous ranges, one per wave; wave 0
+ *     takes the best of the ranges (ties to the highest j), then v of it
+ *     from the v outputs.
+ *  3. Phase B (wave 0), in order: anchor c0 + k takes the best of its
+ *     phase-A best and of the candidates inside the chunk (f, p, v); its f
+ *     is then final and the later anchors score it as a candidate.
+ *  4. Wave 0 writes f to the ring and the outputs to HBM.
+ *
+ * LDS ring of `ring` (R) slots per block: rq[slot] = {r - rbase (low 32
+ * bits), q}, written when a chunk's anchors are loaded, and fr[slot] = f,
+ * written when the chunk ends; at phase A of chunk c0 the ring holds every j
+ * >= c0 + 64 - R.  Older candidates (windows longer than the ring), and v,
+ * are read from the anchor array and from the outputs, which wave 0 stored
+ * when their chunk ended.  The candidates use the low 32 bits of r, as the
+ * reference's uint32 SoA does.  Two barriers a chunk: after wave 0's ring
+ * writes, and after the phase-A ranges.
+ */
+template <int WPB, bool TABLE>
 __global__ void __launch_bounds__(64 * WPB) chain_main(ChainParams P, ChainBuffers B)
 {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const int R = P.ring, mask = R - 1;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    uint2 *rq = (uint2 *)(lds_raw + (size_t)wave * chain_wave_lds(R, P.table));
+    /* (readfirstlane: the compiler then knows the wave index is uniform,
+     * and keeps what derives from it in scalar registers) */
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    uint2 *rq = (uint2 *)lds_raw;
     int32_t *fr = (int32_t *)(rq + R);
-    int32_t *gt = fr + R; /* P.table entries */
+    int32_t *gt = fr + R;                  /* P.table entries */
+    int2 *part = (int2 *)(gt + P.table);  /* WPB x 64 phase-A bests {score, j} */
+    int32_t *task_word = (int32_t *)(part + 64 * WPB);
     const uint64_t dr64 = (uint64_t)(uint32_t)P.dr;
     constexpr int AMODE = TABLE ? GAP_TABLE : GAP_VEC;
-    /* The task loop keeps every lane on one path: no continue, and the
-     * lane-0-only updates (the counts) after the loop.  A lane-divergent
-     * branch at the end of the body let the compiler's loop structure send
-     * lanes 1..63 back to the loop head without lane 0, where readfirstlane
-     * then read their task word, not lane 0's. */
+    /* Every loop here is block-uniform: the task comes through LDS after a
+     * barrier, and the lane-0 updates (the counts) come after the loop.  A
+     * lane-divergent branch at the end of a loop body once let the
+     * compiler's loop structure send lanes 1..63 back to the loop head
+     * without lane 0. */
     uint64_t examined = 0, far_anchors = 0;
     for (;;) {
-        int task = 0;
-        if (lane == 0) task = atomicAdd(&B.ctl[CHAIN_CTL_COUNTER], 1);
-        task = __builtin_amdgcn_readfirstlane(task);
+        if (threadIdx.x == 0) *task_word = atomicAdd(&B.ctl[CHAIN_CTL_COUNTER], 1);
+        __syncthreads();
+        const int task = __builtin_amdgcn_readfirstlane(*task_word);
+        __syncthreads(); /* everyone has read the word before it is rewritten */
         if (task >= P.nsets) break;
         const int set = B.order[task];
         const int64_t a0 = B.off[set];
@@ -300,11 +402,12 @@ __global__ void __launch_bounds__(64 * WPB) chain_main(ChainParams P, ChainBuffe
         C.avg_val = (float)(0.01 * (double)avg_qspan);
         C.avg_d = (double)avg_qspan;
         C.gt = gt;
-        /* the vector loop's gap costs for dd = 0 .. bw, and for dd =
-         * INT32_MIN (|INT32_MIN| wraps to itself) at bw + 1 */
+        /* 1 - the vector loop's gap cost for dd = 0 .. bw, and for dd =
+         * INT32_MIN (|INT32_MIN| wraps to itself) at bw + 1; read after the
+         * first chunk's barrier */
         if (TABLE)
-            for (int d = lane; d <= P.bw + 1; d += 64)
-                gt[d] = gap_vec(d <= P.bw ? d : INT32_MIN, C.avg_val);
+            for (int d = threadIdx.x; d <= P.bw + 1; d += 64 * WPB)
+                gt[d] = (int32_t)(1u - (uint32_t)gap_vec(d <= P.bw ? d : INT32_MIN, C.avg_val));
         /* any base works for the low 32 bits of r; with the set's first r,
          * r - rbase is exact for sorted sets */
         const uint32_t rbase = (uint32_t)r_first;
@@ -316,11 +419,15 @@ __global__ void __launch_bounds__(64 * WPB) chain_main(ChainParams P, ChainBuffe
         if (lane < n) nxt = A4[lane];
         for (int32_t c0 = 0; c0 < n; c0 += 64) {
             const uint4 cur = nxt;
-            if (c0 + 64 + lane < n) nxt = A4[c0 + 64 + lane];
             const int32_t kend = min(64, n - c0);
             const uint32_t rr = cur.x - rbase; /* this lane's anchor: r - rbase, q, l */
             const int32_t qq = (int32_t)cur.z, ll = (int32_t)cur.w;
-            if (lane < kend) rq[(c0 + lane) & mask] = make_uint2(rr, cur.z);
+            if (wave == 0 && lane < kend) rq[(c0 + lane) & mask] = make_uint2(rr, cur.z);
+            /* the ring (and wave 0's outputs) of the chunks so far; the
+             * barrier waits for this wave's loads too, so the next chunk's
+             * anchors are loaded after it */
+            __syncthreads();
+            if (c0 + 64 + lane < n) nxt = A4[c0 + 64 + lane];
             const int32_t lo = c0 + 64 - R; /* the oldest j in the ring */
 
             /* 1. st of every anchor of the chunk */
@@ -371,91 +478,80 @@ __global__ void __launch_bounds__(64 * WPB) chain_main(ChainParams P, ChainBuffe
                     st_lane = lane == k ? st : st_lane;
                 }
             }
-            examined += lane < kend ? (uint64_t)(c0 + lane - st_lane) : 0;
 
-            /* 2. phase A: candidates older than the chunk */
+            /* 2. phase A: candidates older than the chunk, one contiguous
+             * range [jlo_w, jhi_w] per wave, wave 0 the highest */
             const int32_t jmin = __builtin_amdgcn_readfirstlane(st_lane); /* lane 0: the smallest */
-            far_anchors += lane < kend && st_lane < lo ? 1 : 0;
             int32_t osc = INT32_MIN, oj = -1, ov = 0;
+            const Target ta = target<AMODE>(rr, qq, ll);
             if (jmin < c0) {
-                phase_a<AMODE, UNROLL, true>(C, rr, qq, ll, st_lane, c0 - 1, max(jmin, lo), mask,
-                                             rq, fr, A, F, rbase, osc, oj);
-                /* this wave stored the outputs of the older chunks; wait
-                 * until the stores are done */
-                __builtin_amdgcn_s_waitcnt(0x0F70); /* vmcnt(0) */
-                if (jmin < lo)
-                    phase_a<AMODE, UNROLL, false>(C, rr, qq, ll, st_lane, lo - 1, jmin, mask, rq,
-                                                  fr, A, F, rbase, osc, oj);
-                /* anchors whose window has at most 6 candidates take the
-                 * reference's scalar loop and its double gap: redo them
-                 * (their old candidates are j >= c0 - 6, in the ring) */
-                const bool scal = lane < kend && c0 + lane - 1 - st_lane <= 5;
-                if (__ballot(scal)) {
-                    int32_t ssc = INT32_MIN, sj = -1;
-                    phase_a<GAP_SCALAR, 1, true>(C, rr, qq, ll, st_lane, c0 - 1,
-                                                 max(jmin, c0 - 6), mask, rq, fr, A, F, rbase,
-                                                 ssc, sj);
-                    osc = scal ? ssc : osc;
-                    oj = scal ? sj : oj;
-                }
-                if (oj >= 0) ov = V[oj];
+                const int32_t span = c0 - jmin;
+                const int32_t per = ((span + WPB - 1) / WPB + 3) / 4 * 4;
+                const int32_t jhi_w = c0 - 1 - wave * per;
+                const int32_t jlo_w = max(jmin, c0 - (wave + 1) * per);
+                if (jhi_w >= lo)
+                    phase_a<AMODE, true>(C, ta, st_lane, jhi_w, max(jlo_w, lo), mask, rq, fr, A,
+                                         F, rbase, osc, oj);
+                if (jlo_w < lo) /* F holds wave 0's outputs of the older chunks
+                                 * (stored before the barrier) */
+                    phase_a<AMODE, false>(C, ta, st_lane, min(jhi_w, lo - 1), jlo_w, mask, rq,
+                                          fr, A, F, rbase, osc, oj);
             }
-
-            /* 3. phase B: the anchors in order */
-            int32_t fk = 0, pk = -1, vk = 0;
-            for (int32_t k = 0; k < kend; k++) {
-                const int32_t i = c0 + k;
-                const int32_t sti = __builtin_amdgcn_readlane(st_lane, k);
-                const uint32_t ri = __builtin_amdgcn_readlane(rr, k);
-                const int32_t qi = __builtin_amdgcn_readlane(qq, k);
-                const int32_t li = __builtin_amdgcn_readlane(ll, k);
-                /* candidates c0 + m, m < k, within the window */
-                const bool inw = lane < k && c0 + lane >= sti;
-                int32_t sc = INT32_MIN;
-                bool ok;
-                if (i - 1 - sti <= 5)
-                    ok = score<GAP_SCALAR>(C, ri, qi, li, rr, qq, fk, sc);
-                else
-                    ok = score<GAP_VEC>(C, ri, qi, li, rr, qq, fk, sc);
-                ok &= inw;
-                const int32_t s = ok ? sc : INT32_MIN;
-                const int32_t msc = __ballot(ok) ? wave_max_i32(s) : INT32_MIN;
-                const int32_t osc_i = __builtin_amdgcn_readlane(osc, k);
-                /* the chunk's candidates have the higher j: they win ties */
-                const bool intra = msc >= osc_i;
-                const int32_t top = intra ? msc : osc_i;
-                int32_t f = li, pj = -1, v = li;
-                if (top > li) { /* top > l_i >= 0: a candidate, not a sentinel */
-                    int32_t vp;
-                    if (intra) {
-                        const uint64_t tie = __ballot(ok && s == msc);
-                        const int m = 63 - __builtin_clzll(tie); /* the highest j */
-                        pj = c0 + m;
-                        vp = __builtin_amdgcn_readlane(vk, m);
-                    } else {
-                        pj = __builtin_amdgcn_readlane(oj, k);
-                        vp = __builtin_amdgcn_readlane(ov, k);
+            if (WPB > 1) {
+                if (wave > 0) part[wave * 64 + lane] = make_int2(osc, oj);
+                __syncthreads();
+            }
+            if (wave == 0) {
+                if (jmin < c0) {
+                    /* the later ranges have the lower j: strict > keeps ties
+                     * on the higher j */
+                    for (int w = 1; w < WPB; w++) {
+                        const int2 x = part[w * 64 + lane];
+                        if (x.x > osc) {
+                            osc = x.x;
+                            oj = x.y;
+                        }
                     }
-                    f = top;
-                    v = vp > f ? vp : f;
+                    /* anchors whose window has at most 6 candidates take the
+                     * reference's scalar loop and its double gap: redo them
+                     * (their old candidates are j >= c0 - 6, in the ring) */
+                    const bool scal = lane < kend && c0 + lane - 1 - st_lane <= 5;
+                    if (__ballot(scal)) {
+                        int32_t ssc = INT32_MIN, sj = -1;
+                        phase_a<GAP_SCALAR, true>(C, target<GAP_SCALAR>(rr, qq, ll), st_lane,
+                                                  c0 - 1, max(jmin, c0 - 6), mask, rq, fr, A, F,
+                                                  rbase, ssc, sj);
+                        osc = scal ? ssc : osc;
+                        oj = scal ? sj : oj;
+                    }
+                    if (oj >= 0) ov = V[oj];
                 }
-                fk = lane == k ? f : fk;
-                pk = lane == k ? pj : pk;
-                vk = lane == k ? v : vk;
-            }
+                examined += lane < kend ? (uint64_t)(c0 + lane - st_lane) : 0;
+                far_anchors += lane < kend && st_lane < lo ? 1 : 0;
 
-            /* 4. the chunk's f into the ring, the outputs to HBM */
-            if (lane < kend) {
-                fr[(c0 + lane) & mask] = fk;
-                B.f[a0 + c0 + lane] = (uint32_t)fk;
-                B.p[a0 + c0 + lane] = pk;
-                B.v[a0 + c0 + lane] = vk;
+                /* 3. phase B: the anchors in order */
+                const bool scal = lane < kend && c0 + lane - 1 - st_lane <= 5;
+                int32_t fk = 0, pk = -1, vk = 0;
+                if (__ballot(scal))
+                    phase_b<AMODE, true>(C, c0, kend, lane, rr, qq, ll, st_lane, scal, osc, oj,
+                                         ov, fk, pk, vk);
+                else
+                    phase_b<AMODE, false>(C, c0, kend, lane, rr, qq, ll, st_lane, scal, osc, oj,
+                                          ov, fk, pk, vk);
+
+                /* 4. the chunk's f into the ring, the outputs to HBM */
+                if (lane < kend) {
+                    fr[(c0 + lane) & mask] = fk;
+                    B.f[a0 + c0 + lane] = (uint32_t)fk;
+                    B.p[a0 + c0 + lane] = pk;
+                    B.v[a0 + c0 + lane] = vk;
+                }
             }
         }
     }
     examined = wave_sum_u64(examined);
     far_anchors = wave_sum_u64(far_anchors);
-    if (lane == 0) {
+    if (threadIdx.x == 0) {
         atomicAdd(B.examined, (unsigned long long)examined);
         atomicAdd(B.far_anchors, (unsigned long long)far_anchors);
     }
@@ -471,16 +567,13 @@ hipError_t launch_chain_check(const tal_chain_anchor *a, const int64_t *off, int
     return hipGetLastError();
 }
 
-#define CHAIN_KERNELS(X)                                                                           \
-    X(1, 1, true) X(1, 2, true) X(1, 4, true) X(2, 1, true) X(2, 2, true) X(2, 4, true)            \
-    X(4, 1, true) X(4, 2, true) X(4, 4, true) X(1, 1, false) X(1, 2, false) X(1, 4, false)         \
-    X(2, 1, false) X(2, 2, false) X(2, 4, false) X(4, 1, false) X(4, 2, false) X(4, 4, false)
+#define CHAIN_KERNELS(X) X(1, true) X(2, true) X(4, true) X(1, false) X(2, false) X(4, false)
 
 hipError_t set_chain_lds_limit(size_t bytes)
 {
-#define SET_LIMIT(W, U, T)                                                                         \
+#define SET_LIMIT(W, T)                                                                         \
     {                                                                                              \
-        const hipError_t e = hipFuncSetAttribute((const void *)chain_main<W, U, T>,                \
+        const hipError_t e = hipFuncSetAttribute((const void *)chain_main<W, T>,                   \
                                                  hipFuncAttributeMaxDynamicSharedMemorySize,       \
                                                  (int)bytes);                                      \
         if (e != hipSuccess) return e;                                                             \
@@ -492,16 +585,16 @@ hipError_t set_chain_lds_limit(size_t bytes)
 
 size_t chain_lds_bytes(int ring, int table, int waves_per_block)
 {
-    return chain_wave_lds(ring, table) * (size_t)waves_per_block;
+    return chain_block_lds(ring, table, waves_per_block);
 }
 
-hipError_t launch_chain_main(int waves_per_block, int unroll, int grid, const ChainParams &P,
+hipError_t launch_chain_main(int waves_per_block, int grid, const ChainParams &P,
                              const ChainBuffers &B, hipStream_t s)
 {
     const size_t lds = chain_lds_bytes(P.ring, P.table, waves_per_block);
-#define LAUNCH(W, U, T)                                                                            \
-    if (waves_per_block == W && unroll == U && (P.table > 0) == T) {                               \
-        hipLaunchKernelGGL((chain_main<W, U, T>), dim3(grid), dim3(64 * W), lds, s, P, B);        \
+#define LAUNCH(W, T)                                                                               \
+    if (waves_per_block == W && (P.table > 0) == T) {                                              \
+        hipLaunchKernelGGL((chain_main<W, T>), dim3(grid), dim3(64 * W), lds, s, P, B);           \
         return hipGetLastError();                                                                  \
     }
     CHAIN_KERNELS(LAUNCH)

@@ -73,16 +73,15 @@ struct tal_chain {
     int cu_count = 0;
     size_t lds_per_block = 0, lds_per_cu = 0;
     Buf keys, keys2, ids, order, sum, cubtmp, d_a, d_off, d_f, d_p, d_v;
-    /* LDS ring slots per wave (TAL_CHAIN_RING, a power of two in 128..4096),
-     * 12 bytes each; waves per block (TAL_CHAIN_WPB: 1, 2 or 4) and waves per
-     * CU (TAL_CHAIN_WAVES_PER_CU, at most what the LDS holds).  Measured on
-     * MI355X: a 1024-slot ring holds the windows of minimap2-like sets (a
-     * shorter one sends windows to the slow HBM reads), and with it 8 waves
-     * per CU (the LDS limit) run as fast as more. */
-    int ring = 1024, wpb = 1, waves_per_cu = 8;
-    /* older candidates scored together in phase A (TAL_CHAIN_UNROLL: 1, 2
-     * or 4), so that their LDS reads overlap */
-    int unroll = 4;
+    /* LDS ring slots per block (TAL_CHAIN_RING, a power of two in
+     * 128..4096), 12 bytes each; waves per block, which share a set
+     * (TAL_CHAIN_WPB: 1, 2 or 4), and waves per CU (TAL_CHAIN_WAVES_PER_CU,
+     * at most what the LDS holds).  Measured on MI355X with 4000 sets of
+     * 200..4000 anchors: a 1024-slot ring holds the windows (512 sends 3% of
+     * the anchors' older candidates to HBM, 7% slower), 2 waves a set at 20
+     * a CU run 16% faster than 1 at 8, 4 a set are slower (phase B, one
+     * wave's, waits for the shared phase A). */
+    int ring = 1024, wpb = 2, waves_per_cu = 20;
     /* gap-cost table entries per wave: bw + 2, rounded up to 64, when at
      * most 2048 (8 KB), else (or TAL_CHAIN_TABLE=0) none: costs computed per
      * candidate */
@@ -155,7 +154,9 @@ int run(tal_chain *h, const tal_chain_anchor *a, const int64_t *off, int32_t nse
     P.dr = dr;
     P.dq = dq;
     P.ring = h->ring;
-    P.table = h->table;
+    /* table mode needs bw + min(dq, dr) < 2^31 (see the kernels) */
+    const int64_t qlim = std::max<int64_t>(std::min(dq, dr), 0);
+    P.table = (int64_t)h->prm.bw + qlim < ((int64_t)1 << 31) ? h->table : 0;
     ChainBuffers B{};
     B.a = a;
     B.off = off;
@@ -166,12 +167,15 @@ int run(tal_chain *h, const tal_chain_anchor *a, const int64_t *off, int32_t nse
     B.ctl = ds->ctl;
     B.examined = &ds->examined;
     B.far_anchors = &ds->far_anchors;
-    /* a persistent grid: waves_per_cu waves on every CU, fewer blocks when
-     * there are fewer sets */
-    const int waves = std::min(nsets, h->cu_count * h->waves_per_cu);
-    const int grid = std::max(1, (waves + h->wpb - 1) / h->wpb);
+    /* a persistent grid of blocks of wpb waves, one set per block at a
+     * time: as many blocks as the LDS and waves_per_cu allow on every CU,
+     * fewer when there are fewer sets */
+    const int per_cu = std::max(
+        1, std::min<int>(h->waves_per_cu / h->wpb,
+                         (int)(h->lds_per_cu / chain_lds_bytes(P.ring, P.table, h->wpb))));
+    const int grid = std::max(1, std::min(nsets, h->cu_count * per_cu));
     CHCHK(h, hipEventRecord(h->ev[1], s), "event");
-    CHCHK(h, launch_chain_main(h->wpb, h->unroll, grid, P, B, s), "launch");
+    CHCHK(h, launch_chain_main(h->wpb, grid, P, B, s), "launch");
     CHCHK(h, hipEventRecord(h->ev[2], s), "event");
     h->timed = true;
     h->st.sets = nsets;
@@ -214,10 +218,6 @@ int tal_chain_create(const tal_chain_params *params, int device, tal_chain **out
         const int x = atoi(e);
         if (x >= 1 && x <= 32) h->waves_per_cu = x;
     }
-    if (const char *e = getenv("TAL_CHAIN_UNROLL")) {
-        const int x = atoi(e);
-        if (x == 1 || x == 2 || x == 4) h->unroll = x;
-    }
     if ((int64_t)params->bw + 2 <= 2048) h->table = (params->bw + 2 + 63) & ~63;
     if (const char *e = getenv("TAL_CHAIN_TABLE"))
         if (atoi(e) == 0) h->table = 0;
@@ -235,8 +235,6 @@ int tal_chain_create(const tal_chain_params *params, int device, tal_chain **out
             h->wpb >>= 1;
         while (chain_lds_bytes(h->ring, h->table, h->wpb) > h->lds_per_block && h->ring > 128)
             h->ring >>= 1;
-        const int fit = (int)(h->lds_per_cu / chain_lds_bytes(h->ring, h->table, 1));
-        h->waves_per_cu = std::max(h->wpb, std::min(h->waves_per_cu, fit) / h->wpb * h->wpb);
         if (set_chain_lds_limit(h->lds_per_block) != hipSuccess) { rc = TAL_EDEVICE; break; }
         if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
             rc = TAL_EDEVICE;
