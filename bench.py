@@ -232,9 +232,12 @@ def _sw_cpu(bits, pairs, buf, m, min_s=10.0):
             "sample": f"{passes} passes over the first {m} pairs, {dt:.1f}s"}
 
 
-# VALU lane-ops per predecessor examined: differences and |dd| 5, the five
-# window tests and their conjunction 9, overlap 2, ilog2 2, float gap 4, the
-# score 3, the (score, j) key and its max 5
+# Algorithmic lane-ops per candidate predecessor, those of the reference's
+# vector loop (parallel_chaining_32_bit.h:262-310): ddr, ddq, ddr - ddq and
+# its absolute value 5, the five window tests and their conjunction 9,
+# min(ddr, ddq, l) 2, ilog2 2, the float gap 4, the score 3, the running
+# maximum and its index 5.  (rocprofv3 counts 32 VALU lane-ops per
+# predecessor in the whole kernel, per-anchor work included.)
 CHAIN_OPS = 30
 
 
@@ -299,7 +302,8 @@ def _chain(args):
         "parity": f"bit-exact on the first {m} sets",
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),
                      "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
-                     "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
+                     "frac": round(achieved / VALU_PEAK_TOPS, 4),
+                     "traffic": (_traffic("chain_main_chain") or {}).get("traffic_bytes"),
                      "kernel": "chain_main", "kernel_ms": round(k_ms, 3),
                      "ops_per_predecessor": CHAIN_OPS, "lds_ring": st["lds_ring"],
                      "far_anchors": st["far_anchors"]},
@@ -343,10 +347,10 @@ def _chain_cpu(a, off, min_s=10.0):
             "kind": kind, "sample": f"{passes} passes over all {nsets} sets, {dt:.1f}s"}
 
 
-def _traffic():
-    """HBM bytes per launch of the 16-bit kernel from the committed rocprofv3
-    FETCH_SIZE/WRITE_SIZE summary of the default workload, if present."""
-    f = os.path.join(ROOT, "profiles", "bsw_main16_workload2.json")
+def _traffic(name="bsw_main16_workload2"):
+    """HBM bytes per launch of a kernel from its committed rocprofv3
+    FETCH_SIZE/WRITE_SIZE summary (profiles/<name>.json), if present."""
+    f = os.path.join(ROOT, "profiles", name + ".json")
     try:
         with open(f) as fh:
             return json.load(fh)

@@ -77,6 +77,15 @@ def build_sw_cli(out=None):
     return out
 
 
+def build_chain_cli(out=None):
+    """tal-bench-dp-chaining: bench-dp-chaining's input and output on the engine."""
+    out = out or os.path.join(HERE, "tal-bench-dp-chaining")
+    _run(["g++", "-O2", "-std=c++17", f"-I{os.path.join(ROOT, 'include')}",
+          os.path.join(CSRC, "tal_bench_dp_chaining.cpp"), "-o", out, f"-L{HERE}", "-ltal_chain",
+          "-Wl,-rpath,$ORIGIN"])
+    return out
+
+
 def build_sw(out=None):
     """libtal_sw.so: the full inter-task SW engine (include/tal_sw.h)."""
     return build_hip(out or os.path.join(HERE, "libtal_sw.so"), sources=SW_SOURCES)
@@ -94,6 +103,7 @@ def build_all():
     build_chain()
     build_cli()
     build_sw_cli()
+    build_chain_cli()
 
 
 if __name__ == "__main__":

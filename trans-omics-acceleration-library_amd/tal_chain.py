@@ -34,6 +34,42 @@ class Stats(ctypes.Structure):
                 ("predecessors", ctypes.c_int64), ("far_anchors", ctypes.c_int64)]
 
 
+CLI_PATH = os.path.join(HERE, "tal-bench-dp-chaining")
+
+
+def write_anchor_file(path, anchors, offsets):
+    """bench-dp-chaining's anchors file (bench-dp-chaining.cpp:74-82): per set
+    its anchor count, then one "r q l" line per anchor."""
+    anchors = np.asarray(anchors, ANCHOR_DTYPE)
+    with open(path, "w") as fh:
+        for k in range(len(offsets) - 1):
+            b, e = int(offsets[k]), int(offsets[k + 1])
+            fh.write(f"{e - b}\n")
+            fh.writelines(f"{int(x['r'])} {int(x['q'])} {int(x['l'])}\n" for x in anchors[b:e])
+
+
+def read_anchor_file(path):
+    """(anchors, offsets) of a bench-dp-chaining anchors file."""
+    toks = open(path).read().split()
+    parts, offs, i = [], [0], 0
+    while i < len(toks):
+        m = int(toks[i])
+        i += 1
+        if m < 0 or i + 3 * m > len(toks):
+            raise ValueError(f"{path}: bad or truncated set {len(offs) - 1}")
+        a = np.zeros(m, ANCHOR_DTYPE)
+        vals = np.array(toks[i:i + 3 * m], dtype=object).reshape(m, 3) if m else None
+        if m:
+            a["r"] = [int(x) for x in vals[:, 0]]
+            a["q"] = [int(x) for x in vals[:, 1]]
+            a["l"] = [int(x) for x in vals[:, 2]]
+        parts.append(a)
+        offs.append(offs[-1] + m)
+        i += 3 * m
+    anchors = np.concatenate(parts) if parts else np.zeros(0, ANCHOR_DTYPE)
+    return anchors, np.array(offs, np.int64)
+
+
 class TalError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{_STATUS.get(code, code)}: {msg}")
