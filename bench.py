@@ -39,6 +39,9 @@ import tal_bsw  # noqa: E402
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12
 OPS_PER_CELL = 14  # S select 2, ambiguity 1, M 2, H 1, E 3, F 3, row max 2
 PARAMS = dict(o_del=6, e_del=1, o_ins=6, e_ins=1, zdrop=100, end_bonus=5, w_match=1, w_mismatch=4)
+# parity is checked against the oracle in the CPU leg only; without it the
+# GPU parity tests (tests/) are the check
+NO_CHECK = "not checked by this run (--no-cpu); tests/ -m gpu check it"
 
 
 WORKLOADS = {
@@ -89,6 +92,9 @@ def main():
     ap.add_argument("--chain", action="store_true",
                     help="measure the DP chaining engine (dp_chain::mm_dp_vectorized drop-in) "
                          "on its synthetic anchor sets instead")
+    ap.add_argument("--extend", action="store_true",
+                    help="measure the end-to-end extension pipeline (anchors -> chaining -> "
+                         "extension pairs -> banded SW of both sides) on simulated reads instead")
     ap.add_argument("--sw-weights", default="1,-1,1,1", metavar="M,X,O,E",
                     help="full-SW weights (match, mismatch, gap open, gap extend as the reference "
                          "adds them); default: bench-smith-waterman's")
@@ -112,6 +118,9 @@ def main():
     if args.chain:
         _chain(args)
         return
+    if args.extend:
+        _extend(args)
+        return
 
     _batch(args)
 
@@ -133,7 +142,6 @@ def _sw(args):
     the batch / step time (bench-smith-waterman.cpp:150,246)."""
     import torch
     import tal_sw
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
     global SW_PARAMS
     SW_PARAMS = tuple(int(x) for x in args.sw_weights.split(","))
     bits, n_def, lo, hi = synth.SW_WORKLOADS[args.sw]
@@ -154,15 +162,7 @@ def _sw(args):
 
     step()
     torch.cuda.synchronize()
-    got = d_pairs.cpu().numpy().view(tal_sw.PAIR_DTYPE)
-    g = tal_sw.GROUP[bits]
-    m = min(n, max(g, (args.check_pairs // g) * g))
-    import oracle as O
-    want = O.sw_scores(bits, pairs[:m].copy(), buf, SW_PARAMS, nthreads=16)["score"]
-    bad = int((got["score"][:m] != want).sum())
-    if bad:
-        raise SystemExit(f"[sw] parity FAILED on {bad} of the first {m} pairs")
-    log(f"[sw] parity: first {m} pairs bit-exact against the oracle")
+    first = d_pairs.cpu().numpy().view(tal_sw.PAIR_DTYPE).copy()  # checked by the CPU leg
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -186,7 +186,7 @@ def _sw(args):
         "data": f"synthetic (seeded, {args.sw})",
         "config": {"workload": args.sw, "pairs": n, "len": f"U[{lo},{hi}]", "class": bits,
                    "weights": SW_PARAMS, "cells": cells},
-        "pairs_per_s": round(n / dt, 1), "parity": f"bit-exact on the first {m} pairs",
+        "pairs_per_s": round(n / dt, 1), "parity": NO_CHECK,
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": round(peak, 2),
                      "unit": "TOP/s", "frac": round(achieved / peak, 4), "traffic": None,
                      "kernel": f"sw_main<{bits}>", "kernel_ms": round(k_ms, 3),
@@ -196,6 +196,18 @@ def _sw(args):
                      "regrouped": bool(st["regrouped"])},
     }
     if not args.no_cpu:
+        # the CPU leg: the oracle checks the first call, then the reference
+        # build (or the oracle) is timed
+        g = tal_sw.GROUP[bits]
+        m = min(n, max(g, (args.check_pairs // g) * g))
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        want = O.sw_scores(bits, pairs[:m].copy(), buf, SW_PARAMS, nthreads=16)["score"]
+        bad = int((first["score"][:m] != want).sum())
+        if bad:
+            raise SystemExit(f"[sw] parity FAILED on {bad} of the first {m} pairs")
+        log(f"[sw] parity: first {m} pairs bit-exact against the oracle")
+        line["parity"] = f"bit-exact on the first {m} pairs"
         line["cpu_baseline"] = _sw_cpu(bits, pairs, buf, n if bits != 8 else 200_000)
     print(json.dumps(line), flush=True)
 
@@ -246,8 +258,6 @@ def _chain(args):
     step chains the whole synthetic batch from HBM; value = anchors / s."""
     import torch
     import tal_chain
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
     nsets, lo, hi = synth.CHAIN_WORKLOADS["chain"]
     t = time.time()
     a, off = synth.chain_generate(nsets, lo, hi)
@@ -268,14 +278,8 @@ def _chain(args):
 
     step()
     torch.cuda.synchronize()
-    m = min(nsets, 200)
-    want = O.chain_dp_sets(a[:off[m]], off[:m + 1], nthreads=16)
-    got = (d_f.cpu().numpy().view(np.uint32)[:off[m]], d_p.cpu().numpy()[:off[m]],
-           d_v.cpu().numpy()[:off[m]])
-    bad = sum(int((x != y).sum()) for x, y in zip(got, want))
-    if bad:
-        raise SystemExit(f"[chain] parity FAILED: {bad} outputs differ in the first {m} sets")
-    log(f"[chain] parity: first {m} sets bit-exact against the oracle")
+    first = (d_f.cpu().numpy().view(np.uint32).copy(), d_p.cpu().numpy().copy(),
+             d_v.cpu().numpy().copy())  # checked by the CPU leg
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -299,7 +303,7 @@ def _chain(args):
         "config": {"workload": "chain", "sets": nsets, "anchors_per_set": f"U[{lo},{hi}]",
                    "anchors": n, "bw": 500, "max_iter": 5000, "dr": 5000, "dq": 5000},
         "predecessors_per_s": round(pred / dt, 1),
-        "parity": f"bit-exact on the first {m} sets",
+        "parity": NO_CHECK,
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),
                      "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
                      "frac": round(achieved / VALU_PEAK_TOPS, 4),
@@ -309,6 +313,17 @@ def _chain(args):
                      "far_anchors": st["far_anchors"]},
     }
     if not args.no_cpu:
+        # the CPU leg: the oracle checks the first call, then the reference
+        # build is timed
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O
+        m = min(nsets, 200)
+        want = O.chain_dp_sets(a[:off[m]], off[:m + 1], nthreads=16)
+        bad = sum(int((x[:off[m]] != y).sum()) for x, y in zip(first, want))
+        if bad:
+            raise SystemExit(f"[chain] parity FAILED: {bad} outputs differ in the first {m} sets")
+        log(f"[chain] parity: first {m} sets bit-exact against the oracle")
+        line["parity"] = f"bit-exact on the first {m} sets"
         line["cpu_baseline"] = _chain_cpu(a, off)
     print(json.dumps(line), flush=True)
 
@@ -345,6 +360,142 @@ def _chain_cpu(a, off, min_s=10.0):
                 break
     return {"value": round(len(a) * passes / dt, 1), "unit": "anchors/s", "cores": 16,
             "kind": kind, "sample": f"{passes} passes over all {nsets} sets, {dt:.1f}s"}
+
+
+def _extend(args):
+    """The extension pipeline (include/tal_extend.h) end to end on simulated
+    reads: one step is one tal_extend_run_device call over every read, inputs
+    in HBM; value = reads / s."""
+    import torch
+    import tal_extend
+    nreads, rlen, glen = synth.EXTEND_WORKLOADS["extend"]
+    t = time.time()
+    d = synth.extend_generate(nreads, rlen, glen)
+    na = len(d["anchors"])
+    log(f"[extend] {nreads} reads of {rlen} from a {glen}-base genome, {na} anchors, "
+        f"in {time.time() - t:.1f}s")
+    dev = torch.device("cuda:0")
+    d_g = torch.from_numpy(d["genome"]).to(dev)
+    d_r = torch.from_numpy(d["reads"]).to(dev)
+    d_ro = torch.from_numpy(d["read_off"]).to(dev)
+    d_a = torch.from_numpy(d["anchors"].view(np.uint8)).to(dev)
+    d_ao = torch.from_numpy(d["anchor_off"]).to(dev)
+    d_out = torch.zeros(nreads * tal_extend.READ_DTYPE.itemsize, dtype=torch.uint8, device=dev)
+    eng = tal_extend.Extender()
+    stream = torch.cuda.current_stream(dev).cuda_stream
+
+    def step():
+        eng.run_device(d_g.data_ptr(), glen, d_r.data_ptr(), d_ro.data_ptr(), nreads,
+                       d_a.data_ptr(), d_ao.data_ptr(), d_out.data_ptr(), stream=stream)
+
+    # the parity check runs in the CPU leg, on its own batch (below)
+    step()
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stages = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        stages.append(eng.stats())
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    st = stages[-1]
+    mean = {k: float(np.mean([s[k] for s in stages])) for k in
+            ("ms_chain", "ms_produce", "ms_left", "ms_right", "ms_total")}
+    left, right, _, _ = eng.pairs()
+    cells = float(tal_bsw.band_cells(left["len1"], left["len2"], 100, **PARAMS).sum() +
+                  tal_bsw.band_cells(right["len1"], right["len2"], 100, **PARAMS).sum())
+    ms_ext = mean["ms_left"] + mean["ms_right"]
+    # the extension passes dominate; their roofline as the BSW line's
+    achieved = 14 * cells / (ms_ext / 1e3) / 1e12
+    line = {
+        "metric": "reads/s of the end-to-end extension pipeline (chaining, extension pairs, "
+                  "banded SW of both sides), bit-exact",
+        "value": round(nreads / dt, 1), "unit": "reads/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int16/int8 (SW), int32 (chaining)",
+        "data": "synthetic (reads simulated from a random genome: 1% substitutions, 0.1% indels; "
+                "19-mer anchors plus 6 spurious a read)",
+        "config": {"workload": "extend", "reads": nreads, "read_len": rlen, "genome": glen,
+                   "anchors": na, "left_pairs": st["left_pairs"], "right_pairs": st["right_pairs"],
+                   "w": 100, "pad": 100},
+        "stages_ms": {k[3:]: round(v, 3) for k, v in mean.items()},
+        "parity": NO_CHECK,
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3),
+                     "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s",
+                     "frac": round(achieved / VALU_PEAK_TOPS, 4), "traffic": None,
+                     "kernel": "bsw_main (left + right passes)", "kernel_ms": round(ms_ext, 3),
+                     "band_cells": cells, "ops_per_cell": 14},
+    }
+    if not args.no_cpu:
+        # the CPU leg: the oracles check the first reads, run as their own
+        # batch on a fresh handle (a batch's extension scores depend on its
+        # group composition and on the handle's state, as the reference's
+        # do, so a slice of the big batch is not compared), then the CPU
+        # stages are timed
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import extend_model
+        m = 2000
+        sub = dict(genome=d["genome"], reads=d["reads"][:d["read_off"][m]],
+                   read_off=d["read_off"][:m + 1], anchors=d["anchors"][:d["anchor_off"][m]],
+                   anchor_off=d["anchor_off"][:m + 1])
+        chk = tal_extend.Extender()
+        got = chk.run(**sub)
+        chk.close()
+        want, _, _ = extend_model.pipeline(**sub)
+        bad = sum(int((got[f] != want[f]).sum()) for f in tal_extend.READ_DTYPE.names)
+        if bad:
+            raise SystemExit(f"[extend] parity FAILED: {bad} fields differ on {m} reads")
+        log(f"[extend] parity: a {m}-read batch bit-exact against the oracles")
+        line["parity"] = f"bit-exact on a {m}-read batch (the first reads)"
+        line["cpu_baseline"] = _extend_cpu(d, left, right, eng)
+    print(json.dumps(line), flush=True)
+
+
+def _extend_cpu(d, left, right, eng, min_s=10.0):
+    """The CPU stages on a bounded sample of the reads, 16 threads: the
+    reference's own mm_dp_vectorized (oracle/_ref build) over the reads'
+    anchor sets, then the BSW oracle (a port of the reference's AVX-512
+    path, which cannot be built here) on those reads' left and right pairs.
+    The producer's own CPU time is left out."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from concurrent.futures import ThreadPoolExecutor
+    nreads = len(d["read_off"]) - 1
+    m = 20000
+    off = d["anchor_off"][:m + 1]
+    a = d["anchors"][:off[-1]]
+    chunks = [(k, min(m, k + 500)) for k in range(0, m, 500)]
+    ref_ok = O.ref_chain_available() and O.ref_chain_dp_sets(a[:off[1]], off[:2]) is not None
+
+    def chain_part(c):
+        s0, s1 = c
+        sub, so = a[off[s0]:off[s1]], off[s0:s1 + 1] - off[s0]
+        if ref_ok:
+            O.ref_chain_dp_sets(sub, so)
+        else:
+            O.chain_dp_sets(sub, so)
+
+    lsel = left[left["seqid"] < m]
+    rsel = right[right["seqid"] < m]
+    _, _, ref, qer = eng.pairs()
+    passes, t = 0, time.perf_counter()
+    while True:
+        with ThreadPoolExecutor(16) as ex:
+            list(ex.map(chain_part, chunks))
+        O.run_mt(lsel.copy(), ref, qer, 100, bits=16, nthreads=16)
+        O.run_mt(rsel.copy(), ref, qer, 100, bits=16, nthreads=16)
+        passes += 1
+        dt = time.perf_counter() - t
+        if dt >= min_s:
+            break
+    return {"value": round(m * passes / dt, 1), "unit": "reads/s", "cores": 16, "kind": "port",
+            "sample": f"{passes} passes over the first {m} of {nreads} reads, {dt:.1f}s: "
+                      f"chaining by the reference's build ({'yes' if ref_ok else 'no: oracle'}), "
+                      "the two extension passes by the BSW oracle (16-bit), producer not timed"}
 
 
 def _traffic(name="bsw_main16_workload2"):
@@ -415,21 +566,12 @@ def _batch(args):
             else:
                 c["plan"].launch(stream.cuda_stream)
 
-    # the first call on each fresh handle doubles as the parity check
+    # the first call on each fresh handle: its results are kept for the
+    # parity check of the CPU leg
     step()
     torch.cuda.synchronize()
-    bad, checked = 0, 0
-    if args.check_pairs > 0:
-        per = max(1, args.check_pairs // len(classes))
-        for c in classes:
-            got = c["d_pairs"].cpu().numpy().view(tal_bsw.SEQPAIR_DTYPE)
-            b, k = _check_prefix(c["pairs"], ref, qer, got, c["is8"], c["w"], per)
-            bad += b
-            checked += k
-        check = "bit-exact" if bad == 0 else f"{bad} mismatching fields"
-        log(f"[bench] parity vs oracle on {checked} pairs (dispatcher prefixes): {check}")
-    else:
-        check = "skipped"
+    firsts = [c["d_pairs"].cpu().numpy().view(tal_bsw.SEQPAIR_DTYPE).copy() for c in classes]
+    check = NO_CHECK
     for _ in range(max(0, args.warmup - 1)):
         step()
     torch.cuda.synchronize()
@@ -498,6 +640,17 @@ def _batch(args):
 
     cpu = None
     if not args.no_cpu:
+        # the CPU leg: the oracle checks the first calls' results, then is
+        # timed as the CPU baseline
+        if args.check_pairs > 0:
+            bad, checked = 0, 0
+            per = max(1, args.check_pairs // len(classes))
+            for c, got in zip(classes, firsts):
+                b, k = _check_prefix(c["pairs"], ref, qer, got, c["is8"], c["w"], per)
+                bad += b
+                checked += k
+            check = "bit-exact" if bad == 0 else f"{bad} mismatching fields"
+            log(f"[bench] parity vs oracle on {checked} pairs (dispatcher prefixes): {check}")
         c = classes[0]
         cpu = _cpu_baseline(c["pairs"], ref, qer, c["cells"], c["is8"], c["w"], args.cpu_pairs)
         log(f"[bench] cpu baseline: {cpu}")

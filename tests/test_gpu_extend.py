@@ -1,0 +1,103 @@
+"""GPU parity of the extension pipeline (libtal_extend.so) through the C ABI:
+the producer's pairs and bytes equal the model's (tests/extend_model.py),
+and every extension score equals the BSW oracle's on the same pairs in the
+same order, the right pairs' h0 coming from the left scores."""
+import numpy as np
+import pytest
+
+import extend_model as M
+import synth as S
+import tal_bsw as TB
+import tal_chain as TC
+import tal_extend as TE
+
+pytestmark = pytest.mark.gpu
+
+IN_FIELDS = ["idr", "idq", "id", "len1", "len2", "h0", "seqid", "regid"]
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _check(eng, d, label):
+    got = eng.run(d["genome"], d["reads"], d["read_off"], d["anchors"], d["anchor_off"])
+    want, wl, wr = M.pipeline(d["genome"], d["reads"], d["read_off"], d["anchors"],
+                              d["anchor_off"])
+    for fld in TE.READ_DTYPE.names:
+        bad = np.where(got[fld] != want[fld])[0]
+        assert len(bad) == 0, f"{label}: read field {fld} differs at {bad[:5]}"
+    gl, gr, gref, gqer = eng.pairs()
+    assert len(gl) == len(wl) and len(gr) == len(wr), label
+    for fld in IN_FIELDS + TB.OUT_FIELDS:
+        assert np.array_equal(gl[fld], wl[fld]), f"{label}: left {fld}"
+        assert np.array_equal(gr[fld], wr[fld]), f"{label}: right {fld}"
+    st = eng.stats()
+    assert st["left_pairs"] == len(wl) and st["right_pairs"] == len(wr)
+    return got, gref, gqer
+
+
+def test_pipeline_on_simulated_reads():
+    _say("[gpu] extend pipeline 3000 reads")
+    d = S.extend_generate(3000, 150, 2_000_000, seed=21)
+    eng = TE.Extender()
+    got, ref, qer = _check(eng, d, "sim")
+    assert (got["score"] >= got["left_score"]).mean() > 0.99
+    eng.close()
+
+
+def test_pipeline_edges():
+    _say("[gpu] extend edges")
+    rng = np.random.default_rng(4)
+    L = 5000
+    genome = rng.integers(0, 4, L, dtype=np.uint8)
+    reads, anchors, aoff, roff = [], [], [0], [0]
+
+    def add(read, anc):
+        reads.append(np.asarray(read, np.uint8))
+        roff.append(roff[-1] + len(read))
+        a = np.zeros(len(anc), TC.ANCHOR_DTYPE)
+        for i, (r, q, ln) in enumerate(anc):
+            a[i] = (r, q, ln)
+        anchors.append(a)
+        aoff.append(aoff[-1] + len(anc))
+
+    add(genome[100:250], [])                                # no anchors: no chain
+    add(genome[300:450], [(300, 0, 150)])                   # the whole read: no sides
+    add(genome[0:150], [(20, 20, 40), (60, 60, 40)])        # left side clipped by the genome start
+    add(genome[L - 150:L], [(L - 100, 50, 30)])             # right side clipped by the genome end
+    add(genome[1000:1120], [(1000, 0, 19), (1010, 10, 19), (1050, 50, 19), (1100, 100, 20)])
+    noisy = genome[2000:2150].copy()
+    noisy[70] = (noisy[70] + 1) % 4
+    add(noisy, [(2000, 0, 19), (2020, 20, 19), (2040, 40, 19), (2080, 80, 19), (2100, 100, 19),
+                (4000, 5, 19)])
+    d = dict(genome=genome, reads=np.concatenate(reads), read_off=np.array(roff, np.int64),
+             anchors=np.concatenate(anchors), anchor_off=np.array(aoff, np.int64))
+    eng = TE.Extender()
+    got, _, _ = _check(eng, d, "edges")
+    assert got["chain_anchors"][0] == 0 and got["left_pair"][0] == -1
+    assert got["left_pair"][1] == -1 and got["right_pair"][1] == -1
+    assert got["score"][1] == 150
+    eng.close()
+
+
+def test_pipeline_errors():
+    _say("[gpu] extend errors")
+    rng = np.random.default_rng(5)
+    genome = rng.integers(0, 4, 1000, dtype=np.uint8)
+    eng = TE.Extender()
+    a = np.zeros(1, TC.ANCHOR_DTYPE)
+    a[0] = (10, 140, 20)  # q + l past the 150-base read
+    with pytest.raises(TE.TalError) as e:
+        eng.run(genome, genome[:150], np.array([0, 150]), a, np.array([0, 1]))
+    assert e.value.code == TE.TAL_EINVAL
+    long_read = rng.integers(0, 4, 40000, dtype=np.uint8)
+    a[0] = (10, 0, 19)
+    with pytest.raises(TE.TalError) as e:
+        eng.run(np.concatenate([genome, long_read]), long_read, np.array([0, 40000]), a,
+                np.array([0, 1]))
+    assert e.value.code == TE.TAL_ERANGE
+    # the handle still works afterwards
+    d = S.extend_generate(50, 150, 100_000, seed=8)
+    _check(eng, d, "after errors")
+    eng.close()

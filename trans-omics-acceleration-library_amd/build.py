@@ -25,6 +25,7 @@ SOURCE_FLAGS = {"sw_kernels.hip": ["-mllvm", "-amdgpu-sched-strategy=iterative-i
                 "chain_kernels.hip": ["-fhip-fp32-correctly-rounded-divide-sqrt",
                                       "-ffp-contract=off"]}
 CHAIN_SOURCES = ["chain_kernels.hip", "tal_chain.hip"]
+EXTEND_SOURCES = ["extend_kernels.hip", "tal_extend.hip"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -38,7 +39,7 @@ def _run(cmd):
     subprocess.check_call(cmd)
 
 
-def build_hip(out=None, extra=(), sources=None):
+def build_hip(out=None, extra=(), sources=None, libs=()):
     out = out or os.path.join(HERE, "libtal_bsw.so")
     objs = []
     for src in sources or HIP_SOURCES:
@@ -46,7 +47,8 @@ def build_hip(out=None, extra=(), sources=None):
         _run([HIPCC, *HIP_FLAGS, *SOURCE_FLAGS.get(src, []), *extra, "-c",
               os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", out, "-lpthread"])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", out, "-lpthread",
+          *libs])
     for o in objs:
         os.remove(o)
     return out
@@ -96,11 +98,19 @@ def build_chain(out=None):
     return build_hip(out or os.path.join(HERE, "libtal_chain.so"), sources=CHAIN_SOURCES)
 
 
+def build_extend(out=None):
+    """libtal_extend.so: the extension-pair producer and pipeline
+    (include/tal_extend.h), on libtal_chain.so and libtal_bsw.so."""
+    return build_hip(out or os.path.join(HERE, "libtal_extend.so"), sources=EXTEND_SOURCES,
+                     libs=(f"-L{HERE}", "-ltal_chain", "-ltal_bsw", "-Wl,-rpath,$ORIGIN"))
+
+
 def build_all():
     build_synth()
     build_hip()
     build_sw()
     build_chain()
+    build_extend()
     build_cli()
     build_sw_cli()
     build_chain_cli()

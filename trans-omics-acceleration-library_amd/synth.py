@@ -159,3 +159,69 @@ def chain_generate(nsets, lo, hi, seed=11, noise=0.3, kmer=(15, 28)):
         anchors["q"][a:b] = np.maximum(q[order], 0)
         anchors["l"][a:b] = rng.integers(kmer[0], kmer[1] + 1, m)
     return anchors, offsets
+
+
+# Reads for the extension pipeline (extension-pair producer): reads sampled
+# from a random genome with substitutions and short indels; each read's
+# anchors are its k-mers that match the genome along the read's true
+# alignment (exact runs of k), plus a few random spurious ones, sorted by r.
+EXTEND_WORKLOADS = {
+    # reads, read length, genome length
+    "extend": (200_000, 150, 50_000_000),
+}
+
+
+def extend_generate(nreads, read_len, genome_len, seed=13, sub=0.01, indel=0.001, k=19,
+                    spurious=6):
+    """dict(genome uint8 codes 0-3, reads uint8 codes, read_off int64[nreads+1],
+    anchors ANCHOR_DTYPE, anchor_off int64[nreads+1], origin int64[nreads])."""
+    rng = np.random.default_rng(seed)
+    genome = rng.integers(0, 4, genome_len, dtype=np.uint8)
+    span = read_len + read_len // 4
+    origin = rng.integers(0, genome_len - span, nreads)
+    # edit process per read base: 0 match/sub, 1 insertion, 2 deletion (skip
+    # a genome base, then take the next)
+    ops = rng.choice(3, (nreads, read_len), p=[1 - 2 * indel, indel, indel]).astype(np.int8)
+    is_ins = ops == 1
+    gstep = np.where(is_ins, 0, np.where(ops == 2, 2, 1))
+    gpos = origin[:, None] + np.cumsum(gstep, axis=1) - 1
+    gpos = np.where(is_ins, -1, gpos)
+    gidx = np.clip(gpos, 0, genome_len - 1)
+    reads = np.where(is_ins, rng.integers(0, 4, (nreads, read_len)), genome[gidx]).astype(np.uint8)
+    subs = (rng.random((nreads, read_len)) < sub) & ~is_ins
+    reads[subs] = (reads[subs] + rng.integers(1, 4, int(subs.sum()))).astype(np.uint8) % 4
+    # a read base matches when it is not inserted, not substituted; a k-mer
+    # at i is an anchor when bases i..i+k-1 match on consecutive genome bases
+    ok = ~is_ins & ~subs
+    consec = np.ones((nreads, read_len), bool)
+    consec[:, 1:] = gpos[:, 1:] == gpos[:, :-1] + 1
+    good = ok & consec
+    good[:, 0] = ok[:, 0]
+    # run length of good bases ending at each position (a base starts a new
+    # run when it is not consecutive with the previous one)
+    run = np.zeros((nreads, read_len), np.int32)
+    run[:, 0] = ok[:, 0]
+    for i in range(1, read_len):
+        run[:, i] = np.where(ok[:, i], np.where(consec[:, i], run[:, i - 1] + 1, 1), 0)
+    end_ok = run[:, k - 1:] >= k  # k-mer ending at i + k - 1 starting at i
+    rr, ii = np.nonzero(end_ok)
+    a_r = gpos[rr, ii].astype(np.uint64)
+    a_q = ii.astype(np.int32)
+    # spurious anchors: random positions near and far
+    sr = np.repeat(np.arange(nreads), spurious)
+    s_r = rng.integers(0, genome_len - k, nreads * spurious).astype(np.uint64)
+    s_q = rng.integers(0, read_len - k, nreads * spurious).astype(np.int32)
+    read_id = np.concatenate([rr, sr])
+    r_all = np.concatenate([a_r, s_r])
+    q_all = np.concatenate([a_q, s_q])
+    order = np.lexsort((q_all, r_all, read_id))
+    anchors = np.zeros(len(order), ANCHOR_DTYPE)
+    anchors["r"] = r_all[order]
+    anchors["q"] = q_all[order]
+    anchors["l"] = k
+    counts = np.bincount(read_id, minlength=nreads)
+    anchor_off = np.zeros(nreads + 1, np.int64)
+    anchor_off[1:] = np.cumsum(counts)
+    read_off = np.arange(nreads + 1, dtype=np.int64) * read_len
+    return dict(genome=genome, reads=reads.reshape(-1), read_off=read_off, anchors=anchors,
+                anchor_off=anchor_off, origin=origin.astype(np.int64))
