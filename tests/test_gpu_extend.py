@@ -71,10 +71,13 @@ def test_pipeline_edges():
     noisy[70] = (noisy[70] + 1) % 4
     add(noisy, [(2000, 0, 19), (2020, 20, 19), (2040, 40, 19), (2080, 80, 19), (2100, 100, 19),
                 (4000, 5, 19)])
+    # a long read with more anchors than the LDS stage holds (the walk in HBM)
+    add(genome[2500:4500], [(2500 + q, q, 19) for q in range(0, 1980, 2)])
     d = dict(genome=genome, reads=np.concatenate(reads), read_off=np.array(roff, np.int64),
              anchors=np.concatenate(anchors), anchor_off=np.array(aoff, np.int64))
     eng = TE.Extender()
     got, _, _ = _check(eng, d, "edges")
+    assert got["chain_anchors"][-1] > 512
     assert got["chain_anchors"][0] == 0 and got["left_pair"][0] == -1
     assert got["left_pair"][1] == -1 and got["right_pair"][1] == -1
     assert got["score"][1] == 150

@@ -325,9 +325,10 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
     }
 
     int beg = 0, end = ncol, nbeg = 0, nend = ncol;
-    /* previous row's column range and this lane's store band in it; before
-     * row 0 no column has been stored (pend = -1) */
-    int pbeg = 0, pend = -1, phd = 0, ptl = -1;
+    /* end of the previous row's column range [pbeg, pend] (its start pbeg is
+     * at most this row's beg, so only pend is kept) and this lane's store
+     * band in it; before row 0 no column has been stored (pend = -1) */
+    int pend = -1, phd = 0, ptl = -1;
     /* stale stretch [st_lo, st_hi) below the range start and its first/last
      * nonzero column for this lane */
     int st_lo = -1, st_hi = -1, st_first = NONE_COL, st_last = -1;
@@ -635,7 +636,6 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
             fz = fz == NONE_COL ? end : fz;
             lz = (r != 0u && j >= beg) ? j : (lz >= beg ? lz : beg - 1);
         }
-        pbeg = beg;
         pend = j;
         phd = hd;
         ptl = tl;

@@ -46,8 +46,7 @@ struct ChainBuffers {
 };
 
 hipError_t launch_chain_check(const tal_chain_anchor *a, const int64_t *off, int32_t nsets,
-                              uint32_t *keys, int32_t *ids, int32_t *ctl,
-                              unsigned long long *maxn, hipStream_t s);
+                              uint32_t *keys, int32_t *ids, int32_t *ctl, hipStream_t s);
 hipError_t set_chain_lds_limit(size_t bytes);
 /* LDS bytes of one block of the DP kernel */
 size_t chain_lds_bytes(int ring, int table, int waves_per_block);

@@ -64,10 +64,10 @@ __device__ __forceinline__ int32_t ilog2_u32(uint32_t x) { return 31 - __clz((in
 
 /* Validation and sort keys, one wave per set: offsets must be non-negative
  * and non-decreasing, a set at most INT32_MAX anchors, every seed length
- * >= 0.  Reduces the largest set. */
+ * >= 0.  (No atomic per set: with 10^5 sets one contended word would
+ * serialize them.) */
 __global__ void chain_check(const tal_chain_anchor *a, const int64_t *off, int32_t nsets,
-                            uint32_t *keys, int32_t *ids, int32_t *ctl,
-                            unsigned long long *maxn)
+                            uint32_t *keys, int32_t *ids, int32_t *ctl)
 {
     const int k = blockIdx.x;
     const int lane = threadIdx.x;
@@ -85,7 +85,6 @@ __global__ void chain_check(const tal_chain_anchor *a, const int64_t *off, int32
     const int64_t n = err ? 0 : e - b;
     keys[k] = (uint32_t)min(n, (int64_t)0xFFFFFFFF);
     ids[k] = k;
-    atomicMax(maxn, (unsigned long long)n);
 }
 
 /* x86 converts an out-of-range float or double to INT32_MIN, and the
@@ -558,12 +557,10 @@ __global__ void __launch_bounds__(64 * WPB) chain_main(ChainParams P, ChainBuffe
 }
 
 hipError_t launch_chain_check(const tal_chain_anchor *a, const int64_t *off, int32_t nsets,
-                              uint32_t *keys, int32_t *ids, int32_t *ctl,
-                              unsigned long long *maxn, hipStream_t s)
+                              uint32_t *keys, int32_t *ids, int32_t *ctl, hipStream_t s)
 {
     if (nsets <= 0) return hipSuccess;
-    hipLaunchKernelGGL(chain_check, dim3(nsets), dim3(64), 0, s, a, off, nsets, keys, ids, ctl,
-                       maxn);
+    hipLaunchKernelGGL(chain_check, dim3(nsets), dim3(64), 0, s, a, off, nsets, keys, ids, ctl);
     return hipGetLastError();
 }
 

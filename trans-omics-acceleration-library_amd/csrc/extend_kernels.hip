@@ -1,11 +1,15 @@
 /*
  * extend_kernels.hip -- gfx950 kernels of the extension-pair producer
  * (include/tal_extend.h): from each read's chaining result to its left and
- * right extension pairs.  Byte-moving and pointer-chasing work: one wave per
- * read, lanes over the read's anchors or bytes.
+ * right extension pairs.  Pointer-chasing work one lane per read, byte moving
+ * one wave per read with lanes over its bytes.
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include <stdlib.h>
+
+#include <algorithm>
 
 #include "extend_internal.h"
 
@@ -32,105 +36,168 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t x)
 
 } // namespace
 
+/* anchors of a walk's register window */
+constexpr int PICK_WIN = 8;
+
+/* The walk of read k from its chain end (end_key = f << 32 | ~i of it). */
+__device__ __forceinline__ void pick_read(const ExtParams &P, const ExtBuffers &B, int k,
+                                          uint64_t end_key)
+{
+    const int64_t a0 = B.aoff[k];
+    const int32_t n = (int32_t)(B.aoff[k + 1] - a0);
+    const int64_t m64 = B.read_off[k + 1] - B.read_off[k];
+    tal_extend_read o;
+    o.seed_r = 0;
+    o.seed_q = o.seed_len = o.chain_score = o.chain_anchors = 0;
+    o.left_pair = o.right_pair = -1;
+    o.left_score = o.score = 0;
+    int32_t lq = 0, lt = 0, rq = 0, rt = 0;
+    if (m64 < 0 || m64 > 32767) {
+        atomicMax(B.err, EXT_ERR_READ);
+    } else if (n > 0) {
+        const int32_t m = (int32_t)m64;
+        const uint32_t fmax = (uint32_t)(end_key >> 32);
+        const int32_t e = (int32_t)~(uint32_t)end_key;
+        o.chain_score = (int32_t)fmax;
+        /* walk back from e; seeds are runs of anchors on one diagonal
+         * that overlap or touch in q */
+        bool open = false, bad = false;
+        uint64_t diag = 0, best_diag = 0;
+        int32_t qlo = 0, qhi = 0, best_lo = 0, best_len = -1;
+        int32_t j = e, steps = 0;
+        /* the walk's next anchors [whi - PICK_WIN + 1, whi] (clipped at 0)
+         * and their p, fetched together: chains mostly step to j - 1, so a
+         * window serves several steps from registers */
+        int32_t whi = -1;
+        tal_chain_anchor wx[PICK_WIN];
+        int32_t wp[PICK_WIN];
+        while (j >= 0 && steps < n) {
+            if (j > whi || j <= whi - PICK_WIN) {
+                whi = j;
+#pragma unroll
+                for (int u = 0; u < PICK_WIN; u++) {
+                    const int64_t at = a0 + max(j - u, 0);
+                    wx[u] = B.a[at];
+                    wp[u] = B.p[at];
+                }
+            }
+            const int32_t u = whi - j;
+            tal_chain_anchor x = wx[0];
+            int32_t pj = wp[0];
+#pragma unroll
+            for (int v = 1; v < PICK_WIN; v++) {
+                if (u == v) {
+                    x = wx[v];
+                    pj = wp[v];
+                }
+            }
+            if (x.q < 0 || x.l < 0 || (int64_t)x.q + x.l > m ||
+                x.r + (uint64_t)x.l > (uint64_t)P.genome_len || x.r < (uint64_t)x.q) {
+                bad = true;
+                break;
+            }
+            const uint64_t d = x.r - (uint64_t)x.q;
+            if (open && d == diag && x.q + x.l >= qlo) {
+                qlo = min(qlo, x.q);
+                qhi = max(qhi, x.q + x.l);
+            } else {
+                if (open && (qhi - qlo > best_len || (qhi - qlo == best_len && qlo < best_lo))) {
+                    best_len = qhi - qlo;
+                    best_lo = qlo;
+                    best_diag = diag;
+                }
+                open = true;
+                diag = d;
+                qlo = x.q;
+                qhi = x.q + x.l;
+            }
+            steps++;
+            j = pj < j ? pj : -1; /* p points back; anything else ends the walk */
+        }
+        if (open && (qhi - qlo > best_len || (qhi - qlo == best_len && qlo < best_lo))) {
+            best_len = qhi - qlo;
+            best_lo = qlo;
+            best_diag = diag;
+        }
+        if (bad) {
+            atomicMax(B.err, EXT_ERR_ANCHOR);
+        } else {
+            o.chain_anchors = steps;
+            const int32_t qb = best_lo, qe = best_lo + best_len;
+            const uint64_t rb = best_diag + (uint64_t)qb;
+            o.seed_r = rb;
+            o.seed_q = qb;
+            o.seed_len = best_len;
+            lq = qb;
+            lt = (int32_t)min((uint64_t)qb + (uint64_t)P.pad, rb);
+            if (lq == 0 || lt == 0) lq = lt = 0;
+            const uint64_t re = rb + (uint64_t)best_len;
+            rq = m - qe;
+            rt = (int32_t)min((uint64_t)(rq + P.pad), (uint64_t)P.genome_len - re);
+            if (rq == 0 || rt == 0) rq = rt = 0;
+        }
+    }
+    o.left_score = o.score = P.match * o.seed_len;
+    B.out[k] = o;
+    B.lhas[k] = lq > 0 ? 1 : 0;
+    B.rhas[k] = rq > 0 ? 1 : 0;
+    B.rbytes[k] = (int64_t)lt + rt;
+    B.qbytes[k] = (int64_t)lq + rq;
+    B.lq[k] = lq;
+    B.lt[k] = lt;
+    B.rq[k] = rq;
+    B.rt[k] = rt;
+}
+
 /*
- * Steps 1-3 of the producer for one read per wave (grid-stride over reads):
- * the chain end (highest f, the first on ties, from a wave reduction of
- * keys f << 32 | ~index), the walk back through p, the merge of
- * the chain's anchors into seeds along diagonals, the longest seed, and the
- * lengths of the two sides.
+ * Steps 1-3 of the producer for 64 reads per wave: the chain end (the first
+ * anchor with the highest f), the walk back through p, the merge of the
+ * chain's anchors into seeds along diagonals, the longest seed, and the
+ * lengths of the two sides.  The chain ends are found read by read with
+ * lanes over the read's anchors (coalesced reads of f, one wave maximum of
+ * f << 32 | ~i, so ties go to the first anchor); the walks then run one lane
+ * per read: a walk is a chain of dependent loads, and one load instruction
+ * serves a step of 64 walks, so their latencies overlap.
  */
 __global__ void __launch_bounds__(64) extend_pick(ExtParams P, ExtBuffers B)
 {
     const int lane = threadIdx.x;
-    for (int k = blockIdx.x; k < P.nreads; k += gridDim.x) {
-        const int64_t a0 = B.aoff[k];
-        const int32_t n = (int32_t)(B.aoff[k + 1] - a0);
-        const int64_t m64 = B.read_off[k + 1] - B.read_off[k];
-        uint64_t key = 0;
-        for (int32_t i = lane; i < n; i += 64) {
-            const uint64_t c = ((uint64_t)B.f[a0 + i] << 32) | (0xFFFFFFFFu - (uint32_t)i);
-            key = c > key ? c : key;
-        }
-        key = wave_max_u64(key);
-        /* every lane walks the chain on the same (broadcast) data and writes
-         * the same records: no lane-divergent branch in the read loop */
-        tal_extend_read o;
-        o.seed_r = 0;
-        o.seed_q = o.seed_len = o.chain_score = o.chain_anchors = 0;
-        o.left_pair = o.right_pair = -1;
-        o.left_score = o.score = 0;
-        int32_t lq = 0, lt = 0, rq = 0, rt = 0;
-        if (m64 < 0 || m64 > 32767) {
-            atomicMax(B.err, EXT_ERR_READ);
-        } else if (n > 0) {
-            const int32_t m = (int32_t)m64;
-            const int32_t e = (int32_t)(0xFFFFFFFFu - (uint32_t)key);
-            o.chain_score = (int32_t)(key >> 32);
-            /* walk back from e; seeds are runs of anchors on one diagonal
-             * that overlap or touch in q */
-            bool open = false, bad = false;
-            uint64_t diag = 0, best_diag = 0;
-            int32_t qlo = 0, qhi = 0, best_lo = 0, best_len = -1;
-            int32_t j = e, steps = 0;
-            while (j >= 0 && steps < n) {
-                const tal_chain_anchor x = B.a[a0 + j];
-                if (x.q < 0 || x.l < 0 || (int64_t)x.q + x.l > m ||
-                    x.r + (uint64_t)x.l > (uint64_t)P.genome_len || x.r < (uint64_t)x.q) {
-                    bad = true;
-                    break;
+    for (int base = blockIdx.x * 64; base < P.nreads; base += gridDim.x * 64) {
+        const int cnt = min(64, P.nreads - base);
+        /* keys of absent anchors are 0, below every real key (~i has its
+         * top bit set); the first two rows of 4 reads are loaded before any
+         * is reduced */
+        uint64_t end_key = 0;
+        for (int t = 0; t < cnt; t += 4) {
+            int64_t b0[4];
+            int32_t bn[4];
+            uint64_t key[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                b0[u] = 0;
+                bn[u] = 0;
+                if (t + u < cnt) {
+                    b0[u] = B.aoff[base + t + u];
+                    bn[u] = (int32_t)min(B.aoff[base + t + u + 1] - b0[u], (int64_t)INT32_MAX - 64);
                 }
-                const uint64_t d = x.r - (uint64_t)x.q;
-                if (open && d == diag && x.q + x.l >= qlo) {
-                    qlo = min(qlo, x.q);
-                    qhi = max(qhi, x.q + x.l);
-                } else {
-                    if (open && (qhi - qlo > best_len || (qhi - qlo == best_len && qlo < best_lo))) {
-                        best_len = qhi - qlo;
-                        best_lo = qlo;
-                        best_diag = diag;
-                    }
-                    open = true;
-                    diag = d;
-                    qlo = x.q;
-                    qhi = x.q + x.l;
+                const uint64_t c0 = lane < bn[u]
+                    ? ((uint64_t)B.f[b0[u] + lane] << 32) | (uint32_t)~lane : 0;
+                const uint64_t c1 = lane + 64 < bn[u]
+                    ? ((uint64_t)B.f[b0[u] + lane + 64] << 32) | (uint32_t)~(lane + 64) : 0;
+                key[u] = c1 > c0 ? c1 : c0;
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                for (int32_t i = lane + 128; i < bn[u]; i += 64) {
+                    const uint64_t c = ((uint64_t)B.f[b0[u] + i] << 32) | (uint32_t)~i;
+                    key[u] = c > key[u] ? c : key[u];
                 }
-                steps++;
-                const int32_t pj = B.p[a0 + j];
-                j = pj < j ? pj : -1; /* p points back; anything else ends the walk */
-            }
-            if (open && (qhi - qlo > best_len || (qhi - qlo == best_len && qlo < best_lo))) {
-                best_len = qhi - qlo;
-                best_lo = qlo;
-                best_diag = diag;
-            }
-            if (bad) {
-                atomicMax(B.err, EXT_ERR_ANCHOR);
-            } else {
-                o.chain_anchors = steps;
-                const int32_t qb = best_lo, qe = best_lo + best_len;
-                const uint64_t rb = best_diag + (uint64_t)qb;
-                o.seed_r = rb;
-                o.seed_q = qb;
-                o.seed_len = best_len;
-                lq = qb;
-                lt = (int32_t)min((uint64_t)qb + (uint64_t)P.pad, rb);
-                if (lq == 0 || lt == 0) lq = lt = 0;
-                const uint64_t re = rb + (uint64_t)best_len;
-                rq = m - qe;
-                rt = (int32_t)min((uint64_t)(rq + P.pad), (uint64_t)P.genome_len - re);
-                if (rq == 0 || rt == 0) rq = rt = 0;
+                const uint64_t k = wave_max_u64(key[u]);
+                end_key = lane == t + u ? k : end_key;
             }
         }
-        o.left_score = o.score = P.match * o.seed_len;
-        B.out[k] = o;
-        B.lhas[k] = lq > 0 ? 1 : 0;
-        B.rhas[k] = rq > 0 ? 1 : 0;
-        B.rbytes[k] = (int64_t)lt + rt;
-        B.qbytes[k] = (int64_t)lq + rq;
-        B.lq[k] = lq;
-        B.lt[k] = lt;
-        B.rq[k] = rq;
-        B.rt[k] = rt;
+        const int k = base + lane;
+        if (k < P.nreads) pick_read(P, B, k, end_key);
     }
 }
 
@@ -215,9 +282,27 @@ namespace {
 int read_grid(int32_t nreads) { return nreads < 65536 ? (nreads > 0 ? nreads : 1) : 65536; }
 } // namespace
 
+/* Waves of the pick kernel per CU (TAL_EXTEND_PICK_WAVES overrides).  Each
+ * walk keeps a line of anchors and one of p in use; few waves per CU keep
+ * them in L1/L2 between steps, many waves hide more latency. */
+int pick_waves_per_cu()
+{
+    static const int w = [] {
+        const char *e = getenv("TAL_EXTEND_PICK_WAVES");
+        const int v = e ? atoi(e) : 0;
+        return v > 0 ? v : 8;
+    }();
+    return w;
+}
+
 hipError_t launch_extend_pick(const ExtParams &P, const ExtBuffers &B, hipStream_t s)
 {
-    hipLaunchKernelGGL(extend_pick, dim3(read_grid(P.nreads)), dim3(64), 0, s, P, B);
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int grid =
+        std::max(1, std::min((P.nreads + 63) / 64, std::max(1, cus) * pick_waves_per_cu()));
+    hipLaunchKernelGGL(extend_pick, dim3(grid), dim3(64), 0, s, P, B);
     return hipGetLastError();
 }
 

@@ -59,7 +59,6 @@ struct Buf {
 /* control words the host reads back after the check kernel */
 struct Summary {
     int32_t ctl[CHAIN_CTL_WORDS];
-    unsigned long long maxn;
     unsigned long long examined;
     unsigned long long far_anchors;
 };
@@ -127,7 +126,7 @@ int run(tal_chain *h, const tal_chain_anchor *a, const int64_t *off, int32_t nse
     CHCHK(h, hipEventRecord(h->ev[0], s), "event");
     CHCHK(h, hipMemsetAsync(ds, 0, sizeof(Summary), s), "clear summary");
     CHCHK(h, launch_chain_check(a, off, nsets, h->keys.as<uint32_t>(), h->ids.as<int32_t>(),
-                                ds->ctl, &ds->maxn, s),
+                                ds->ctl, s),
           "check");
     CHCHK(h, hipMemcpyAsync(h->host, ds, sizeof(Summary), hipMemcpyDeviceToHost, s), "read");
     CHCHK(h, hipStreamSynchronize(s), "sync check");
