@@ -20,6 +20,12 @@ struct ExtParams {
     int32_t pad, match;
 };
 
+/* What the host reads back after the scans. */
+struct ExtTotals {
+    int32_t err, nleft, nright, pad_;
+    int64_t ref_bytes, qer_bytes;
+};
+
 /* Per read, from the pick kernel: what the scans and the build need. */
 struct ExtBuffers {
     const tal_chain_anchor *a;
@@ -40,6 +46,8 @@ struct ExtBuffers {
 };
 
 hipError_t launch_extend_pick(const ExtParams &P, const ExtBuffers &B, hipStream_t s);
+hipError_t launch_extend_totals(const ExtParams &P, const ExtBuffers &B, ExtTotals *t,
+                                hipStream_t s);
 hipError_t launch_extend_build(const ExtParams &P, const ExtBuffers &B, hipStream_t s);
 hipError_t launch_extend_right_h0(const ExtParams &P, const ExtBuffers &B, hipStream_t s);
 hipError_t launch_extend_finish(const ExtParams &P, const ExtBuffers &B, hipStream_t s);

@@ -201,6 +201,19 @@ __global__ void __launch_bounds__(64) extend_pick(ExtParams P, ExtBuffers B)
     }
 }
 
+/* The error word, the pair counts and the byte totals: the last element of
+ * each exclusive scan plus its input. */
+__global__ void extend_totals(ExtParams P, ExtBuffers B, ExtTotals *t)
+{
+    if (threadIdx.x != 0) return;
+    const int64_t last = (int64_t)P.nreads - 1;
+    t->err = *B.err;
+    t->nleft = B.lidx[last] + B.lhas[last];
+    t->nright = B.ridx[last] + B.rhas[last];
+    t->ref_bytes = B.roff[last] + B.rbytes[last];
+    t->qer_bytes = B.qoff[last] + B.qbytes[last];
+}
+
 /*
  * Step 4 for one read per wave: the two sides' bytes (left ones reversed)
  * into the reference and query buffers at the read's scanned offsets, and
@@ -303,6 +316,13 @@ hipError_t launch_extend_pick(const ExtParams &P, const ExtBuffers &B, hipStream
     const int grid =
         std::max(1, std::min((P.nreads + 63) / 64, std::max(1, cus) * pick_waves_per_cu()));
     hipLaunchKernelGGL(extend_pick, dim3(grid), dim3(64), 0, s, P, B);
+    return hipGetLastError();
+}
+
+hipError_t launch_extend_totals(const ExtParams &P, const ExtBuffers &B, ExtTotals *t,
+                                hipStream_t s)
+{
+    hipLaunchKernelGGL(extend_totals, dim3(1), dim3(64), 0, s, P, B, t);
     return hipGetLastError();
 }
 

@@ -4,7 +4,7 @@
  *
  * A run: DP chaining of every read's anchors (tal_chain_dp_device), the
  * producer's pick kernel, four hipCUB scans (pair indexes, byte offsets),
- * one wait for the totals, the build kernel, the left extension pass
+ * one totals kernel and one wait for it, the build kernel, the left extension pass
  * (tal_bsw_get_scores_auto_device), the left scores into the right pairs'
  * h0, the right pass, and the final scores.  Everything stays in HBM.
  */
@@ -54,14 +54,6 @@ struct Buf {
     }
 };
 
-/* what the host reads back after the scans */
-struct Totals {
-    int32_t err;
-    int32_t last_lhas, last_rhas, last_lidx, last_ridx;
-    int32_t pad_;
-    int64_t last_rbytes, last_qbytes, last_roff, last_qoff;
-};
-
 } // namespace
 
 struct tal_extend {
@@ -75,7 +67,7 @@ struct tal_extend {
         ref, qer, err, cubtmp, totals;
     /* the host entry point's device copies */
     Buf h_genome, h_reads, h_read_off, h_anchors, h_anchor_off, h_out;
-    Totals *host = nullptr; /* pinned */
+    ExtTotals *host = nullptr; /* pinned */
     hipEvent_t ev[6] = {};
     int nev = 0;
     tal_extend_stats st{};
@@ -103,14 +95,32 @@ int fail(tal_extend *h, int code, const char *fmt, ...)
                         (what), hipGetErrorString(e_));                                            \
     } while (0)
 
+/* hipCUB temporary storage of one call (sized by a dry run) */
+hipError_t cub_tmp(tal_extend *h, size_t bytes) { return h->cubtmp.ensure(bytes + 16); }
+
 template <class T> hipError_t excl_sum(tal_extend *h, const T *in, T *out, int32_t n, hipStream_t s)
 {
     size_t tmp = 0;
     hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, n, s);
-    if (e != hipSuccess) return e;
-    e = h->cubtmp.ensure(tmp + 16);
+    if (e == hipSuccess) e = cub_tmp(h, tmp);
     if (e != hipSuccess) return e;
     return hipcub::DeviceScan::ExclusiveSum(h->cubtmp.p, tmp, in, out, n, s);
+}
+
+/* Sizes the hipCUB storage for every scan of a run up front, so that no
+ * scan reallocates it while an earlier one is queued. */
+hipError_t cub_reserve(tal_extend *h, int32_t n)
+{
+    size_t a = 0;
+    const hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, a, (const int64_t *)nullptr,
+                                                          (int64_t *)nullptr, n);
+    if (e != hipSuccess) return e;
+    return cub_tmp(h, a);
+}
+
+ExtParams ext_params(const tal_extend *h, int32_t nreads, int64_t genome_len)
+{
+    return ExtParams{nreads, genome_len, h->prm.pad, h->bsw_prm.w_match};
 }
 
 /* Steps 1-4 of the producer; fills the stats' pair counts and byte sizes. */
@@ -133,7 +143,7 @@ int produce(tal_extend *h, const uint8_t *genome, int64_t genome_len, const uint
     EXCHK(h, h->rq.ensure(4 * n), "alloc");
     EXCHK(h, h->rt.ensure(4 * n), "alloc");
     EXCHK(h, h->err.ensure(4), "alloc");
-    ExtParams P{nreads, genome_len, h->prm.pad, h->bsw_prm.w_match};
+    const ExtParams P = ext_params(h, nreads, genome_len);
     ExtBuffers B{};
     B.a = anchors;
     B.aoff = anchor_off;
@@ -156,34 +166,23 @@ int produce(tal_extend *h, const uint8_t *genome, int64_t genome_len, const uint
     B.rq = h->rq.as<int32_t>();
     B.rt = h->rt.as<int32_t>();
     B.err = h->err.as<int32_t>();
+    ExtTotals *dt = h->totals.as<ExtTotals>();
+    EXCHK(h, cub_reserve(h, nreads), "alloc");
     EXCHK(h, hipMemsetAsync(B.err, 0, 4, s), "clear");
     EXCHK(h, launch_extend_pick(P, B, s), "pick");
     EXCHK(h, excl_sum(h, B.lhas, B.lidx, nreads, s), "scan");
     EXCHK(h, excl_sum(h, B.rhas, B.ridx, nreads, s), "scan");
     EXCHK(h, excl_sum(h, B.rbytes, B.roff, nreads, s), "scan");
     EXCHK(h, excl_sum(h, B.qbytes, B.qoff, nreads, s), "scan");
-    /* the totals: last element of each scan plus its input */
-    Totals *dt = h->totals.as<Totals>();
-    const size_t last = (size_t)nreads - 1;
-    EXCHK(h, hipMemcpyAsync(&dt->err, B.err, 4, hipMemcpyDeviceToDevice, s), "copy");
-    EXCHK(h, hipMemcpyAsync(&dt->last_lhas, B.lhas + last, 4, hipMemcpyDeviceToDevice, s), "copy");
-    EXCHK(h, hipMemcpyAsync(&dt->last_rhas, B.rhas + last, 4, hipMemcpyDeviceToDevice, s), "copy");
-    EXCHK(h, hipMemcpyAsync(&dt->last_lidx, B.lidx + last, 4, hipMemcpyDeviceToDevice, s), "copy");
-    EXCHK(h, hipMemcpyAsync(&dt->last_ridx, B.ridx + last, 4, hipMemcpyDeviceToDevice, s), "copy");
-    EXCHK(h, hipMemcpyAsync(&dt->last_rbytes, B.rbytes + last, 8, hipMemcpyDeviceToDevice, s),
-          "copy");
-    EXCHK(h, hipMemcpyAsync(&dt->last_qbytes, B.qbytes + last, 8, hipMemcpyDeviceToDevice, s),
-          "copy");
-    EXCHK(h, hipMemcpyAsync(&dt->last_roff, B.roff + last, 8, hipMemcpyDeviceToDevice, s), "copy");
-    EXCHK(h, hipMemcpyAsync(&dt->last_qoff, B.qoff + last, 8, hipMemcpyDeviceToDevice, s), "copy");
-    EXCHK(h, hipMemcpyAsync(h->host, dt, sizeof(Totals), hipMemcpyDeviceToHost, s), "read");
+    EXCHK(h, launch_extend_totals(P, B, dt, s), "totals");
+    EXCHK(h, hipMemcpyAsync(h->host, dt, sizeof(ExtTotals), hipMemcpyDeviceToHost, s), "read");
     EXCHK(h, hipStreamSynchronize(s), "sync");
-    const Totals t = *h->host;
+    const ExtTotals t = *h->host;
     if (t.err == EXT_ERR_READ) return fail(h, TAL_ERANGE, "a read is longer than 32767 bases");
     if (t.err == EXT_ERR_ANCHOR)
         return fail(h, TAL_EINVAL, "an anchor of a chain lies outside its read or the genome");
-    const int32_t nleft = t.last_lidx + t.last_lhas, nright = t.last_ridx + t.last_rhas;
-    const int64_t rb = t.last_roff + t.last_rbytes, qb = t.last_qoff + t.last_qbytes;
+    const int32_t nleft = t.nleft, nright = t.nright;
+    const int64_t rb = t.ref_bytes, qb = t.qer_bytes;
     if (rb >= ((int64_t)1 << 31) || qb >= ((int64_t)1 << 31))
         return fail(h, TAL_ERANGE, "the extension buffers pass 2^31 bytes");
     EXCHK(h, h->left.ensure(sizeof(tal_seqpair) * std::max(nleft, 1)), "alloc pairs");
@@ -248,9 +247,9 @@ int tal_extend_create(const tal_extend_params *params, const tal_chain_params *c
     }
     for (auto &e : h->ev)
         if (rc == TAL_OK && hipEventCreate(&e) != hipSuccess) rc = TAL_EDEVICE;
-    if (rc == TAL_OK && hipHostMalloc((void **)&h->host, sizeof(Totals)) != hipSuccess)
+    if (rc == TAL_OK && hipHostMalloc((void **)&h->host, sizeof(ExtTotals)) != hipSuccess)
         rc = TAL_ENOMEM;
-    if (rc == TAL_OK && h->totals.ensure(sizeof(Totals)) != hipSuccess) rc = TAL_ENOMEM;
+    if (rc == TAL_OK && h->totals.ensure(sizeof(ExtTotals)) != hipSuccess) rc = TAL_ENOMEM;
     if (rc != TAL_OK) {
         tal_extend_destroy(h);
         return rc;
@@ -312,10 +311,11 @@ int tal_extend_run_device(tal_extend *h, const uint8_t *d_genome, int64_t genome
     hipStream_t s = stream ? (hipStream_t)stream : h->stream;
     /* anchors per read -> f, p, v */
     int64_t na = 0;
-    EXCHK(h, hipMemcpyAsync(&h->host->last_roff, d_anchor_off + nreads, 8, hipMemcpyDeviceToHost, s),
+    EXCHK(h, hipMemcpyAsync(&h->host->ref_bytes, d_anchor_off + nreads, 8, hipMemcpyDeviceToHost,
+                            s),
           "read anchor count");
     EXCHK(h, hipStreamSynchronize(s), "sync");
-    na = h->host->last_roff;
+    na = h->host->ref_bytes;
     if (na < 0) return fail(h, TAL_EINVAL, "anchor offsets");
     const size_t nn = (size_t)std::max<int64_t>(na, 1);
     EXCHK(h, h->f.ensure(4 * nn), "alloc f");
@@ -331,7 +331,7 @@ int tal_extend_run_device(tal_extend *h, const uint8_t *d_genome, int64_t genome
     if (rc != TAL_OK) return rc;
     const tal_extend_stats counts = h->st;
     EXCHK(h, hipEventRecord(h->ev[2], s), "event");
-    const ExtParams P{nreads, genome_len, h->prm.pad, h->bsw_prm.w_match};
+    const ExtParams P = ext_params(h, nreads, genome_len);
     const ExtBuffers B = pass_buffers(h, d_out);
     if (counts.left_pairs > 0) {
         rc = tal_bsw_get_scores_auto_device(h->bsw, h->left.as<tal_seqpair>(), h->ref.as<uint8_t>(),
