@@ -106,8 +106,9 @@ def main():
         _copy_calibration(args.copy_calibration)
         return
     if args.gpus != 1:
-        print("bench.py runs on one GPU: the engine has no multi-GPU path in this build "
-              "(pairs are independent; see DESIGN.md for how it would shard).", file=sys.stderr)
+        print("bench.py runs on one GPU: the engine has no multi-GPU path. Pairs, anchor sets "
+              "and reads are independent, so a batch would shard by contiguous ranges, one "
+              "process and handle per GPU, with no collectives on the hot path.", file=sys.stderr)
         sys.exit(2)
     if args.serve:
         _serve(args)
