@@ -35,6 +35,23 @@ def test_synthetic_sets(nsets, lo, hi):
     eng.close()
 
 
+@pytest.mark.parametrize("nsets,size,ring", [(300, 256, 512), (300, 257, 1024),
+                                              (2000, 40, 512), (30, 3000, 1024)])
+def test_block_shape_follows_set_size(nsets, size, ring, monkeypatch):
+    """Batches of at most 256 anchors a set on average run one wave a set
+    with a 512-slot ring, longer ones two waves a set with 1024 slots; both
+    bit-exact."""
+    _say(f"[gpu] chain shape sets={nsets} size={size}")
+    monkeypatch.delenv("TAL_CHAIN_RING", raising=False)
+    monkeypatch.delenv("TAL_CHAIN_WPB", raising=False)
+    a, off = S.chain_generate(nsets, size, size, seed=size)
+    assert len(a) == nsets * size
+    eng = T.DpChain()
+    _check(eng, a, off, f"sets={nsets} size={size}")
+    assert eng.stats()["lds_ring"] == ring
+    eng.close()
+
+
 @pytest.mark.parametrize("kw,ctor", [(dict(dr=300, dq=200), dict(bw=20, max_iter=50)),
                                      (dict(dr=5000, dq=5000), dict(bw=5000, max_iter=8)),
                                      (dict(dr=100000, dq=100000), dict(bw=0, max_iter=5000)),

@@ -16,6 +16,7 @@ namespace talchain {
 enum {
     CHAIN_CTL_ERR = 0,
     CHAIN_CTL_COUNTER = 1, /* task counter of the DP kernel */
+    CHAIN_CTL_ANCHORS = 2, /* two words: the batch's anchors, low and high half */
     CHAIN_CTL_WORDS = 4
 };
 enum { CHAIN_ERR_OFFSETS = 1, CHAIN_ERR_RANGE = 2 };

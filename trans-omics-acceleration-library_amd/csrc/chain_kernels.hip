@@ -82,6 +82,11 @@ __global__ void chain_check(const tal_chain_anchor *a, const int64_t *off, int32
     if (__ballot(neg)) err = CHAIN_ERR_RANGE;
     if (lane != 0) return;
     if (err) atomicMax(&ctl[CHAIN_CTL_ERR], err);
+    if (k == 0) { /* the batch's anchors, for the host's choice of block shape */
+        const int64_t t = off[nsets] - off[0];
+        ctl[CHAIN_CTL_ANCHORS] = (int32_t)(uint32_t)t;
+        ctl[CHAIN_CTL_ANCHORS + 1] = (int32_t)(t >> 32);
+    }
     const int64_t n = err ? 0 : e - b;
     keys[k] = (uint32_t)min(n, (int64_t)0xFFFFFFFF);
     ids[k] = k;
