@@ -71,13 +71,22 @@ def test_pipeline_edges():
     noisy[70] = (noisy[70] + 1) % 4
     add(noisy, [(2000, 0, 19), (2020, 20, 19), (2040, 40, 19), (2080, 80, 19), (2100, 100, 19),
                 (4000, 5, 19)])
-    # a long read with more anchors than the LDS stage holds (the walk in HBM)
+    # a long read whose chain has 990 anchors: a walk over many register windows
     add(genome[2500:4500], [(2500 + q, q, 19) for q in range(0, 1980, 2)])
+    # two equal seeds on diagonals too far apart to chain: f ties, the first wins
+    add(genome[600:750], [(600, 0, 30), (3600, 100, 30)])
+    # a chain whose back-pointers skip runs of off-diagonal anchors (window refills)
+    chain = [(1500 + q, q, 19) for q in (0, 40, 80, 120)]
+    junk = [(1500 + q + 2 + k, int(rng.integers(0, 131)), 19) for q in (0, 40, 80)
+            for k in range(0, 36, 3)]
+    add(genome[1500:1650], sorted(chain + junk))
     d = dict(genome=genome, reads=np.concatenate(reads), read_off=np.array(roff, np.int64),
              anchors=np.concatenate(anchors), anchor_off=np.array(aoff, np.int64))
     eng = TE.Extender()
     got, _, _ = _check(eng, d, "edges")
-    assert got["chain_anchors"][-1] > 512
+    assert got["chain_anchors"][6] == 990
+    assert got["chain_score"][7] == 30 and got["seed_r"][7] == 600
+    assert got["chain_anchors"][8] >= 4
     assert got["chain_anchors"][0] == 0 and got["left_pair"][0] == -1
     assert got["left_pair"][1] == -1 and got["right_pair"][1] == -1
     assert got["score"][1] == 150
