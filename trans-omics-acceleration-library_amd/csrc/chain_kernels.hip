@@ -229,15 +229,38 @@ __device__ __forceinline__ void phase_a(const SetConst &C, const Target &t, int3
             const uint2 e = rq[j & mask];
             take_a<MODE>(C, t, st, j, e.x, (int32_t)e.y, fr[j & mask], bsc, bj);
         }
-        for (; j - 3 >= jlo; j -= 4) {
-            const int s0 = (j & mask) - 3; /* R is a multiple of 4: no wrap inside */
-            const uint4 x0 = *(const uint4 *)(rq + s0);
-            const uint4 x1 = *(const uint4 *)(rq + s0 + 2);
-            const int4 f4 = *(const int4 *)(fr + s0);
-            take_a<MODE>(C, t, st, j, x1.z, (int32_t)x1.w, f4.w, bsc, bj);
-            take_a<MODE>(C, t, st, j - 1, x1.x, (int32_t)x1.y, f4.z, bsc, bj);
-            take_a<MODE>(C, t, st, j - 2, x0.z, (int32_t)x0.w, f4.y, bsc, bj);
-            take_a<MODE>(C, t, st, j - 3, x0.x, (int32_t)x0.y, f4.x, bsc, bj);
+        /* groups of four slots j-3..j, j = 3 mod 4 (R is a multiple of 4:
+         * no wrap inside); two register sets used in turn, so a group's
+         * reads are issued while the previous group is scored (the last
+         * group re-reads itself instead of reading past jlo) */
+        auto ld4 = [&](int32_t jj, uint4 &x0, uint4 &x1, int4 &f4) {
+            const int s0 = (jj & mask) - 3;
+            x0 = *(const uint4 *)(rq + s0);
+            x1 = *(const uint4 *)(rq + s0 + 2);
+            f4 = *(const int4 *)(fr + s0);
+        };
+        auto take4 = [&](int32_t jj, const uint4 &x0, const uint4 &x1, const int4 &f4) {
+            take_a<MODE>(C, t, st, jj, x1.z, (int32_t)x1.w, f4.w, bsc, bj);
+            take_a<MODE>(C, t, st, jj - 1, x1.x, (int32_t)x1.y, f4.z, bsc, bj);
+            take_a<MODE>(C, t, st, jj - 2, x0.z, (int32_t)x0.w, f4.y, bsc, bj);
+            take_a<MODE>(C, t, st, jj - 3, x0.x, (int32_t)x0.y, f4.x, bsc, bj);
+        };
+        if (j - 3 >= jlo) {
+            uint4 a0, a1, b0, b1;
+            int4 af, bf;
+            ld4(j, a0, a1, af);
+            for (;;) {
+                const bool m1 = j - 7 >= jlo;
+                ld4(m1 ? j - 4 : j, b0, b1, bf);
+                take4(j, a0, a1, af);
+                j -= 4;
+                if (!m1) break;
+                const bool m2 = j - 7 >= jlo;
+                ld4(m2 ? j - 4 : j, a0, a1, af);
+                take4(j, b0, b1, bf);
+                j -= 4;
+                if (!m2) break;
+            }
         }
         for (; j >= jlo; j--) {
             const uint2 e = rq[j & mask];
