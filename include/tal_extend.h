@@ -81,8 +81,9 @@ const char *tal_extend_last_error(const tal_extend *h);
  * read_off[k+1]); its anchors are anchors[anchor_off[k] .. anchor_off[k+1]).
  * Fills out[nreads].  Waits once on the stream, for the pair counts.  Both
  * extension passes run on the handle's one BSW object, left then right, as
- * two getScores calls on one BandedPairWiseSW would.  TAL_ERANGE: a buffer
- * passes 2^31 bytes or a read 32767 bases. */
+ * two getScores16 calls on one BandedPairWiseSW would (the 16-bit path for
+ * every pair, as bench-bsw calls it: bench-bsw.cpp:252-253).  TAL_ERANGE: a
+ * buffer passes 2^31 bytes or a read 32767 bases. */
 int tal_extend_run_device(tal_extend *h, const uint8_t *d_genome, int64_t genome_len,
                           const uint8_t *d_reads, const int64_t *d_read_off, int32_t nreads,
                           const tal_chain_anchor *d_anchors, const int64_t *d_anchor_off,

@@ -83,26 +83,10 @@ def produce(genome, reads, read_off, anchors, anchor_off, f, p, pad=100, match=1
             cat(qer, np.uint8))
 
 
-def _auto(orc, pairs, ref, qer, w):
-    """tal_bsw_get_scores_auto on one oracle object: the 8-bit pairs, then
-    the 16-bit ones."""
-    is8 = TB.is_8bit(pairs)
-    res = pairs.copy()
-    if is8.any():
-        a = orc.scores8(np.ascontiguousarray(pairs[is8]), ref, qer, w)
-        for fld in TB.OUT_FIELDS:
-            res[fld][is8] = a[fld]
-    if (~is8).any():
-        b = orc.scores16(np.ascontiguousarray(pairs[~is8]), ref, qer, w)
-        for fld in TB.OUT_FIELDS:
-            res[fld][~is8] = b[fld]
-    return res
-
-
 def pipeline(genome, reads, read_off, anchors, anchor_off, pad=100, w=100, match=1):
     """The whole pipeline's read records: the chaining oracle, the producer
     model, the left pass, h0 of the right pairs, the right pass (both passes
-    on one BSW oracle object)."""
+    16-bit, on one BSW oracle object)."""
     f, p, _ = O.chain_dp_sets(anchors, anchor_off, nthreads=8)
     out, left, right, ref, qer = produce(genome, reads, read_off, anchors, anchor_off, f, p, pad,
                                          match)
@@ -110,7 +94,7 @@ def pipeline(genome, reads, read_off, anchors, anchor_off, pad=100, w=100, match
     refb = ref if len(ref) else np.zeros(1, np.uint8)
     qerb = qer if len(qer) else np.zeros(1, np.uint8)
     if len(left):
-        left = _auto(orc, left, refb, qerb, w)
+        left = orc.scores16(left.copy(), refb, qerb, w)
     for k in range(len(out)):
         lp = out["left_pair"][k]
         h0 = int(left["score"][lp]) if lp >= 0 else match * int(out["seed_len"][k])
@@ -119,7 +103,7 @@ def pipeline(genome, reads, read_off, anchors, anchor_off, pad=100, w=100, match
         if rp >= 0:
             right["h0"][rp] = h0
     if len(right):
-        right = _auto(orc, right, refb, qerb, w)
+        right = orc.scores16(right.copy(), refb, qerb, w)
     for k in range(len(out)):
         rp = out["right_pair"][k]
         if rp >= 0:

@@ -5,7 +5,7 @@
  * A run: DP chaining of every read's anchors (tal_chain_dp_device), the
  * producer's pick kernel, four hipCUB scans (pair indexes, byte offsets),
  * one totals kernel and one wait for it, the build kernel, the left extension pass
- * (tal_bsw_get_scores_auto_device), the left scores into the right pairs'
+ * (tal_bsw_get_scores16_device, bench-bsw's getScores16), the left scores into the right pairs'
  * h0, the right pass, and the final scores.  Everything stays in HBM.
  */
 #include <hip/hip_runtime.h>
@@ -334,14 +334,14 @@ int tal_extend_run_device(tal_extend *h, const uint8_t *d_genome, int64_t genome
     const ExtParams P = ext_params(h, nreads, genome_len);
     const ExtBuffers B = pass_buffers(h, d_out);
     if (counts.left_pairs > 0) {
-        rc = tal_bsw_get_scores_auto_device(h->bsw, h->left.as<tal_seqpair>(), h->ref.as<uint8_t>(),
+        rc = tal_bsw_get_scores16_device(h->bsw, h->left.as<tal_seqpair>(), h->ref.as<uint8_t>(),
                                             h->qer.as<uint8_t>(), counts.left_pairs, h->prm.w, s);
         if (rc != TAL_OK) return fail(h, rc, "left extension: %s", tal_bsw_last_error(h->bsw));
     }
     EXCHK(h, launch_extend_right_h0(P, B, s), "right h0");
     EXCHK(h, hipEventRecord(h->ev[3], s), "event");
     if (counts.right_pairs > 0) {
-        rc = tal_bsw_get_scores_auto_device(h->bsw, h->right.as<tal_seqpair>(),
+        rc = tal_bsw_get_scores16_device(h->bsw, h->right.as<tal_seqpair>(),
                                             h->ref.as<uint8_t>(), h->qer.as<uint8_t>(),
                                             counts.right_pairs, h->prm.w, s);
         if (rc != TAL_OK) return fail(h, rc, "right extension: %s", tal_bsw_last_error(h->bsw));
