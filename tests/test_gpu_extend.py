@@ -6,10 +6,12 @@ import numpy as np
 import pytest
 
 import extend_model as M
+import oracle as O
 import synth as S
 import tal_bsw as TB
 import tal_chain as TC
 import tal_extend as TE
+from helpers import DeviceArrays
 
 pytestmark = pytest.mark.gpu
 
@@ -112,4 +114,53 @@ def test_pipeline_errors():
     # the handle still works afterwards
     d = S.extend_generate(50, 150, 100_000, seed=8)
     _check(eng, d, "after errors")
+    eng.close()
+
+
+def _device_inputs(d):
+    arrays = [np.ascontiguousarray(d["genome"], np.uint8),
+              np.ascontiguousarray(d["reads"], np.uint8),
+              np.ascontiguousarray(d["read_off"], np.int64),
+              np.ascontiguousarray(d["anchors"], TC.ANCHOR_DTYPE),
+              np.ascontiguousarray(d["anchor_off"], np.int64)]
+    return arrays, DeviceArrays(*arrays)
+
+
+def test_device_entry_points():
+    """tal_extend_run_device on device buffers gives what tal_extend_run
+    gives on host buffers; tal_extend_produce_device on the chaining
+    oracle's f and p gives the model's pairs, bytes and seed records."""
+    _say("[gpu] extend device entry points 500 reads")
+    d = S.extend_generate(500, 150, 500_000, seed=31)
+    n = len(d["read_off"]) - 1
+    host = TE.Extender()
+    want = host.run(d["genome"], d["reads"], d["read_off"], d["anchors"], d["anchor_off"])
+    host.close()
+    arrays, dev = _device_inputs(d)
+    out = np.zeros(n, TE.READ_DTYPE)
+    dout = DeviceArrays(out)
+    eng = TE.Extender()
+    eng.run_device(dev.ptr(0), len(arrays[0]), dev.ptr(1), dev.ptr(2), n, dev.ptr(3), dev.ptr(4),
+                   dout.ptr(0))
+    got = dout.download(0, out.copy())
+    for fld in TE.READ_DTYPE.names:
+        assert np.array_equal(got[fld], want[fld]), f"run_device: {fld}"
+    # the producer alone, on the oracle's chaining results
+    f, p, _ = O.chain_dp_sets(d["anchors"], d["anchor_off"], nthreads=8)
+    fp = DeviceArrays(np.ascontiguousarray(f, np.uint32), np.ascontiguousarray(p, np.int32))
+    eng.produce_device(dev.ptr(0), len(arrays[0]), dev.ptr(1), dev.ptr(2), n, dev.ptr(3),
+                       dev.ptr(4), fp.ptr(0), fp.ptr(1), dout.ptr(0))
+    got = dout.download(0, out.copy())
+    mout, ml, mr, mref, mqer = M.produce(d["genome"], d["reads"], d["read_off"], d["anchors"],
+                                         d["anchor_off"], f, p)
+    for fld in ("seed_r", "seed_q", "seed_len", "chain_score", "chain_anchors", "left_pair",
+                "right_pair"):
+        assert np.array_equal(got[fld], mout[fld]), f"produce_device: {fld}"
+    gl, gr, gref, gqer = eng.pairs()
+    for fld in IN_FIELDS:
+        assert np.array_equal(gl[fld], ml[fld]), f"produce_device: left {fld}"
+        assert np.array_equal(gr[fld], mr[fld]), f"produce_device: right {fld}"
+    assert np.array_equal(gref, mref) and np.array_equal(gqer, mqer)
+    for h in (fp, dout, dev):
+        h.close()
     eng.close()

@@ -134,6 +134,16 @@ class Extender:
                                                  c(d_anchors), c(d_anchor_off), c(d_out),
                                                  c(stream)))
 
+    def produce_device(self, d_genome, genome_len, d_reads, d_read_off, nreads, d_anchors,
+                       d_anchor_off, d_f, d_p, d_out, stream=0):
+        """The producer alone on chaining results already in device memory
+        (f, p); the pairs and buffers then come from pairs()."""
+        c = ctypes.c_void_p
+        self._check(load().tal_extend_produce_device(self._h, c(d_genome), int(genome_len),
+                                                     c(d_reads), c(d_read_off), int(nreads),
+                                                     c(d_anchors), c(d_anchor_off), c(d_f),
+                                                     c(d_p), c(d_out), c(stream)))
+
     def pairs(self):
         """(left pairs, right pairs, ref bytes, query bytes) of the last call."""
         st = self.stats()
