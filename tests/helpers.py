@@ -55,3 +55,30 @@ class DeviceArrays:
         for d in self.ptrs:
             self.hip.hipFree(d)
         self.ptrs = []
+
+
+def check_c_layouts(tmp_path, root, header, dtypes=(), structs=()):
+    """Compiles `header` as C11 with warnings as errors, together with
+    static assertions that each C struct has the size and field offsets of
+    its Python mirror: dtypes are (C name, numpy dtype), structs are (C name,
+    ctypes.Structure subclass)."""
+    import ctypes
+    import os
+    import subprocess
+    checks = []
+    for cname, dt in dtypes:
+        checks.append(f"_Static_assert(sizeof({cname}) == {dt.itemsize}, \"{cname}\");")
+        for name in dt.names:
+            off = dt.fields[name][1]
+            checks.append(f"_Static_assert(offsetof({cname}, {name}) == {off}, \"{cname}.{name}\");")
+    for cname, cls in structs:
+        checks.append(f"_Static_assert(sizeof({cname}) == {ctypes.sizeof(cls)}, \"{cname}\");")
+        for name, _ in cls._fields_:
+            off = getattr(cls, name).offset
+            checks.append(f"_Static_assert(offsetof({cname}, {name}) == {off}, \"{cname}.{name}\");")
+    src = tmp_path / "layout_check.c"
+    src.write_text(f"#include <stddef.h>\n#include \"{header}\"\n" + "\n".join(checks) +
+                   "\nint main(void) { return 0; }\n")
+    subprocess.check_call(["gcc", "-std=c11", "-Wall", "-Wextra", "-Werror", "-I",
+                           os.path.join(root, "include"), "-c", str(src), "-o",
+                           str(tmp_path / "layout_check.o")])

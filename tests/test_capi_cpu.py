@@ -168,3 +168,24 @@ def test_null_handle_calls():
     assert L.tal_bsw_count_8bit(None, None, 0) == -1
     assert L.tal_bsw_get_scores16(None, None, None, 0, None, 0, 0, 100) == T.TAL_EINVAL
     assert L.tal_bsw_last_error(None) == b"null handle"
+
+
+def test_header_layouts_match_the_bindings(tmp_path):
+    """Every struct that a Python binding mirrors has, in its C header, the
+    binding's size and field offsets (the headers compile as C11 with
+    warnings as errors)."""
+    import tal_chain
+    import tal_sw
+    from helpers import check_c_layouts
+    check_c_layouts(tmp_path, ROOT, "tal_bsw.h", dtypes=[("tal_seqpair", T.SEQPAIR_DTYPE)],
+                    structs=[("tal_bsw_params", T.Params), ("tal_bsw_stats", T.Stats),
+                             ("tal_bsw_tuning", T.Tuning),
+                             ("tal_bsw_server_config", T.ServerConfig),
+                             ("tal_bsw_request_info", T.RequestInfo),
+                             ("tal_bsw_server_stats", T.ServerStats), ("tal_bsw_pack", T.Pack)])
+    check_c_layouts(tmp_path, ROOT, "tal_sw.h", dtypes=[("tal_sw_pair", tal_sw.PAIR_DTYPE)],
+                    structs=[("tal_sw_stats", tal_sw.Stats)])
+    check_c_layouts(tmp_path, ROOT, "tal_chain.h",
+                    dtypes=[("tal_chain_anchor", tal_chain.ANCHOR_DTYPE)],
+                    structs=[("tal_chain_params", tal_chain.Params),
+                             ("tal_chain_stats", tal_chain.Stats)])

@@ -82,3 +82,11 @@ def test_extend_header_declares_binding_exports():
 
 def test_read_record_layout():
     assert TE.READ_DTYPE.itemsize == 40
+
+
+def test_extend_header_layouts_match_the_binding(tmp_path):
+    """tal_extend.h compiles as C11 with warnings as errors, and its structs
+    have the sizes and field offsets that the Python binding assumes."""
+    from helpers import check_c_layouts
+    check_c_layouts(tmp_path, ROOT, "tal_extend.h", dtypes=[("tal_extend_read", TE.READ_DTYPE)],
+                    structs=[("tal_extend_params", TE.Params), ("tal_extend_stats", TE.Stats)])
