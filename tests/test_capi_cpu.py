@@ -189,3 +189,14 @@ def test_header_layouts_match_the_bindings(tmp_path):
                     dtypes=[("tal_chain_anchor", tal_chain.ANCHOR_DTYPE)],
                     structs=[("tal_chain_params", tal_chain.Params),
                              ("tal_chain_stats", tal_chain.Stats)])
+
+
+@pytest.mark.parametrize("modname", ["tal_bsw", "tal_sw", "tal_chain", "tal_extend"])
+def test_missing_library_fails_loudly(modname, monkeypatch, tmp_path):
+    """Without its built library a binding raises; there is no fallback."""
+    import importlib
+    mod = importlib.import_module(modname)
+    monkeypatch.setattr(mod, "LIB_PATH", str(tmp_path / "absent.so"))
+    monkeypatch.setattr(mod, "_lib", None)
+    with pytest.raises(RuntimeError, match="not built"):
+        mod.load()
