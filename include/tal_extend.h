@@ -83,7 +83,9 @@ const char *tal_extend_last_error(const tal_extend *h);
  * extension passes run on the handle's one BSW object, left then right, as
  * two getScores16 calls on one BandedPairWiseSW would (the 16-bit path for
  * every pair, as bench-bsw calls it: bench-bsw.cpp:252-253).  TAL_ERANGE: a
- * buffer passes 2^31 bytes or a read 32767 bases. */
+ * buffer passes 2^31 bytes or a read 32767 bases.  TAL_EINVAL: read or anchor
+ * offsets negative or decreasing, or a chain anchor outside its read or the
+ * genome (nothing outside the offsets' ranges is read). */
 int tal_extend_run_device(tal_extend *h, const uint8_t *d_genome, int64_t genome_len,
                           const uint8_t *d_reads, const int64_t *d_read_off, int32_t nreads,
                           const tal_chain_anchor *d_anchors, const int64_t *d_anchor_off,
@@ -98,7 +100,7 @@ int tal_extend_run(tal_extend *h, const uint8_t *genome, int64_t genome_len, con
  * and p of the anchors): fills out[nreads] but the scores, and the pair
  * arrays and buffers of the handle, which tal_extend_get_pairs copies out.
  * Right pairs carry h0 = the seed score; the pipeline sets it from the left
- * scores. */
+ * scores.  Errors as for tal_extend_run_device. */
 int tal_extend_produce_device(tal_extend *h, const uint8_t *d_genome, int64_t genome_len,
                               const uint8_t *d_reads, const int64_t *d_read_off, int32_t nreads,
                               const tal_chain_anchor *d_anchors, const int64_t *d_anchor_off,

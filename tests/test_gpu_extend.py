@@ -164,3 +164,38 @@ def test_device_entry_points():
     for h in (fp, dout, dev):
         h.close()
     eng.close()
+
+
+def test_invalid_offsets_are_reported():
+    """Negative or decreasing anchor or read offsets give TAL_EINVAL from the
+    producer (nothing outside the offsets' ranges is read), and the handle
+    still works afterwards."""
+    _say("[gpu] extend invalid offsets")
+    rng = np.random.default_rng(9)
+    genome = rng.integers(0, 4, 2000, dtype=np.uint8)
+    reads = np.concatenate([genome[100:250], genome[500:650]])
+    a = np.zeros(8, TC.ANCHOR_DTYPE)
+    for i in range(8):
+        a[i] = (100 + 10 * i, 10 * i, 19)
+    f = np.full(8, 19, np.uint32)
+    p = np.full(8, -1, np.int32)
+    out = np.zeros(2, TE.READ_DTYPE)
+    eng = TE.Extender()
+    cases = [("decreasing anchor offsets", [0, 150, 300], [0, 5, 3]),
+             ("negative anchor start", [0, 150, 300], [-4, 2, 8]),
+             ("decreasing read offsets", [0, 150, 100], [0, 8, 8])]
+    for label, roff, aoff in cases:
+        dev = DeviceArrays(genome, reads, np.array(roff, np.int64), a, np.array(aoff, np.int64),
+                           f, p, out)
+        with pytest.raises(TE.TalError) as e:
+            eng.produce_device(dev.ptr(0), len(genome), dev.ptr(1), dev.ptr(2), 2, dev.ptr(3),
+                               dev.ptr(4), dev.ptr(5), dev.ptr(6), dev.ptr(7))
+        assert e.value.code == TE.TAL_EINVAL, label
+        assert "offsets" in str(e.value), label
+        dev.close()
+    with pytest.raises(TE.TalError) as e:
+        eng.run(genome, reads, np.array([0, 150, 100]), a, np.array([0, 8, 8]))
+    assert e.value.code == TE.TAL_EINVAL
+    d = S.extend_generate(40, 150, 100_000, seed=12)
+    _check(eng, d, "after invalid offsets")
+    eng.close()

@@ -12,7 +12,7 @@
 
 namespace talext {
 
-enum { EXT_ERR_ANCHOR = 1, EXT_ERR_READ = 2 };
+enum { EXT_ERR_ANCHOR = 1, EXT_ERR_READ = 2, EXT_ERR_OFFSETS = 3 };
 
 struct ExtParams {
     int32_t nreads;

@@ -6,7 +6,6 @@
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
-
 #include <stdlib.h>
 
 #include <algorithm>
@@ -44,7 +43,8 @@ __device__ __forceinline__ void pick_read(const ExtParams &P, const ExtBuffers &
                                           uint64_t end_key)
 {
     const int64_t a0 = B.aoff[k];
-    const int32_t n = (int32_t)(B.aoff[k + 1] - a0);
+    const int64_t n64 = B.aoff[k + 1] - a0;
+    const int32_t n = (int32_t)n64;
     const int64_t m64 = B.read_off[k + 1] - B.read_off[k];
     tal_extend_read o;
     o.seed_r = 0;
@@ -52,7 +52,9 @@ __device__ __forceinline__ void pick_read(const ExtParams &P, const ExtBuffers &
     o.left_pair = o.right_pair = -1;
     o.left_score = o.score = 0;
     int32_t lq = 0, lt = 0, rq = 0, rt = 0;
-    if (m64 < 0 || m64 > 32767) {
+    if (a0 < 0 || n64 < 0 || n64 > INT32_MAX || B.read_off[k] < 0 || m64 < 0) {
+        atomicMax(B.err, EXT_ERR_OFFSETS); /* no anchor or base of it is read */
+    } else if (m64 > 32767) {
         atomicMax(B.err, EXT_ERR_READ);
     } else if (n > 0) {
         const int32_t m = (int32_t)m64;
@@ -177,8 +179,13 @@ __global__ void __launch_bounds__(64) extend_pick(ExtParams P, ExtBuffers B)
                 b0[u] = 0;
                 bn[u] = 0;
                 if (t + u < cnt) {
-                    b0[u] = B.aoff[base + t + u];
-                    bn[u] = (int32_t)min(B.aoff[base + t + u + 1] - b0[u], (int64_t)INT32_MAX - 64);
+                    /* a set with invalid offsets reads nothing (pick_read
+                     * reports it) */
+                    const int64_t s0 = B.aoff[base + t + u], d = B.aoff[base + t + u + 1] - s0;
+                    if (s0 >= 0 && d > 0) {
+                        b0[u] = s0;
+                        bn[u] = (int32_t)min(d, (int64_t)INT32_MAX - 64);
+                    }
                 }
                 const uint64_t c0 = lane < bn[u]
                     ? ((uint64_t)B.f[b0[u] + lane] << 32) | (uint32_t)~lane : 0;

@@ -178,6 +178,8 @@ int produce(tal_extend *h, const uint8_t *genome, int64_t genome_len, const uint
     EXCHK(h, hipMemcpyAsync(h->host, dt, sizeof(ExtTotals), hipMemcpyDeviceToHost, s), "read");
     EXCHK(h, hipStreamSynchronize(s), "sync");
     const ExtTotals t = *h->host;
+    if (t.err == EXT_ERR_OFFSETS)
+        return fail(h, TAL_EINVAL, "read or anchor offsets must be non-negative and non-decreasing");
     if (t.err == EXT_ERR_READ) return fail(h, TAL_ERANGE, "a read is longer than 32767 bases");
     if (t.err == EXT_ERR_ANCHOR)
         return fail(h, TAL_EINVAL, "an anchor of a chain lies outside its read or the genome");
