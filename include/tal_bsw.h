@@ -107,10 +107,15 @@ int tal_bsw_get_scores_auto(tal_bsw *h, tal_seqpair *pairs, const uint8_t *ref,
 
 /* Device-resident variants: pairs/ref/qer are device pointers on the handle's
  * device; outputs are written in place into the device pairs.  `stream` is a
- * hipStream_t (NULL = the handle's own stream); the call returns once the
- * work is queued, except that the 16-bit path may wait for a small status
- * word when a group's stale-column read needs a rerun.  No host copies of the
- * inputs are made. */
+ * hipStream_t (NULL = the handle's own stream).  The call waits on the stream
+ * for a few status words: the batch's largest group (the kernels' buffers are
+ * sized from it), the 8-bit path's padding length, the dispatcher's 8-bit
+ * count, and, in the 16-bit path, the groups whose stale-column read needs a
+ * rerun; the DP itself and the result writes stay queued.  No host copies of
+ * the inputs are made.  As in the reference, which reads seqBufRef + idr and
+ * seqBufQer + idq unchecked (bandedSWA.cpp:2806, 2848), keeping every pair's
+ * sequences inside ref and qer is the caller's part; a negative length, or
+ * one over the path's limit, gives TAL_ERANGE. */
 int tal_bsw_get_scores16_device(tal_bsw *h, tal_seqpair *pairs,
                                 const uint8_t *ref, const uint8_t *qer,
                                 int32_t n, int32_t w, void *stream);
