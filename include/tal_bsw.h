@@ -4,9 +4,11 @@
  * Drop-in for TAL's BandedPairWiseSW (src/alignment/BSW/bandedSWA.h:113-336):
  * the same 56-byte SeqPair records, the same packed reference/query byte
  * buffers addressed by per-pair offsets, the same scoring parameters, and the
- * same six outputs written in place, bit-identical to the AVX-512BW build of
- * getScores16 / getScores8.  Plain C types only; every call returns a status
- * code and never exits the process.
+ * same six outputs written in place, bit-identical to a line-by-line CPU
+ * restatement of the AVX-512BW build of getScores16 / getScores8 (the
+ * reference's BSW source needs bwa-mem2's macro.h, which it does not ship, so
+ * it is not compiled for comparison).  Plain C types only; every call returns
+ * a status code and never exits the process.
  *
  * Group semantics.  Like the reference, getScores16 processes pairs in
  * consecutive groups of 32 and getScores8 in groups of 64; pairs of one group

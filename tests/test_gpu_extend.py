@@ -196,6 +196,27 @@ def test_invalid_offsets_are_reported():
     with pytest.raises(TE.TalError) as e:
         eng.run(genome, reads, np.array([0, 150, 100]), a, np.array([0, 8, 8]))
     assert e.value.code == TE.TAL_EINVAL
+    # a negative genome length is refused before any kernel runs
+    dev = DeviceArrays(genome, reads, np.array([0, 150, 300], np.int64), a,
+                       np.array([0, 4, 8], np.int64), f, p, out)
+    with pytest.raises(TE.TalError) as e:
+        eng.produce_device(dev.ptr(0), -1, dev.ptr(1), dev.ptr(2), 2, dev.ptr(3),
+                           dev.ptr(4), dev.ptr(5), dev.ptr(6), dev.ptr(7))
+    assert e.value.code == TE.TAL_EINVAL and "genome_len" in str(e.value)
+    dev.close()
+    # an anchor whose reference position is near 2^64 (r + l wraps) is the
+    # best chain of read 0 and must be reported, not read through
+    a2 = a.copy()
+    a2[1]["r"] = np.uint64(2**64 - 5)
+    f2 = f.copy()
+    f2[1] = 60
+    dev = DeviceArrays(genome, reads, np.array([0, 150, 300], np.int64), a2,
+                       np.array([0, 4, 8], np.int64), f2, p, out)
+    with pytest.raises(TE.TalError) as e:
+        eng.produce_device(dev.ptr(0), len(genome), dev.ptr(1), dev.ptr(2), 2, dev.ptr(3),
+                           dev.ptr(4), dev.ptr(5), dev.ptr(6), dev.ptr(7))
+    assert e.value.code == TE.TAL_EINVAL and "outside" in str(e.value)
+    dev.close()
     d = S.extend_generate(40, 150, 100_000, seed=12)
     _check(eng, d, "after invalid offsets")
     eng.close()

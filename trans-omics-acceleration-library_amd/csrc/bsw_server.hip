@@ -270,10 +270,15 @@ struct WorkerBufs {
     template <class T> static hipError_t grow(T **p, size_t *cap, size_t n)
     {
         if (n <= *cap) return hipSuccess;
-        if (*p) (void)hipFree(*p);
-        *p = nullptr;
-        *cap = 0;
+        /* 1.5x growth from the old capacity, so that a slowly growing batch
+         * size does not reallocate every time */
         const size_t want = std::max(n, *cap * 3 / 2);
+        if (*p) {
+            const hipError_t f = hipFree(*p);
+            *p = nullptr;
+            *cap = 0;
+            if (f != hipSuccess) return f;
+        }
         hipError_t e = hipMalloc((void **)p, sizeof(T) * want);
         if (e == hipSuccess) *cap = want;
         return e;

@@ -93,8 +93,10 @@ __device__ __forceinline__ void pick_read(const ExtParams &P, const ExtBuffers &
                     pj = wp[v];
                 }
             }
-            if (x.q < 0 || x.l < 0 || (int64_t)x.q + x.l > m ||
-                x.r + (uint64_t)x.l > (uint64_t)P.genome_len || x.r < (uint64_t)x.q) {
+            /* r + l <= genome_len, written without the sum (r can be near
+             * 2^64; genome_len >= 0 is checked by the host) */
+            if (x.q < 0 || x.l < 0 || (int64_t)x.q + x.l > m || x.r > (uint64_t)P.genome_len ||
+                (uint64_t)x.l > (uint64_t)P.genome_len - x.r || x.r < (uint64_t)x.q) {
                 bad = true;
                 break;
             }

@@ -213,13 +213,14 @@ ExtBuffers pass_buffers(tal_extend *h, tal_extend_read *out)
     return B;
 }
 
-int check_common(tal_extend *h, const uint8_t *genome, const uint8_t *reads,
+int check_common(tal_extend *h, const uint8_t *genome, int64_t genome_len, const uint8_t *reads,
                  const int64_t *read_off, int32_t nreads, const tal_chain_anchor *anchors,
                  const int64_t *anchor_off, tal_extend_read *out)
 {
     if (!h) return TAL_EINVAL;
     h->errmsg.clear();
     if (nreads < 0) return fail(h, TAL_EINVAL, "nreads < 0");
+    if (genome_len < 0) return fail(h, TAL_EINVAL, "genome_len < 0");
     if (nreads > 0 && (!genome || !reads || !read_off || !anchors || !anchor_off || !out))
         return fail(h, TAL_EINVAL, "null buffer");
     return TAL_OK;
@@ -287,7 +288,7 @@ int tal_extend_produce_device(tal_extend *h, const uint8_t *d_genome, int64_t ge
                               const uint32_t *d_f, const int32_t *d_p, tal_extend_read *d_out,
                               void *stream)
 {
-    const int c = check_common(h, d_genome, d_reads, d_read_off, nreads, d_anchors, d_anchor_off,
+    const int c = check_common(h, d_genome, genome_len, d_reads, d_read_off, nreads, d_anchors, d_anchor_off,
                                d_out);
     if (c != TAL_OK) return c;
     h->st = tal_extend_stats{};
@@ -304,7 +305,7 @@ int tal_extend_run_device(tal_extend *h, const uint8_t *d_genome, int64_t genome
                           const tal_chain_anchor *d_anchors, const int64_t *d_anchor_off,
                           tal_extend_read *d_out, void *stream)
 {
-    const int c = check_common(h, d_genome, d_reads, d_read_off, nreads, d_anchors, d_anchor_off,
+    const int c = check_common(h, d_genome, genome_len, d_reads, d_read_off, nreads, d_anchors, d_anchor_off,
                                d_out);
     if (c != TAL_OK) return c;
     h->st = tal_extend_stats{};
@@ -366,7 +367,7 @@ int tal_extend_run(tal_extend *h, const uint8_t *genome, int64_t genome_len, con
                    const int64_t *read_off, int32_t nreads, const tal_chain_anchor *anchors,
                    const int64_t *anchor_off, tal_extend_read *out)
 {
-    const int c = check_common(h, genome, reads, read_off, nreads, anchors, anchor_off, out);
+    const int c = check_common(h, genome, genome_len, reads, read_off, nreads, anchors, anchor_off, out);
     if (c != TAL_OK) return c;
     h->st = tal_extend_stats{};
     if (nreads == 0) return TAL_OK;
