@@ -8,7 +8,9 @@
  * Differences a caller can observe: failures throw std::runtime_error instead
  * of exit(EXIT_FAILURE) (bandedSWA.cpp:88-93); nothing is written past
  * pairArray[numPairs-1]; numThreads is accepted and ignored (the reference
- * only sizes per-thread scratch with it); the scalar path is not provided.
+ * only sizes per-thread scratch with it).  Like the reference, the object
+ * keeps the constructor's mat_ pointer and its scalar path reads the matrix
+ * at call time.
  */
 #ifndef TAL_BANDEDSWA_TAL_HPP
 #define TAL_BANDEDSWA_TAL_HPP
@@ -37,8 +39,8 @@ class BandedPairWiseSW {
                      const int zdrop, const int end_bonus, const int8_t *mat_,
                      const int8_t w_match, const int8_t w_mismatch, int numThreads,
                      int device = 0)
+        : mat_ptr_(mat_)
     {
-        (void)mat_;
         (void)numThreads;
         tal_bsw_params p;
         p.o_del = o_del;
@@ -83,8 +85,42 @@ class BandedPairWiseSW {
         check(tal_bsw_get_scores_auto(h_, pairArray, seqBufRef, rl, seqBufQer, ql, numPairs, w));
     }
 
+    /* scalarBandedSWA (bandedSWA.h:124-129, bandedSWA.cpp:116-237): one pair,
+     * scored on the GPU by the per-pair kernel; returns the score. */
+    int scalarBandedSWA(int qlen, const uint8_t *query, int tlen, const uint8_t *target,
+                        int32_t w, int h0, int *_qle, int *_tle, int *_gtle, int *_gscore,
+                        int *_max_off)
+    {
+        SeqPair p{};
+        p.idr = 0;
+        p.idq = 0;
+        p.len1 = tlen;
+        p.len2 = qlen;
+        p.h0 = h0;
+        check(tal_bsw_scalar_scores(h_, mat_ptr_, &p, target, (size_t)(tlen > 0 ? tlen : 0), query,
+                                    (size_t)(qlen > 0 ? qlen : 0), 1, w));
+        if (_qle) *_qle = p.qle;
+        if (_tle) *_tle = p.tle;
+        if (_gtle) *_gtle = p.gtle;
+        if (_gscore) *_gscore = p.gscore;
+        if (_max_off) *_max_off = p.max_off;
+        return p.score;
+    }
+
+    /* scalarBandedSWAWrapper (bandedSWA.h:131-136, bandedSWA.cpp:242-260). */
+    void scalarBandedSWAWrapper(SeqPair *seqPairArray, uint8_t *seqBufRef, uint8_t *seqBufQer,
+                                int numPairs, int nthreads, int32_t w)
+    {
+        (void)nthreads;
+        size_t rl = 0, ql = 0;
+        extents(seqPairArray, numPairs, &rl, &ql);
+        check(tal_bsw_scalar_scores(h_, mat_ptr_, seqPairArray, seqBufRef, rl, seqBufQer, ql,
+                                    numPairs, w));
+    }
+
   private:
     tal_bsw *h_ = nullptr;
+    const int8_t *mat_ptr_ = nullptr;
 
     /* The reference addresses the buffers without extents; the C ABI copies
      * them to the device, so the shim derives max(idr+len1), max(idq+len2). */

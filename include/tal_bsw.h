@@ -128,6 +128,25 @@ int tal_bsw_get_scores_auto_device(tal_bsw *h, tal_seqpair *pairs,
                                    const uint8_t *ref, const uint8_t *qer,
                                    int32_t n, int32_t w, void *stream);
 
+/* Replaces BandedPairWiseSW::scalarBandedSWAWrapper (bandedSWA.cpp:242-260),
+ * and with n = 1 scalarBandedSWA (:116-237): the textbook per-pair extension,
+ * each pair scored on its own (no group coupling; its own column range,
+ * narrowing and exits), in int32 arithmetic, with the 5x5 scoring matrix
+ * `mat` (host memory; mat[r * 5 + q] scores reference code r against query
+ * code q, as the reference indexes the matrix it was constructed with, which
+ * bench-bsw fills with bwa_fill_scmat, benchmarks/bench-bsw.cpp:56-64).  The
+ * gap and end-bonus parameters are the handle's.  Every sequence byte must be
+ * a code 0-4, since the reference indexes mat with it; another byte, or a
+ * length over 32767, gives TAL_ERANGE.  The host form is blocking; the device
+ * form takes pairs/ref/qer in device memory and waits on the stream for the
+ * batch's longest length and the error word. */
+int tal_bsw_scalar_scores(tal_bsw *h, const int8_t *mat, tal_seqpair *pairs, const uint8_t *ref,
+                          size_t ref_len, const uint8_t *qer, size_t qer_len, int32_t n,
+                          int32_t w);
+int tal_bsw_scalar_scores_device(tal_bsw *h, const int8_t *mat, tal_seqpair *pairs,
+                                 const uint8_t *ref, const uint8_t *qer, int32_t n, int32_t w,
+                                 void *stream);
+
 /* Number of pairs of pairs[0..n) that the dispatcher sends to the 8-bit path
  * (host memory). */
 int32_t tal_bsw_count_8bit(const tal_bsw *h, const tal_seqpair *pairs,
@@ -178,6 +197,13 @@ int tal_bsw_plan_create(tal_bsw *h, tal_seqpair *pairs, const uint8_t *ref,
                         const uint8_t *qer, int32_t n, int32_t w, tal_bsw_plan **out);
 int tal_bsw_plan_launch(tal_bsw_plan *plan, void *stream);
 void tal_bsw_plan_destroy(tal_bsw_plan *plan);
+
+/* Stale-column rerun rounds captured in the graph of plans created after this
+ * call (0..8, default 2).  Groups still pending after the replay's rounds are
+ * rerun eagerly by tal_bsw_plan_launch; results are the same for every value.
+ * After a launch, tal_bsw_get_stats reports in reruns16 the groups rerun in the
+ * graph and after it. */
+int tal_bsw_set_plan_rounds(tal_bsw *h, int32_t rounds);
 
 /* Autotuner (no reference counterpart).  Times candidate launch
  * configurations -- waves per CU, which sets how many DP state columns each

@@ -304,6 +304,10 @@ static void dp_group16(bsw_oracle *o, uint16_t nrow, uint16_t ncol,
                 const int out1 = (pj > tail[l]) || left;
                 const int16_t hst = out1 ? 0 : h10[l];
                 if (out1) f21 = 0;
+                /* mutation 8: the F of column tail (one past the lane's row-max
+                 * band) is not stored, as the scalar path stores eh[end].e = 0
+                 * there (:201) */
+                if (o->mutation == 8 && pj == tail[l]) f21 = 0;
                 /* row max and its column, :3158-3169 */
                 const int16_t b = maxRS[l];
                 const int16_t mr = smax16(b, h11);
@@ -915,14 +919,13 @@ static void fill_mat(const bsw_oracle *o, int8_t mat[25])
     for (int j = 0; j < 5; j++) mat[k++] = o->w_ambig;
 }
 
-/* scalarBandedSWA restated, bandedSWA.cpp:116-237. */
-static int scalar_one(const bsw_oracle *o, int qlen, const uint8_t *query,
+/* scalarBandedSWA restated, bandedSWA.cpp:116-237, with the scoring matrix
+ * mat (the one the reference object was constructed with). */
+static int scalar_one(const bsw_oracle *o, const int8_t *mat, int qlen, const uint8_t *query,
                       int tlen, const uint8_t *target, int32_t w, int h0,
                       int *_qle, int *_tle, int *_gtle, int *_gscore,
                       int *_max_off)
 {
-    int8_t mat[25];
-    fill_mat(o, mat);
     const int oe_del = o->o_del + o->e_del, oe_ins = o->o_ins + o->e_ins;
     int *eh_h = (int *)calloc((size_t)qlen + 1, sizeof(int));
     int *eh_e = (int *)calloc((size_t)qlen + 1, sizeof(int));
@@ -1011,12 +1014,25 @@ static int scalar_one(const bsw_oracle *o, int qlen, const uint8_t *query,
     return max;
 }
 
-void bsw_oracle_scalar(bsw_oracle *o, oracle_seqpair *pairs, const uint8_t *ref,
-                       const uint8_t *qer, int32_t n, int32_t w)
+void bsw_oracle_scalar_mat(bsw_oracle *o, const int8_t *mat, oracle_seqpair *pairs,
+                           const uint8_t *ref, const uint8_t *qer, int32_t n, int32_t w)
 {
     for (int32_t i = 0; i < n; i++) {
         oracle_seqpair *p = pairs + i;
-        p->score = scalar_one(o, p->len2, qer + p->idq, p->len1, ref + p->idr, w,
+        p->score = scalar_one(o, mat, p->len2, qer + p->idq, p->len1, ref + p->idr, w,
+                              p->h0, &p->qle, &p->tle, &p->gtle, &p->gscore,
+                              &p->max_off);
+    }
+}
+
+void bsw_oracle_scalar(bsw_oracle *o, oracle_seqpair *pairs, const uint8_t *ref,
+                       const uint8_t *qer, int32_t n, int32_t w)
+{
+    int8_t mat[25];
+    fill_mat(o, mat);
+    for (int32_t i = 0; i < n; i++) {
+        oracle_seqpair *p = pairs + i;
+        p->score = scalar_one(o, mat, p->len2, qer + p->idq, p->len1, ref + p->idr, w,
                               p->h0, &p->qle, &p->tle, &p->gtle, &p->gscore,
                               &p->max_off);
     }

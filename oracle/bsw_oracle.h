@@ -65,6 +65,12 @@ void bsw_oracle_scalar(bsw_oracle *o, oracle_seqpair *pairs,
                        const uint8_t *ref, const uint8_t *qer,
                        int32_t n, int32_t w);
 
+/* The same with a caller-given 5x5 matrix, mat[r * 5 + q] (reference code r,
+ * query code q), as the reference's scalar path uses the matrix its object
+ * was constructed with (bandedSWA.cpp:53, :138-141). */
+void bsw_oracle_scalar_mat(bsw_oracle *o, const int8_t *mat, oracle_seqpair *pairs,
+                           const uint8_t *ref, const uint8_t *qer, int32_t n, int32_t w);
+
 /* Full-band DP on one pair: every cell with |i-j| inside the per-pair band is
  * evaluated, no beg/end narrowing.  Returns score, writes the other outputs. */
 int bsw_oracle_band_dp(const bsw_oracle *o, int qlen, const uint8_t *query,
@@ -80,7 +86,8 @@ void bsw_oracle_set_stale_mode(bsw_oracle *o, int mode);
  * 1 strict row-max tie rule, 2 z-drop offset sign, 3 ins/del penalties
  * swapped, 4 first-column boundary one row early, 5 ambiguity ignored,
  * 6 gscore not gated by lane exit, 7 16-bit range reset before the head/tail
- * scans dropped.  0 restores the restatement. */
+ * scans dropped, 8 16-bit F of the column one past a lane's band (j == tail)
+ * stored as 0.  0 restores the restatement. */
 void bsw_oracle_set_mutation(bsw_oracle *o, int m);
 /* Counters since creation: [0] groups, [1] stale F[ncol] reads in which the
  * stale value decided some lane's tail, [2] such reads where a lane's stale

@@ -36,6 +36,7 @@ def lib():
         for name in ("bsw_oracle_get_scores16", "bsw_oracle_get_scores8",
                      "bsw_oracle_scalar"):
             getattr(L, name).argtypes = [P, P, P, P, i32, i32]
+        L.bsw_oracle_scalar_mat.argtypes = [P, P, P, P, P, i32, i32]
         L.bsw_oracle_band_dp.restype = ctypes.c_int
         L.bsw_oracle_band_dp.argtypes = [P, ctypes.c_int, P, ctypes.c_int, P,
                                          i32, ctypes.c_int] + [P] * 5
@@ -91,6 +92,16 @@ class Oracle:
 
     def scalar(self, pairs, ref, qer, w):
         return self._run(lib().bsw_oracle_scalar, pairs, ref, qer, w)
+
+    def scalar_mat(self, mat, pairs, ref, qer, w):
+        """The scalar routine with a caller-given 5x5 matrix (mat[r*5+q])."""
+        m = np.ascontiguousarray(mat, dtype=np.int8).reshape(25)
+        assert pairs.dtype == SEQPAIR_DTYPE and pairs.flags.c_contiguous
+        ref = np.ascontiguousarray(ref, dtype=np.uint8)
+        qer = np.ascontiguousarray(qer, dtype=np.uint8)
+        lib().bsw_oracle_scalar_mat(self._h, _ptr(m), _ptr(pairs), _ptr(ref), _ptr(qer),
+                                    len(pairs), int(w))
+        return pairs
 
     def band_dp(self, query, target, w, h0):
         q = np.ascontiguousarray(query, dtype=np.uint8)

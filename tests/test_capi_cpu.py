@@ -81,6 +81,35 @@ def test_cpp_shim_compiles(tmp_path):
                            str(src)])
 
 
+def test_bsw_shim_compiles_reference_call_pattern(tmp_path):
+    """bench-bsw's calls (benchmarks/bench-bsw.cpp:213-218, 253, 256-258),
+    both branches, and scalarBandedSWA's signature (bandedSWA.h:124-129)
+    compile against the shim unchanged."""
+    src = tmp_path / "bsw_caller.cpp"
+    src.write_text("""
+#include "bandedSWA_tal.hpp"
+void bwa_fill_scmat(int a, int b, int ambig, int8_t mat[25]);
+void run(SeqPair *seqPairArray, uint8_t *seqBufRef, uint8_t *seqBufQer, int numPairs) {
+    int8_t mat[25];
+    bwa_fill_scmat(1, 4, -1, mat);
+    int zdrop = 100, w = 100, end_bonus = 5, w_open = 6, w_extend = 1;
+    BandedPairWiseSW *bsw = new BandedPairWiseSW(w_open, w_extend, w_open, w_extend,
+                                                 zdrop, end_bonus, mat, 1, 4, 1);
+    bsw->getScores16(seqPairArray, seqBufRef, seqBufQer, numPairs, 1, w);
+    bsw->getScores8(seqPairArray, seqBufRef, seqBufQer, numPairs, 1, w);
+    bsw->scalarBandedSWAWrapper(seqPairArray, seqBufRef, seqBufQer, numPairs, 1, w);
+    SeqPair *p = seqPairArray;
+    int qle, tle, gtle, gscore, max_off;
+    int score = bsw->scalarBandedSWA(p->len2, seqBufQer + p->idq, p->len1, seqBufRef + p->idr,
+                                     w, p->h0, &qle, &tle, &gtle, &gscore, &max_off);
+    (void)score;
+    delete bsw;
+}
+""")
+    subprocess.check_call(["g++", "-std=c++17", "-Wall", "-Wshadow", "-Werror", "-fsyntax-only",
+                           "-I", os.path.join(ROOT, "include"), str(src)])
+
+
 def test_pairwisesw_shim_compiles_reference_call_pattern(tmp_path):
     # the calls bench-smith-waterman.cpp:203-223 makes, unchanged
     src = tmp_path / "pwsw.cpp"

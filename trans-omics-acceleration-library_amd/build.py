@@ -13,7 +13,8 @@ CSRC = os.path.join(HERE, "csrc")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("TAL_BSW_ARCH", "gfx950")
 
-HIP_SOURCES = ["bsw_kernels.hip", "tal_bsw.hip", "bsw_server.hip", "bsw_pairfile.cpp"]
+HIP_SOURCES = ["bsw_kernels.hip", "bsw_scalar.hip", "tal_bsw.hip", "bsw_server.hip",
+               "bsw_pairfile.cpp"]
 SW_SOURCES = ["sw_kernels.hip", "tal_sw.hip"]
 # per-source flags: the SW cell update is four independent packed-16-bit
 # chains per step; the ILP scheduler interleaves them, where the default one

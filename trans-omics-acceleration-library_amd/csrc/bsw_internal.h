@@ -61,6 +61,21 @@ struct KBuffers {
     const int32_t *abort_flag;    /* nonzero: the batch left its plan; do nothing */
 };
 
+/* Parameters of the per-pair (scalar) kernel: the object's gaps and the
+ * caller's 5x5 matrix (row = reference code, column = query code). */
+struct ScalarParams {
+    int32_t n, w, max_qlen;
+    int32_t o_del, e_del, o_ins, e_ins, zdrop, end_bonus;
+    int32_t mat_max; /* max(0, every entry of mat) */
+    int8_t mat[28];
+};
+
+hipError_t launch_scalar_cost(const tal_seqpair *pairs, int32_t n, int32_t w, uint32_t *key,
+                              int32_t *ids, int32_t *maxlen, hipStream_t s);
+hipError_t launch_scalar(int grid, const ScalarParams &P, tal_seqpair *pairs, const uint8_t *ref,
+                         const uint8_t *qer, const int32_t *order, int32_t ntasks,
+                         int32_t *counter, int2 *slabs, int32_t *err, hipStream_t s);
+
 /* Per-group shape: rows (max len1) and columns (max len2) of the group. */
 struct GroupShape {
     int32_t nrow, ncol;

@@ -10,12 +10,15 @@
  * -maxi each pair's outputs go to stderr in the format of the reference's
  * MAXI prints (bandedSWA.cpp:2676-2686): "score (tle qle) gscore max_off gtle".
  *
- *   tal-bench-bsw -pairs FILE [-match 1] [-mismatch 4] [-gapo 6] [-gape 1]
- *                 [-w 100] [-zdrop 100] [-end_bonus 5] [-auto] [-maxi]
- *                 [-ref_slot 256] [-qer_slot 128]
+ *   tal-bench-bsw -pairs FILE [-match 1] [-mismatch 4] [-ambig -1] [-gapo 6]
+ *                 [-gape 1] [-w 100] [-zdrop 100] [-end_bonus 5] [-auto]
+ *                 [-scalar] [-maxi] [-ref_slot 256] [-qer_slot 128]
  *
  * -auto scores each chunk with the 8/16-bit dispatcher instead of
- * getScores16; -ref_slot/-qer_slot widen the per-pair slots (a slot of S
+ * getScores16; -scalar with scalarBandedSWAWrapper, bench-bsw's branch for
+ * builds without AVX2 (:256-258), whose matrix comes from bwa_fill_scmat of
+ * -match/-mismatch/-ambig (:56-64, :213); -ref_slot/-qer_slot widen the
+ * per-pair slots (a slot of S
  * bytes holds S-2 bases) for inputs longer than bench-bsw's.  A summary
  * (pairs, seconds, band cells, GCUPS) goes to stdout.
  */
@@ -62,14 +65,24 @@ long long band_cells(int len1, int len2, int w, int match, int mismatch, int o_i
     return c;
 }
 
+/* bench-bsw's scoring matrix (bwa_fill_scmat, bench-bsw.cpp:56-64): match on
+ * the diagonal of codes 0-3, -mismatch off it, ambig in row and column 4 */
+void fill_scmat(int a, int b, int ambig, int8_t mat[25])
+{
+    for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++)
+            mat[i * 5 + j] = (int8_t)((i == 4 || j == 4) ? ambig : (i == j ? a : -b));
+}
+
 } // namespace
 
 int main(int argc, char **argv)
 {
     const char *path = nullptr;
-    int match = 1, mismatch = 4, gapo = 6, gape = 1, w = 100, zdrop = 100, end_bonus = 5;
+    int match = 1, mismatch = 4, ambig = -1, gapo = 6, gape = 1, w = 100, zdrop = 100,
+        end_bonus = 5;
     int ref_slot = REF_SLOT, qer_slot = QER_SLOT;
-    bool maxi = false, autodispatch = false;
+    bool maxi = false, autodispatch = false, scalar = false;
     for (int i = 1; i < argc; i++) {
         const char *a = argv[i];
         const bool has = i + 1 < argc;
@@ -85,7 +98,9 @@ int main(int argc, char **argv)
         else if (!strcmp(a, "-qer_slot") && has) qer_slot = atoi(argv[++i]);
         else if (!strcmp(a, "-maxi")) maxi = true;
         else if (!strcmp(a, "-auto")) autodispatch = true;
-        else if (!strcmp(a, "-h0") || !strcmp(a, "-ambig")) i++; /* accepted, unused as in bench-bsw */
+        else if (!strcmp(a, "-scalar")) scalar = true;
+        else if (!strcmp(a, "-ambig") && has) ambig = atoi(argv[++i]);
+        else if (!strcmp(a, "-h0")) i++; /* accepted, unused as in bench-bsw */
         else {
             fprintf(stderr, "unknown argument %s\n", a);
             return 2;
@@ -93,7 +108,7 @@ int main(int argc, char **argv)
     }
     if (!path) {
         printf("usage: %s -pairs <InSeqFile> [-match m] [-mismatch x] [-gapo o] [-gape e] "
-               "[-w w] [-auto] [-maxi]\n", argv[0]);
+               "[-w w] [-auto] [-scalar] [-maxi]\n", argv[0]);
         return 2;
     }
     if (ref_slot < 3 || ref_slot > MAX_SLOT || qer_slot < 3 || qer_slot > MAX_SLOT) {
@@ -111,7 +126,9 @@ int main(int argc, char **argv)
     double secs = 0;
     int rc = 0;
     try {
-        BandedPairWiseSW bsw(gapo, gape, gapo, gape, zdrop, end_bonus, nullptr, (int8_t)match,
+        int8_t mat[25];
+        fill_scmat(match, mismatch, ambig, mat);
+        BandedPairWiseSW bsw(gapo, gape, gapo, gape, zdrop, end_bonus, mat, (int8_t)match,
                              (int8_t)mismatch, 1);
         for (;;) {
             int32_t n = 0;
@@ -124,7 +141,9 @@ int main(int argc, char **argv)
             }
             if (n == 0) break;
             const auto t0 = std::chrono::steady_clock::now();
-            if (autodispatch)
+            if (scalar)
+                bsw.scalarBandedSWAWrapper(pairs.data(), ref.data(), qer.data(), n, 1, w);
+            else if (autodispatch)
                 bsw.getScoresAuto(pairs.data(), ref.data(), qer.data(), n, w);
             else
                 bsw.getScores16(pairs.data(), ref.data(), qer.data(), n, 1, w);

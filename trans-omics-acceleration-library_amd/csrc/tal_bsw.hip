@@ -104,12 +104,15 @@ struct tal_bsw {
     size_t lds_per_cu = 0;
     int waves_per_cu = 8;  /* target waves per CU for the LDS/global split */
     int order = 1;         /* task order: 0 = input order, 1 = longest groups first */
+    int plan_rounds = 2;   /* stale-column rerun rounds captured in a plan's graph */
     DevBuf image; /* IMAGE_COLS u32: F[c] nonzero lanes of the 16-bit scratch */
     DevBuf zero_image; /* the image of a freshly constructed object; its first
                           bytes double as the kernels' zero block */
     PathBufs b16, b8;
     /* dispatcher scratch */
     DevBuf flag, idx_all, idx_sel, nsel, pairs8, pairs16, cubtmp;
+    /* scalar path scratch */
+    DevBuf sc_key, sc_key2, sc_ids, sc_order, sc_ctl, sc_slabs;
     /* host-API staging */
     DevBuf d_pairs, d_ref, d_qer;
     /* per-call timing */
@@ -263,6 +266,9 @@ __global__ void tasks_from_count(const int32_t *count, int32_t gpw, int32_t *nta
 {
     *ntasks = (*count + gpw - 1) / gpw;
 }
+
+/* Adds a rerun round's group count to the replay's total. */
+__global__ void add_count(const int32_t *count, int32_t *total) { *total += *count; }
 
 /* Control word of a replayed batch: skip the in-graph image update when
  * reruns are still pending or the batch left the plan's bounds. */
@@ -545,7 +551,7 @@ struct tal_bsw_plan {
     int rounds = 2; /* rerun rounds inside the graph */
     WidthPlan p8, p16;
     /* ctl: [0] error, [1] 8-bit padding len2, [2] rerun tasks, [3] skip,
-     * [4] pending reruns after the graph */
+     * [4] pending reruns after the graph, [5] groups rerun inside the graph */
     DevBuf blk, sel8, sel16, pairs8, pairs16, ctl;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -698,6 +704,7 @@ int plan_enqueue(tal_bsw *h, tal_bsw_plan *P, hipStream_t s)
                "resolve");
         const int cap = (G + 1) / 2;
         for (int r = 0; r < P->rounds; r++) {
+            hipLaunchKernelGGL(add_count, dim3(1), dim3(1), 0, s, cnt, ctl + 5);
             hipLaunchKernelGGL(make_tasks, dim3((cap + 255) / 256), dim3(256), 0, s, list, cnt,
                                W.pb.tasks.as<int32_t>(), cap, 2);
             hipLaunchKernelGGL(tasks_from_count, dim3(1), dim3(1), 0, s, cnt, 2, ctl + 2);
@@ -802,7 +809,8 @@ int check_pairs(tal_bsw *h, const tal_seqpair *pairs, int32_t n, size_t ref_len,
 }
 
 int host_call(tal_bsw *h, int mode, tal_seqpair *pairs, const uint8_t *ref, size_t ref_len,
-              const uint8_t *qer, size_t qer_len, int32_t n, int32_t w)
+              const uint8_t *qer, size_t qer_len, int32_t n, int32_t w,
+              const int8_t *mat = nullptr)
 {
     int rc = check_common(h, pairs, n, w);
     if (rc) return rc;
@@ -828,7 +836,10 @@ int host_call(tal_bsw *h, int mode, tal_seqpair *pairs, const uint8_t *ref, size
         ~InCall() { h->in_call = false; }
     } guard{h};
     h->in_call = true;
-    if (mode == 16)
+    if (mode == 1)
+        rc = tal_bsw_scalar_scores_device(h, mat, dp, h->d_ref.as<uint8_t>(),
+                                          h->d_qer.as<uint8_t>(), n, w, s);
+    else if (mode == 16)
         rc = tal_bsw_get_scores16_device(h, dp, h->d_ref.as<uint8_t>(), h->d_qer.as<uint8_t>(), n,
                                          w, s);
     else if (mode == 8)
@@ -1083,6 +1094,7 @@ int tal_bsw_plan_create(tal_bsw *h, tal_seqpair *pairs, const uint8_t *ref, cons
     P->qer = qer;
     P->n = n;
     P->w = w;
+    P->rounds = h->plan_rounds;
     struct Guard {
         tal_bsw_plan *P;
         bool keep = false;
@@ -1175,6 +1187,7 @@ int tal_bsw_plan_launch(tal_bsw_plan *P, void *stream)
     h->st.pairs16 = P->n16;
     h->st.groups8 = P->p8.G;
     h->st.groups16 = P->p16.G;
+    h->st.reruns16 = ctl[5];
     if (ctl[0] != 0)
         return fail(h, TAL_ERANGE, "the batch no longer fits its plan (dispatcher split or "
                                    "group columns changed); build a new plan");
@@ -1189,6 +1202,14 @@ void tal_bsw_plan_destroy(tal_bsw_plan *P)
     (void)hipStreamSynchronize(P->h->stream);
     plan_release(P);
     delete P;
+}
+
+int tal_bsw_set_plan_rounds(tal_bsw *h, int32_t rounds)
+{
+    if (!h) return TAL_EINVAL;
+    if (rounds < 0 || rounds > 8) return fail(h, TAL_EINVAL, "plan rounds must be 0..8");
+    h->plan_rounds = rounds;
+    return TAL_OK;
 }
 
 int tal_bsw_set_tuning(tal_bsw *h, int32_t waves_per_cu, int32_t order)
@@ -1263,6 +1284,87 @@ int tal_bsw_autotune(tal_bsw *h, const tal_seqpair *pairs, const uint8_t *ref,
     h->order = res.order;
     *out = res;
     return TAL_OK;
+}
+
+int tal_bsw_scalar_scores_device(tal_bsw *h, const int8_t *mat, tal_seqpair *pairs,
+                                 const uint8_t *ref, const uint8_t *qer, int32_t n, int32_t w,
+                                 void *stream)
+{
+    int rc = check_common(h, pairs, n, w);
+    if (rc) return rc;
+    if (!mat) return fail(h, TAL_EINVAL, "null scoring matrix");
+    if (n == 0) return TAL_OK;
+    HIPCHK(h, hipSetDevice(h->device), "set device");
+    hipStream_t s = stream ? (hipStream_t)stream : h->stream;
+    ScalarParams P{};
+    P.n = n;
+    P.w = w;
+    P.o_del = h->prm.o_del;
+    P.e_del = h->prm.e_del;
+    P.o_ins = h->prm.o_ins;
+    P.e_ins = h->prm.e_ins;
+    P.zdrop = h->prm.zdrop;
+    P.end_bonus = h->prm.end_bonus;
+    P.mat_max = 0; /* :148-150 */
+    for (int k = 0; k < 25; k++) {
+        P.mat[k] = mat[k];
+        P.mat_max = std::max(P.mat_max, (int)mat[k]);
+    }
+    const int ntasks = (n + 63) / 64;
+    HIPCHK(h, h->sc_key.ensure(sizeof(uint32_t) * n), "alloc keys");
+    HIPCHK(h, h->sc_key2.ensure(sizeof(uint32_t) * n), "alloc keys");
+    HIPCHK(h, h->sc_ids.ensure(sizeof(int32_t) * n), "alloc ids");
+    HIPCHK(h, h->sc_order.ensure(sizeof(int32_t) * (size_t)ntasks * 64), "alloc order");
+    HIPCHK(h, h->sc_ctl.ensure(sizeof(int32_t) * 4), "alloc control");
+    int32_t *ctl = h->sc_ctl.as<int32_t>(); /* [0] task counter, [1] max length, [2] error */
+    HIPCHK(h, hipMemsetAsync(ctl, 0, sizeof(int32_t) * 4, s), "clear control");
+    HIPCHK(h, launch_scalar_cost(pairs, n, w, h->sc_key.as<uint32_t>(), h->sc_ids.as<int32_t>(),
+                                 ctl + 1, s),
+           "scalar cost");
+    size_t tmp = 0;
+    HIPCHK(h, hipcub::DeviceRadixSort::SortPairsDescending(
+                  nullptr, tmp, h->sc_key.as<uint32_t>(), h->sc_key2.as<uint32_t>(),
+                  h->sc_ids.as<int32_t>(), h->sc_order.as<int32_t>(), n, 0, 32, s),
+           "sort size");
+    HIPCHK(h, h->cubtmp.ensure(tmp + 16), "alloc sort tmp");
+    HIPCHK(h, hipcub::DeviceRadixSort::SortPairsDescending(
+                  h->cubtmp.p, tmp, h->sc_key.as<uint32_t>(), h->sc_key2.as<uint32_t>(),
+                  h->sc_ids.as<int32_t>(), h->sc_order.as<int32_t>(), n, 0, 32, s),
+           "sort pairs");
+    if (ntasks * 64 > n) {
+        hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, s, h->sc_order.as<int32_t>() + n,
+                           ntasks * 64 - n, -1);
+        HIPCHK(h, hipGetLastError(), "pad order");
+    }
+    int32_t maxlen = 0;
+    HIPCHK(h, hipMemcpyAsync(&maxlen, ctl + 1, sizeof maxlen, hipMemcpyDeviceToHost, s),
+           "read lengths");
+    HIPCHK(h, hipStreamSynchronize(s), "sync lengths");
+    if (maxlen > MAX_LEN16)
+        return fail(h, TAL_ERANGE, "a pair length is negative or exceeds %d", MAX_LEN16);
+    P.max_qlen = maxlen;
+    /* one slab of (longest length + 2) columns x 64 lanes of (H, E) per wave;
+     * as many waves as fit in 1 GiB, at most 8 per CU */
+    const size_t slab = (size_t)(maxlen + 2) * 64 * sizeof(int2);
+    const long long fit = std::max<long long>(1, (1ll << 30) / (long long)slab);
+    const int grid = (int)std::max<long long>(
+        1, std::min<long long>({(long long)h->cu_count * 8, (long long)ntasks, fit}));
+    HIPCHK(h, h->sc_slabs.ensure(slab * grid), "alloc scalar slabs");
+    HIPCHK(h, launch_scalar(grid, P, pairs, ref, qer, h->sc_order.as<int32_t>(), ntasks, ctl,
+                            h->sc_slabs.as<int2>(), ctl + 2, s),
+           "scalar kernel");
+    int32_t err = 0;
+    HIPCHK(h, hipMemcpyAsync(&err, ctl + 2, sizeof err, hipMemcpyDeviceToHost, s), "read error");
+    HIPCHK(h, hipStreamSynchronize(s), "sync scalar");
+    if (err) return fail(h, TAL_ERANGE, "a sequence byte is not a base code 0-4");
+    return TAL_OK;
+}
+
+int tal_bsw_scalar_scores(tal_bsw *h, const int8_t *mat, tal_seqpair *pairs, const uint8_t *ref,
+                          size_t ref_len, const uint8_t *qer, size_t qer_len, int32_t n, int32_t w)
+{
+    if (h && !mat) return fail(h, TAL_EINVAL, "null scoring matrix");
+    return host_call(h, 1, pairs, ref, ref_len, qer, qer_len, n, w, mat);
 }
 
 int tal_bsw_get_stats(tal_bsw *h, tal_bsw_stats *out)

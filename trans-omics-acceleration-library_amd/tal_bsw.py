@@ -26,9 +26,10 @@ _STATUS = {1: "TAL_EINVAL", 2: "TAL_ENOMEM", 3: "TAL_EDEVICE", 4: "TAL_ERANGE", 
 EXPORTS = ["tal_bsw_create", "tal_bsw_destroy", "tal_bsw_last_error",
            "tal_bsw_get_scores16", "tal_bsw_get_scores8", "tal_bsw_get_scores_auto",
            "tal_bsw_get_scores16_device", "tal_bsw_get_scores8_device",
-           "tal_bsw_get_scores_auto_device", "tal_bsw_count_8bit", "tal_bsw_get_stats",
+           "tal_bsw_get_scores_auto_device", "tal_bsw_scalar_scores",
+           "tal_bsw_scalar_scores_device", "tal_bsw_count_8bit", "tal_bsw_get_stats",
            "tal_bsw_plan_create", "tal_bsw_plan_launch", "tal_bsw_plan_destroy",
-           "tal_bsw_autotune", "tal_bsw_set_tuning",
+           "tal_bsw_autotune", "tal_bsw_set_tuning", "tal_bsw_set_plan_rounds",
            "tal_bsw_server_create", "tal_bsw_server_submit", "tal_bsw_server_wait",
            "tal_bsw_server_get_stats", "tal_bsw_server_destroy",
            "tal_bsw_pairfile_open", "tal_bsw_pairfile_read", "tal_bsw_pairfile_close",
@@ -123,6 +124,12 @@ def load():
         L.tal_bsw_autotune.argtypes = [P, P, P, P, i32, i32, i32, ctypes.POINTER(Tuning)]
         L.tal_bsw_set_tuning.restype = ctypes.c_int
         L.tal_bsw_set_tuning.argtypes = [P, i32, i32]
+        L.tal_bsw_set_plan_rounds.restype = ctypes.c_int
+        L.tal_bsw_scalar_scores.restype = ctypes.c_int
+        L.tal_bsw_scalar_scores.argtypes = [P, P, P, P, sz, P, sz, i32, i32]
+        L.tal_bsw_scalar_scores_device.restype = ctypes.c_int
+        L.tal_bsw_scalar_scores_device.argtypes = [P, P, P, P, P, i32, i32, P]
+        L.tal_bsw_set_plan_rounds.argtypes = [P, i32]
         L.tal_bsw_server_create.restype = ctypes.c_int
         L.tal_bsw_server_create.argtypes = [ctypes.POINTER(Params), ctypes.c_int,
                                             ctypes.POINTER(ServerConfig), ctypes.POINTER(P)]
@@ -196,6 +203,26 @@ class BandedPairWiseSW:
     def get_scores_auto(self, pairs, ref, qer, w):
         return self._host(load().tal_bsw_get_scores_auto, pairs, ref, qer, w)
 
+    def scalar_scores(self, pairs, ref, qer, w, mat=None):
+        """scalarBandedSWAWrapper: every pair on its own, scored with the 5x5
+        matrix mat (default: bwa_fill_scmat of the handle's match/mismatch,
+        ambiguity -1)."""
+        m = fill_scmat(self.params.w_match, self.params.w_mismatch) if mat is None else mat
+        m = np.ascontiguousarray(m, dtype=np.int8).reshape(25)
+        assert pairs.dtype == SEQPAIR_DTYPE and pairs.flags.c_contiguous
+        ref = np.ascontiguousarray(ref, dtype=np.uint8)
+        qer = np.ascontiguousarray(qer, dtype=np.uint8)
+        self._check(load().tal_bsw_scalar_scores(self._h, _ptr(m), _ptr(pairs), _ptr(ref),
+                                                 ref.nbytes, _ptr(qer), qer.nbytes, len(pairs),
+                                                 int(w)))
+        return pairs
+
+    def scalar_scores_device(self, mat, d_pairs, d_ref, d_qer, n, w, stream=0):
+        m = np.ascontiguousarray(mat, dtype=np.int8).reshape(25)
+        self._check(load().tal_bsw_scalar_scores_device(
+            self._h, _ptr(m), ctypes.c_void_p(d_pairs), ctypes.c_void_p(d_ref),
+            ctypes.c_void_p(d_qer), int(n), int(w), ctypes.c_void_p(stream or None)))
+
     def get_scores_device(self, bits, d_pairs, d_ref, d_qer, n, w, stream=0):
         """Device pointers (ints); bits in {16, 8, 0 (auto)}."""
         fn = {16: load().tal_bsw_get_scores16_device, 8: load().tal_bsw_get_scores8_device,
@@ -229,6 +256,10 @@ class BandedPairWiseSW:
 
     def set_tuning(self, waves_per_cu, order):
         self._check(load().tal_bsw_set_tuning(self._h, int(waves_per_cu), int(order)))
+
+    def set_plan_rounds(self, rounds):
+        """Stale-column rerun rounds captured in the graph of later plans."""
+        self._check(load().tal_bsw_set_plan_rounds(self._h, int(rounds)))
 
 
 class Plan:
@@ -312,6 +343,17 @@ class Server:
 
     def __del__(self):
         self.close()
+
+
+def fill_scmat(match, mismatch, ambig=-1):
+    """The 5x5 matrix bench-bsw builds (bwa_fill_scmat, bench-bsw.cpp:56-64):
+    match on the diagonal of codes 0-3, -mismatch off it, ambig in row and
+    column 4."""
+    m = np.full((5, 5), ambig, np.int8)
+    for i in range(4):
+        for j in range(4):
+            m[i, j] = match if i == j else -mismatch
+    return m.reshape(25)
 
 
 def _smax(w_match, w_mismatch):
