@@ -24,6 +24,8 @@ IN_FIELDS = ["idr", "idq", "len1", "len2", "h0"]
 
 
 def _cases():
+    if not os.path.exists(MANIFEST):  # written by tests/golden/make_digests.py
+        return []
     with open(MANIFEST) as f:
         return json.load(f)["cases"]
 
