@@ -45,13 +45,15 @@ def _plan_twice(pairs, ref, qer, w, rounds=None, label=""):
         if rounds is not None:
             eng.set_plan_rounds(rounds)
         plan = eng.plan(d.ptr(0), d.ptr(1), d.ptr(2), len(pairs), w)
-        for call in range(2):
-            plan.launch()
-            stats.append(eng.stats())
-            got = d.download(0, pairs.copy())
-            want = oracle_auto(o, pairs, ref, qer, w)
-            assert_same_outputs(got, want, f"{label} launch {call}")
-        plan.close()
+        try:
+            for call in range(2):
+                plan.launch()
+                stats.append(eng.stats())
+                got = d.download(0, pairs.copy())
+                want = oracle_auto(o, pairs, ref, qer, w)
+                assert_same_outputs(got, want, f"{label} launch {call}")
+        finally:
+            plan.close()
     finally:
         eng.close()
         d.close()

@@ -775,6 +775,10 @@ int plan_finish(tal_bsw *h, tal_bsw_plan *P, hipStream_t s, int32_t pending)
                                   W.pb.last.as<int32_t>(), W.B.fmask, W.B.fmask_off,
                                   h->image.as<uint32_t>(), s),
            "image update");
+    /* the graph scattered the 16-bit results before these reruns */
+    HIPCHK(h, launch_scatter_out(P->pairs16.as<tal_seqpair>(), P->sel16.as<int32_t>(), P->n16,
+                                 P->pairs, s),
+           "scatter16");
     return TAL_OK;
 }
 

@@ -8,6 +8,7 @@ library must be built (build.py); there is no CPU fallback.
 """
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -173,9 +174,13 @@ class BandedPairWiseSW:
         if rc != TAL_OK:
             raise TalError(rc, "tal_bsw_create failed")
         self._h = h
+        self._plans = weakref.WeakSet()
 
     def close(self):
         if getattr(self, "_h", None):
+            # a plan uses its handle's buffers and stream: release plans first
+            for p in list(self._plans):
+                p.close()
             load().tal_bsw_destroy(self._h)
             self._h = None
 
@@ -240,7 +245,9 @@ class BandedPairWiseSW:
 
     def plan(self, d_pairs, d_ref, d_qer, n, w):
         """A replayable, graph-captured plan of a device batch (Plan)."""
-        return Plan(self, d_pairs, d_ref, d_qer, n, w)
+        p = Plan(self, d_pairs, d_ref, d_qer, n, w)
+        self._plans.add(p)
+        return p
 
     def autotune(self, d_pairs, d_ref, d_qer, n, w, sample=0):
         """Times the launch configurations on the batch (device pointers) and
