@@ -47,6 +47,10 @@ def lib():
         L.bsw_oracle_f16_nonzero_mask.argtypes = [P, ctypes.c_int]
         L.bsw_oracle_band_cells.restype = ctypes.c_uint64
         L.bsw_oracle_band_cells.argtypes = [P, ctypes.c_int, ctypes.c_int, i32]
+        L.bsw_simd_run_mt.restype = ctypes.c_int
+        L.bsw_simd_run_mt.argtypes = [ctypes.c_int] * 6 + [i8, i8, ctypes.c_int,
+                                      P, P, P, i32, i32, i32, ctypes.c_int]
+        L.bsw_simd_width.restype = ctypes.c_int
         L.bsw_oracle_run_mt.restype = ctypes.c_int
         L.bsw_oracle_run_mt.argtypes = [ctypes.c_int] * 6 + [i8, i8, ctypes.c_int,
                                         P, P, P, i32, i32, i32, ctypes.c_int]
@@ -141,6 +145,25 @@ def run_mt(pairs, ref, qer, w, bits=16, nthreads=1, chunk=20000, params=None):
     if rc != 0:
         raise MemoryError("bsw_oracle_run_mt failed")
     return pairs
+
+
+def simd_run_mt(pairs, ref, qer, w, bits=16, nthreads=1, chunk=20000, params=None):
+    """The vectorised restatement (bsw_simd.cpp, the CPU baseline): each
+    thread owns one object and takes contiguous chunks, as run_mt."""
+    p = params or (6, 1, 6, 1, 100, 5, 1, 4)
+    ref = np.ascontiguousarray(ref, dtype=np.uint8)
+    qer = np.ascontiguousarray(qer, dtype=np.uint8)
+    rc = lib().bsw_simd_run_mt(*p, int(bits), _ptr(pairs), _ptr(ref), _ptr(qer),
+                               len(pairs), int(w), int(chunk), int(nthreads))
+    if rc != 0:
+        raise MemoryError("bsw_simd_run_mt failed")
+    return pairs
+
+
+def simd_width():
+    """512 if the AVX-512BW build of the vectorised restatement runs here,
+    else 256 (AVX2)."""
+    return int(lib().bsw_simd_width())
 
 
 def make_pairs(refs, queries, h0s):
