@@ -128,6 +128,28 @@ int tal_bsw_get_scores_auto_device(tal_bsw *h, tal_seqpair *pairs,
                                    const uint8_t *ref, const uint8_t *qer,
                                    int32_t n, int32_t w, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code that must not call hipMalloc, tight serving loops): the same
+ * call as tal_bsw_get_scores_auto_device, with the same results, but every
+ * piece of call scratch comes from a static device array of
+ * TAL_BSW_STATIC_SCRATCH_BYTES that the library holds on each device until
+ * the process ends; the call makes no hipMalloc or hipFree.  The handle's own
+ * buffers from tal_bsw_create (the 16-bit F-scratch image) are used as in the
+ * other calls.  A call needs about 70 bytes per pair, plus per 16-bit group of
+ * 32 pairs 4 bytes per column of its longest query, plus one wave-state slab
+ * (about 256 bytes per column of the longest query); the kernels then run as
+ * many blocks as the rest of the array holds slabs for.  A call whose
+ * scratch does not fit returns TAL_ERANGE: before any work when the pairs
+ * alone do not fit, else possibly after the 8-bit pairs' outputs were written.
+ * All handles on one device share the one array, which keeps the last call's
+ * contents: calls that overlap on one device race and their results are
+ * undefined. */
+#define TAL_BSW_STATIC_SCRATCH_BYTES ((size_t)64 << 20)
+int tal_bsw_get_scores_auto_device_noalloc(tal_bsw *h, tal_seqpair *pairs,
+                                           const uint8_t *ref,
+                                           const uint8_t *qer, int32_t n,
+                                           int32_t w, void *stream);
+
 /* Replaces BandedPairWiseSW::scalarBandedSWAWrapper (bandedSWA.cpp:242-260),
  * and with n = 1 scalarBandedSWA (:116-237): the textbook per-pair extension,
  * each pair scored on its own (no group coupling; its own column range,

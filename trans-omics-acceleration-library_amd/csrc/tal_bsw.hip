@@ -14,11 +14,42 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
 
 #include "bsw_internal.h"
+
+/* Scratch of tal_bsw_get_scores_auto_device_noalloc.  Every device the
+ * library runs on holds its own copy, at its own address, until the process
+ * ends.  In host code `scratch` names a host-side stand-in, never the device
+ * memory: host code uses the per-device address that hipGetSymbolAddress
+ * gives (static_scratch below). */
+static __device__ __align__(256) uint8_t scratch[TAL_BSW_STATIC_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<uint8_t *> g_scratch_addr; /* per device; null = not looked up yet */
+
+/* This device's address of `scratch`, looked up once per device with that
+ * device current. */
+hipError_t static_scratch(int device, uint8_t **out)
+{
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if ((int)g_scratch_addr.size() <= device) g_scratch_addr.resize(device + 1, nullptr);
+    if (!g_scratch_addr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_addr[device] = (uint8_t *)p;
+    }
+    *out = g_scratch_addr[device];
+    return hipSuccess;
+}
+
+} // namespace
 
 namespace talbsw {
 hipError_t launch_main(int bits, bool lds, int grid, size_t lds_bytes, const KParams &K,
@@ -87,6 +118,13 @@ struct DevBuf {
     }
 };
 
+/* The scratch of a no-allocation call: consecutive 256-byte-aligned slices
+ * of the static device array, handed out in call order. */
+struct Arena {
+    uint8_t *base = nullptr;
+    size_t cap = 0, used = 0;
+};
+
 /* Per-width scratch of one call. */
 struct PathBufs {
     DevBuf shapes, ncp1, foff, dep, wrote, stale, fmask, rerun, tasks, counter, slabs, last,
@@ -110,11 +148,15 @@ struct tal_bsw {
                           bytes double as the kernels' zero block */
     PathBufs b16, b8;
     /* dispatcher scratch */
-    DevBuf flag, idx_all, idx_sel, nsel, pairs8, pairs16, cubtmp;
+    DevBuf flag, idx_all, idx_sel, nsel, pairs8, cubtmp;
     /* scalar path scratch */
     DevBuf sc_key, sc_key2, sc_ids, sc_order, sc_ctl, sc_slabs;
     /* host-API staging */
     DevBuf d_pairs, d_ref, d_qer;
+    /* this device's copy of the static scratch array, and the arena of the
+     * no-allocation call in progress (null in every other call) */
+    uint8_t *scratch = nullptr;
+    Arena *arena = nullptr;
     /* per-call timing */
     /* 16-bit start/end, 8-bit start/end, rerun start/end */
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -148,6 +190,32 @@ int hip_fail(tal_bsw *h, hipError_t e, const char *what)
     do {                                                                                           \
         hipError_t e_ = (expr);                                                                    \
         if (e_ != hipSuccess) return hip_fail((h), e_, (what));                                    \
+    } while (0)
+
+/* `bytes` of call scratch: the handle's buffer b, grown with hipMalloc, or,
+ * in a no-allocation call, the next slice of the static array (TAL_ERANGE
+ * when the array is exhausted; b is left alone). */
+template <class T> int take(tal_bsw *h, DevBuf &b, size_t bytes, const char *what, T **out)
+{
+    if (!h->arena) {
+        HIPCHK(h, b.ensure(bytes), what);
+        *out = (T *)b.p;
+        return TAL_OK;
+    }
+    Arena &a = *h->arena;
+    const size_t off = (a.used + 255) & ~(size_t)255;
+    if (off > a.cap || bytes > a.cap - off)
+        return fail(h, TAL_ERANGE, "%s: %zu bytes do not fit the %zu-byte static scratch (%zu in use)",
+                    what, bytes, a.cap, off);
+    a.used = off + bytes;
+    *out = (T *)(a.base + off);
+    return TAL_OK;
+}
+
+#define TAKE(h, buf, bytes, what, out)                                                             \
+    do {                                                                                           \
+        const int rc_ = take((h), (buf), (bytes), (what), (out));                                  \
+        if (rc_) return rc_;                                                                       \
     } while (0)
 
 int smax_of(const tal_bsw_params &p)
@@ -337,32 +405,33 @@ Sizing sizing(const tal_bsw *h, int max_ncol)
  * similar size, so that the persistent grid ends evenly and the two
  * half-waves of a wave run loops of similar length.  Any order gives the
  * same results; this one only changes the schedule.  Leaves the task list
- * (ntasks * GPW ids, -1 padded) in pb.order. */
+ * (ntasks * GPW ids, -1 padded) in *order_out. */
 int make_order(tal_bsw *h, PathBufs &pb, const GroupShape *shapes, int32_t G, int32_t w,
-               int ntasks, int GPW, hipStream_t s)
+               int ntasks, int GPW, hipStream_t s, int32_t **order_out)
 {
-    HIPCHK(h, pb.ckey.ensure(sizeof(uint32_t) * G), "alloc keys");
-    HIPCHK(h, pb.ckey2.ensure(sizeof(uint32_t) * G), "alloc keys");
-    HIPCHK(h, pb.gids.ensure(sizeof(int32_t) * G), "alloc ids");
-    HIPCHK(h, pb.order.ensure(sizeof(int32_t) * ((size_t)ntasks * GPW)), "alloc order");
-    hipLaunchKernelGGL(group_cost, dim3((G + 255) / 256), dim3(256), 0, s, shapes, G, w,
-                       pb.ckey.as<uint32_t>(), pb.gids.as<int32_t>());
+    uint32_t *ckey, *ckey2;
+    int32_t *gids, *order;
+    void *cubtmp;
+    TAKE(h, pb.ckey, sizeof(uint32_t) * G, "alloc keys", &ckey);
+    TAKE(h, pb.ckey2, sizeof(uint32_t) * G, "alloc keys", &ckey2);
+    TAKE(h, pb.gids, sizeof(int32_t) * G, "alloc ids", &gids);
+    TAKE(h, pb.order, sizeof(int32_t) * ((size_t)ntasks * GPW), "alloc order", &order);
+    hipLaunchKernelGGL(group_cost, dim3((G + 255) / 256), dim3(256), 0, s, shapes, G, w, ckey, gids);
     HIPCHK(h, hipGetLastError(), "group cost");
     size_t tmp = 0;
-    HIPCHK(h, hipcub::DeviceRadixSort::SortPairsDescending(
-                  nullptr, tmp, pb.ckey.as<uint32_t>(), pb.ckey2.as<uint32_t>(),
-                  pb.gids.as<int32_t>(), pb.order.as<int32_t>(), G, 0, 32, s),
+    HIPCHK(h, hipcub::DeviceRadixSort::SortPairsDescending(nullptr, tmp, ckey, ckey2, gids, order,
+                                                           G, 0, 32, s),
            "sort size");
-    HIPCHK(h, pb.cubtmp.ensure(tmp + 16), "alloc sort tmp");
-    HIPCHK(h, hipcub::DeviceRadixSort::SortPairsDescending(
-                  pb.cubtmp.p, tmp, pb.ckey.as<uint32_t>(), pb.ckey2.as<uint32_t>(),
-                  pb.gids.as<int32_t>(), pb.order.as<int32_t>(), G, 0, 32, s),
+    TAKE(h, pb.cubtmp, tmp + 16, "alloc sort tmp", &cubtmp);
+    HIPCHK(h, hipcub::DeviceRadixSort::SortPairsDescending(cubtmp, tmp, ckey, ckey2, gids, order,
+                                                           G, 0, 32, s),
            "sort groups");
     const int pad = ntasks * GPW - G;
     if (pad > 0) {
-        hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, s, pb.order.as<int32_t>() + G, pad, -1);
+        hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, s, order + G, pad, -1);
         HIPCHK(h, hipGetLastError(), "pad tasks");
     }
+    *order_out = order;
     return TAL_OK;
 }
 
@@ -387,33 +456,30 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
     K.ngroups = G;
     K.last_len2 = last_len2;
 
-    HIPCHK(h, pb.shapes.ensure(sizeof(GroupShape) * G), "alloc shapes");
-    HIPCHK(h, pb.summary.ensure(sizeof(ShapeSummary)), "alloc summary");
-    GroupShape *shapes = pb.shapes.as<GroupShape>();
+    GroupShape *shapes;
+    ShapeSummary *summary;
+    int64_t *ncp1 = nullptr, *foff = nullptr;
+    TAKE(h, pb.shapes, sizeof(GroupShape) * G, "alloc shapes", &shapes);
+    TAKE(h, pb.summary, sizeof(ShapeSummary), "alloc summary", &summary);
     if (bits == 16) {
-        HIPCHK(h, pb.ncp1.ensure(sizeof(int64_t) * G), "alloc ncol");
-        HIPCHK(h, pb.foff.ensure(sizeof(int64_t) * G), "alloc offsets");
-        HIPCHK(h, launch_shapes(16, pairs, n, G, 0, shapes, pb.ncp1.as<int64_t>(), s), "shapes");
+        TAKE(h, pb.ncp1, sizeof(int64_t) * G, "alloc ncol", &ncp1);
+        TAKE(h, pb.foff, sizeof(int64_t) * G, "alloc offsets", &foff);
+        HIPCHK(h, launch_shapes(16, pairs, n, G, 0, shapes, ncp1, s), "shapes");
         size_t tmp = 0;
-        HIPCHK(h, hipcub::DeviceScan::InclusiveSum(nullptr, tmp, pb.ncp1.as<int64_t>(),
-                                                   pb.ncp1.as<int64_t>(), G, s),
-               "scan size");
-        HIPCHK(h, pb.cubtmp.ensure(tmp + 16), "alloc scan tmp");
-        HIPCHK(h, hipcub::DeviceScan::InclusiveSum(pb.cubtmp.p, tmp, pb.ncp1.as<int64_t>(),
-                                                   pb.ncp1.as<int64_t>(), G, s),
-               "scan");
-        hipLaunchKernelGGL(summarize, dim3(1), dim3(256), 0, s, shapes, pb.ncp1.as<int64_t>(), G,
-                           pb.summary.as<ShapeSummary>());
-        hipLaunchKernelGGL(to_exclusive, dim3((G + 255) / 256), dim3(256), 0, s,
-                           pb.ncp1.as<int64_t>(), pb.foff.as<int64_t>(), G);
+        void *cubtmp;
+        HIPCHK(h, hipcub::DeviceScan::InclusiveSum(nullptr, tmp, ncp1, ncp1, G, s), "scan size");
+        TAKE(h, pb.cubtmp, tmp + 16, "alloc scan tmp", &cubtmp);
+        HIPCHK(h, hipcub::DeviceScan::InclusiveSum(cubtmp, tmp, ncp1, ncp1, G, s), "scan");
+        hipLaunchKernelGGL(summarize, dim3(1), dim3(256), 0, s, shapes, ncp1, G, summary);
+        hipLaunchKernelGGL(to_exclusive, dim3((G + 255) / 256), dim3(256), 0, s, ncp1, foff, G);
     } else {
         HIPCHK(h, launch_shapes(8, pairs, n, G, last_len2, shapes, nullptr, s), "shapes");
         hipLaunchKernelGGL(summarize, dim3(1), dim3(256), 0, s, shapes, (const int64_t *)nullptr,
-                           G, pb.summary.as<ShapeSummary>());
+                           G, summary);
     }
     HIPCHK(h, hipGetLastError(), "summary");
     ShapeSummary sum{};
-    HIPCHK(h, hipMemcpyAsync(&sum, pb.summary.p, sizeof sum, hipMemcpyDeviceToHost, s),
+    HIPCHK(h, hipMemcpyAsync(&sum, summary, sizeof sum, hipMemcpyDeviceToHost, s),
            "read summary");
     HIPCHK(h, hipStreamSynchronize(s), "sync summary");
     const int limit = bits == 16 ? MAX_LEN16 : MAX_LEN8;
@@ -440,31 +506,39 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
     B.ref = ref;
     B.qer = qer;
     if (h->order == 1 && G > 1) {
-        const int rc = make_order(h, pb, shapes, G, w, ntasks, GPW, s);
+        int32_t *order;
+        const int rc = make_order(h, pb, shapes, G, w, ntasks, GPW, s, &order);
         if (rc) return rc;
-        B.task_groups = pb.order.as<int32_t>();
+        B.task_groups = order;
     }
-    HIPCHK(h, pb.counter.ensure(sizeof(int32_t) * 4), "alloc counter");
-    B.task_counter = pb.counter.as<int32_t>();
-    HIPCHK(h, pb.slabs.ensure((size_t)grid * slab_cols * 64 * sizeof(uint32_t)),
-           "alloc state slabs");
-    B.slabs = pb.slabs.as<uint32_t>();
+    TAKE(h, pb.counter, sizeof(int32_t) * 4, "alloc counter", &B.task_counter);
     B.zero = h->zero_image.as<uint32_t>();
+    int32_t *list = nullptr, *tasks = nullptr, *last = nullptr;
+    uint32_t *stale = nullptr;
     if (bits == 16) {
-        HIPCHK(h, pb.dep.ensure(sizeof(uint32_t) * G), "alloc dep");
-        HIPCHK(h, pb.wrote.ensure(G), "alloc wrote");
-        HIPCHK(h, pb.stale.ensure(sizeof(uint32_t) * G), "alloc stale");
-        HIPCHK(h, pb.fmask.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(total_cols, 1)),
-               "alloc fmask");
-        HIPCHK(h, pb.rerun.ensure(sizeof(int32_t) * (G + 1)), "alloc rerun");
-        HIPCHK(h, pb.tasks.ensure(sizeof(int32_t) * (size_t)(G + 2) * 2), "alloc tasks");
-        HIPCHK(h, hipMemsetAsync(pb.stale.p, 0, sizeof(uint32_t) * G, s), "clear stale");
-        B.stale_in = pb.stale.as<uint32_t>();
-        B.dep_out = pb.dep.as<uint32_t>();
-        B.wrote_out = pb.wrote.as<uint8_t>();
-        B.fmask = pb.fmask.as<uint32_t>();
-        B.fmask_off = pb.foff.as<int64_t>();
+        TAKE(h, pb.dep, sizeof(uint32_t) * G, "alloc dep", &B.dep_out);
+        TAKE(h, pb.wrote, G, "alloc wrote", &B.wrote_out);
+        TAKE(h, pb.stale, sizeof(uint32_t) * G, "alloc stale", &stale);
+        TAKE(h, pb.fmask, sizeof(uint32_t) * (size_t)std::max<int64_t>(total_cols, 1),
+             "alloc fmask", &B.fmask);
+        TAKE(h, pb.rerun, sizeof(int32_t) * (G + 1), "alloc rerun", &list);
+        TAKE(h, pb.tasks, sizeof(int32_t) * (size_t)(G + 2) * 2, "alloc tasks", &tasks);
+        if (!fresh) TAKE(h, pb.last, sizeof(int32_t) * IMAGE_COLS, "alloc last", &last);
+        HIPCHK(h, hipMemsetAsync(stale, 0, sizeof(uint32_t) * G, s), "clear stale");
+        B.stale_in = stale;
+        B.fmask_off = foff;
     }
+    /* The state slabs come last: a no-allocation call runs as many blocks as
+     * the rest of the static scratch holds slabs for (the grid is persistent,
+     * so any grid of at least one block scores every task). */
+    const size_t slab_bytes = (size_t)slab_cols * 64 * sizeof(uint32_t);
+    if (h->arena) {
+        const Arena &a = *h->arena;
+        const size_t off = (a.used + 255) & ~(size_t)255;
+        const size_t fit = off < a.cap ? (a.cap - off) / slab_bytes : 0;
+        grid = (int)std::min<size_t>((size_t)grid, std::max<size_t>(fit, 1));
+    }
+    TAKE(h, pb.slabs, (size_t)grid * slab_bytes, "alloc state slabs", &B.slabs);
     HIPCHK(h, hipMemsetAsync(B.task_counter, 0, sizeof(int32_t), s), "clear counter");
     hipEvent_t ev_beg = h->ev[bits == 16 ? 0 : 2], ev_end = h->ev[bits == 16 ? 1 : 3];
     HIPCHK(h, hipEventRecord(ev_beg, s), "event");
@@ -474,13 +548,12 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
     float ms_rerun = 0.f;
 
     if (bits == 16) {
-        int32_t *cnt = pb.counter.as<int32_t>() + 1;
-        int32_t *list = pb.rerun.as<int32_t>();
+        int32_t *cnt = B.task_counter + 1;
         for (int iter = 0;; iter++) {
             HIPCHK(h, hipMemsetAsync(cnt, 0, sizeof(int32_t), s), "clear rerun count");
             HIPCHK(h, launch_resolve(G, shapes, B.dep_out, B.wrote_out, B.fmask, B.fmask_off,
                                      fresh ? h->zero_image.as<uint32_t>() : h->image.as<uint32_t>(),
-                                     seg_first, pb.stale.as<uint32_t>(), list, cnt, s),
+                                     seg_first, stale, list, cnt, s),
                    "resolve");
             int32_t c = 0;
             HIPCHK(h, hipMemcpyAsync(&c, cnt, sizeof c, hipMemcpyDeviceToHost, s), "read reruns");
@@ -491,12 +564,12 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
             /* rerun the listed groups with their corrected F[ncol] masks */
             const int rt = (c + GPW - 1) / GPW;
             hipLaunchKernelGGL(make_tasks, dim3((rt + 255) / 256), dim3(256), 0, s, list, cnt,
-                               pb.tasks.as<int32_t>(), rt, GPW);
+                               tasks, rt, GPW);
             HIPCHK(h, hipGetLastError(), "make tasks");
             KParams K2 = K;
             K2.ntasks = rt;
             KBuffers B2 = B;
-            B2.task_groups = pb.tasks.as<int32_t>();
+            B2.task_groups = tasks;
             HIPCHK(h, hipMemsetAsync(B.task_counter, 0, sizeof(int32_t), s), "clear counter");
             const int g2 = grid_for(h, lds, lds_bytes, rt);
             HIPCHK(h, hipEventRecord(h->ev[4], s), "event");
@@ -514,12 +587,9 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
         h->st.reruns16 = reruns;
         h->st.lds_state16 = slab_cols == qwords ? 1 : 0;
         if (!fresh) {
-            HIPCHK(h, pb.last.ensure(sizeof(int32_t) * IMAGE_COLS), "alloc last");
-            HIPCHK(h, hipMemsetAsync(pb.last.p, 0xFF, sizeof(int32_t) * IMAGE_COLS, s),
-                   "clear last");
-            HIPCHK(h, launch_image_update(G, max_ncol + 1, shapes, B.wrote_out,
-                                          pb.last.as<int32_t>(), B.fmask, B.fmask_off,
-                                          h->image.as<uint32_t>(), s),
+            HIPCHK(h, hipMemsetAsync(last, 0xFF, sizeof(int32_t) * IMAGE_COLS, s), "clear last");
+            HIPCHK(h, launch_image_update(G, max_ncol + 1, shapes, B.wrote_out, last, B.fmask,
+                                          B.fmask_off, h->image.as<uint32_t>(), s),
                    "image update");
         }
     } else {
@@ -626,9 +696,10 @@ int plan_width(tal_bsw *h, tal_bsw_plan *P, int bits, hipStream_t s)
     B.ref = P->ref;
     B.qer = P->qer;
     if (h->order == 1 && G > 1) {
-        const int rc = make_order(h, pb, shapes, G, P->w, W.ntasks, GPW, s);
+        int32_t *order;
+        const int rc = make_order(h, pb, shapes, G, P->w, W.ntasks, GPW, s, &order);
         if (rc) return rc;
-        B.task_groups = pb.order.as<int32_t>();
+        B.task_groups = order;
     }
     HIPCHK(h, pb.counter.ensure(sizeof(int32_t) * 4), "alloc counter");
     B.task_counter = pb.counter.as<int32_t>();
@@ -920,6 +991,7 @@ int tal_bsw_create(const tal_bsw_params *params, int device, tal_bsw **out)
         bool evok = true;
         for (hipEvent_t &e : h->ev) evok = evok && hipEventCreate(&e) == hipSuccess;
         if (!evok) { rc = TAL_EDEVICE; break; }
+        if (static_scratch(device, &h->scratch) != hipSuccess) { rc = TAL_EDEVICE; break; }
         hipError_t e = h->image.ensure(sizeof(uint32_t) * IMAGE_COLS);
         if (e != hipSuccess) { rc = TAL_ENOMEM; break; }
         e = h->zero_image.ensure(sizeof(uint32_t) * IMAGE_COLS);
@@ -943,8 +1015,8 @@ void tal_bsw_destroy(tal_bsw *h)
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    DevBuf *all[] = {&h->image,   &h->zero_image, &h->flag, &h->idx_all, &h->idx_sel, &h->nsel,
-                     &h->pairs8,  &h->pairs16, &h->cubtmp,  &h->d_pairs, &h->d_ref,
+    DevBuf *all[] = {&h->image,  &h->zero_image, &h->flag,    &h->idx_all, &h->idx_sel,
+                     &h->nsel,   &h->pairs8,     &h->cubtmp,  &h->d_pairs, &h->d_ref,
                      &h->d_qer};
     for (DevBuf *b : all) b->release();
     for (PathBufs *pb : {&h->b16, &h->b8}) {
@@ -1014,41 +1086,39 @@ int tal_bsw_get_scores_auto_device(tal_bsw *h, tal_seqpair *pairs, const uint8_t
     if (n == 0) return TAL_OK;
     HIPCHK(h, hipSetDevice(h->device), "set device");
     hipStream_t s = stream ? (hipStream_t)stream : h->stream;
-    HIPCHK(h, h->flag.ensure(2 * (size_t)n), "alloc flags");
-    HIPCHK(h, h->idx_all.ensure(sizeof(int32_t) * n), "alloc index");
-    HIPCHK(h, h->idx_sel.ensure(sizeof(int32_t) * n), "alloc index");
-    HIPCHK(h, h->nsel.ensure(sizeof(int32_t) * 2), "alloc count");
-    HIPCHK(h, launch_flag8(pairs, n, smax_of(h->prm), h->flag.as<uint8_t>(), s), "flag8");
-    hipLaunchKernelGGL(iota32, dim3((n + 255) / 256), dim3(256), 0, s, h->idx_all.as<int32_t>(), n);
+    uint8_t *flag;
+    int32_t *all, *sel, *cnt;
+    void *cubtmp;
+    tal_seqpair *pairs8, *pairs16;
+    TAKE(h, h->flag, 2 * (size_t)n, "alloc flags", &flag);
+    TAKE(h, h->idx_all, sizeof(int32_t) * n, "alloc index", &all);
+    TAKE(h, h->idx_sel, sizeof(int32_t) * n, "alloc index", &sel);
+    TAKE(h, h->nsel, sizeof(int32_t) * 2, "alloc count", &cnt);
+    HIPCHK(h, launch_flag8(pairs, n, smax_of(h->prm), flag, s), "flag8");
+    hipLaunchKernelGGL(iota32, dim3((n + 255) / 256), dim3(256), 0, s, all, n);
     HIPCHK(h, hipGetLastError(), "iota");
     /* two stable selections: 8-bit pair indices, then 16-bit ones, each in
-     * input order */
-    int32_t *sel = h->idx_sel.as<int32_t>();
-    int32_t *cnt = h->nsel.as<int32_t>();
-    size_t tmp = 0;
-    HIPCHK(h, hipcub::DeviceSelect::Flagged(nullptr, tmp, h->idx_all.as<int32_t>(),
-                                            h->flag.as<uint8_t>(), sel, cnt, n, s),
+     * input order (one temporary, sized for the larger of the two) */
+    size_t tmp = 0, tmp2 = 0;
+    HIPCHK(h, hipcub::DeviceSelect::Flagged(nullptr, tmp, all, flag, sel, cnt, n, s),
            "select size");
-    HIPCHK(h, h->cubtmp.ensure(tmp + 16), "alloc select tmp");
-    HIPCHK(h, hipcub::DeviceSelect::Flagged(h->cubtmp.p, tmp, h->idx_all.as<int32_t>(),
-                                            h->flag.as<uint8_t>(), sel, cnt, n, s),
+    HIPCHK(h, hipcub::DeviceSelect::Flagged(nullptr, tmp2, all, flag + n, sel, cnt + 1, n, s),
+           "select size");
+    tmp = std::max(tmp, tmp2);
+    TAKE(h, h->cubtmp, tmp + 16, "alloc select tmp", &cubtmp);
+    TAKE(h, h->pairs8, sizeof(tal_seqpair) * n, "alloc pairs8", &pairs8);
+    HIPCHK(h, hipcub::DeviceSelect::Flagged(cubtmp, tmp, all, flag, sel, cnt, n, s),
            "select 8-bit");
     int32_t n8 = 0;
     HIPCHK(h, hipMemcpyAsync(&n8, cnt, sizeof n8, hipMemcpyDeviceToHost, s), "read n8");
     HIPCHK(h, hipStreamSynchronize(s), "sync n8");
     const int32_t n16 = n - n8;
-    if (n16) {
-        size_t tmp2 = 0;
-        HIPCHK(h, hipcub::DeviceSelect::Flagged(nullptr, tmp2, h->idx_all.as<int32_t>(),
-                                                h->flag.as<uint8_t>() + n, sel + n8, cnt + 1, n,
-                                                s),
-               "select size");
-        HIPCHK(h, h->cubtmp.ensure(tmp2 + 16), "alloc select tmp");
-        HIPCHK(h, hipcub::DeviceSelect::Flagged(h->cubtmp.p, tmp2, h->idx_all.as<int32_t>(),
-                                                h->flag.as<uint8_t>() + n, sel + n8, cnt + 1, n,
+    if (n16)
+        HIPCHK(h, hipcub::DeviceSelect::Flagged(cubtmp, tmp, all, flag + n, sel + n8, cnt + 1, n,
                                                 s),
                "select 16-bit");
-    }
+    /* one buffer holds both halves: the 8-bit pairs, then the 16-bit ones */
+    pairs16 = pairs8 + n8;
     h->st.pairs8 = n8;
     h->st.pairs16 = n16;
     const bool outer = !h->in_call;
@@ -1061,23 +1131,45 @@ int tal_bsw_get_scores_auto_device(tal_bsw *h, tal_seqpair *pairs, const uint8_t
             if (outer) h->in_call = false;
         }
     } restore{h, outer};
-    HIPCHK(h, h->pairs8.ensure(sizeof(tal_seqpair) * std::max(n8, 1)), "alloc pairs8");
-    HIPCHK(h, h->pairs16.ensure(sizeof(tal_seqpair) * std::max(n16, 1)), "alloc pairs16");
+    /* In a no-allocation call the 8-bit path's scratch is free again once its
+     * work is queued: the 16-bit path's kernels run after it on the stream. */
+    const size_t mark = h->arena ? h->arena->used : 0;
     if (n8) {
-        HIPCHK(h, launch_gather(pairs, sel, n8, h->pairs8.as<tal_seqpair>(), s), "gather8");
-        rc = tal_bsw_get_scores8_device(h, h->pairs8.as<tal_seqpair>(), ref, qer, n8, w, s);
+        HIPCHK(h, launch_gather(pairs, sel, n8, pairs8, s), "gather8");
+        rc = tal_bsw_get_scores8_device(h, pairs8, ref, qer, n8, w, s);
         if (rc) return rc;
-        HIPCHK(h, launch_scatter_out(h->pairs8.as<tal_seqpair>(), sel, n8, pairs, s), "scatter8");
+        HIPCHK(h, launch_scatter_out(pairs8, sel, n8, pairs, s), "scatter8");
     }
+    if (h->arena) h->arena->used = mark;
     if (n16) {
-        HIPCHK(h, launch_gather(pairs, sel + n8, n16, h->pairs16.as<tal_seqpair>(), s),
-               "gather16");
-        rc = tal_bsw_get_scores16_device(h, h->pairs16.as<tal_seqpair>(), ref, qer, n16, w, s);
+        HIPCHK(h, launch_gather(pairs, sel + n8, n16, pairs16, s), "gather16");
+        rc = tal_bsw_get_scores16_device(h, pairs16, ref, qer, n16, w, s);
         if (rc) return rc;
-        HIPCHK(h, launch_scatter_out(h->pairs16.as<tal_seqpair>(), sel + n8, n16, pairs, s),
-               "scatter16");
+        HIPCHK(h, launch_scatter_out(pairs16, sel + n8, n16, pairs, s), "scatter16");
     }
     return TAL_OK;
+}
+
+int tal_bsw_get_scores_auto_device_noalloc(tal_bsw *h, tal_seqpair *pairs, const uint8_t *ref,
+                                           const uint8_t *qer, int32_t n, int32_t w, void *stream)
+{
+    if (!h) return TAL_EINVAL;
+    if (n < 0) return fail(h, TAL_EINVAL, "n < 0");
+    /* the dispatcher's own part (flags, indices, one copy of the pairs) must
+     * fit before anything is queued */
+    const size_t own = (size_t)n * (2 + 4 + 4 + sizeof(tal_seqpair)) + 4 * 256;
+    if (own > TAL_BSW_STATIC_SCRATCH_BYTES)
+        return fail(h, TAL_ERANGE, "%d pairs need more than the %zu-byte static scratch", n,
+                    (size_t)TAL_BSW_STATIC_SCRATCH_BYTES);
+    Arena a;
+    a.base = h->scratch;
+    a.cap = TAL_BSW_STATIC_SCRATCH_BYTES;
+    struct Bind {
+        tal_bsw *h;
+        ~Bind() { h->arena = nullptr; }
+    } bind{h};
+    h->arena = &a;
+    return tal_bsw_get_scores_auto_device(h, pairs, ref, qer, n, w, stream);
 }
 
 int tal_bsw_plan_create(tal_bsw *h, tal_seqpair *pairs, const uint8_t *ref, const uint8_t *qer,

@@ -27,7 +27,8 @@ _STATUS = {1: "TAL_EINVAL", 2: "TAL_ENOMEM", 3: "TAL_EDEVICE", 4: "TAL_ERANGE", 
 EXPORTS = ["tal_bsw_create", "tal_bsw_destroy", "tal_bsw_last_error",
            "tal_bsw_get_scores16", "tal_bsw_get_scores8", "tal_bsw_get_scores_auto",
            "tal_bsw_get_scores16_device", "tal_bsw_get_scores8_device",
-           "tal_bsw_get_scores_auto_device", "tal_bsw_scalar_scores",
+           "tal_bsw_get_scores_auto_device", "tal_bsw_get_scores_auto_device_noalloc",
+           "tal_bsw_scalar_scores",
            "tal_bsw_scalar_scores_device", "tal_bsw_count_8bit", "tal_bsw_get_stats",
            "tal_bsw_plan_create", "tal_bsw_plan_launch", "tal_bsw_plan_destroy",
            "tal_bsw_autotune", "tal_bsw_set_tuning", "tal_bsw_set_plan_rounds",
@@ -108,7 +109,7 @@ def load():
             f.restype = ctypes.c_int
             f.argtypes = [P, P, P, sz, P, sz, i32, i32]
         for name in ("tal_bsw_get_scores16_device", "tal_bsw_get_scores8_device",
-                     "tal_bsw_get_scores_auto_device"):
+                     "tal_bsw_get_scores_auto_device", "tal_bsw_get_scores_auto_device_noalloc"):
             f = getattr(L, name)
             f.restype = ctypes.c_int
             f.argtypes = [P, P, P, P, i32, i32, P]
@@ -234,6 +235,13 @@ class BandedPairWiseSW:
               0: load().tal_bsw_get_scores_auto_device}[bits]
         self._check(fn(self._h, ctypes.c_void_p(d_pairs), ctypes.c_void_p(d_ref),
                        ctypes.c_void_p(d_qer), int(n), int(w), ctypes.c_void_p(stream or None)))
+
+    def get_scores_device_noalloc(self, d_pairs, d_ref, d_qer, n, w, stream=0):
+        """The dispatcher on device pointers, with all call scratch taken from
+        the library's static device array (no device allocation)."""
+        self._check(load().tal_bsw_get_scores_auto_device_noalloc(
+            self._h, ctypes.c_void_p(d_pairs), ctypes.c_void_p(d_ref), ctypes.c_void_p(d_qer),
+            int(n), int(w), ctypes.c_void_p(stream or None)))
 
     def stats(self):
         st = Stats()
