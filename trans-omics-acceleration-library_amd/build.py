@@ -106,6 +106,29 @@ def build_extend(out=None):
                      libs=(f"-L{HERE}", "-ltal_chain", "-ltal_bsw", "-Wl,-rpath,$ORIGIN"))
 
 
+# Host-code sanitizers for the GPU library: the C ABI, the dispatcher, the
+# serving loop and the file I/O run under AddressSanitizer and UBSan, the
+# device code is built as usual.  The command-line driver links the
+# sanitizer runtime (clang's, as the library was built by clang).
+ASAN_DIR = os.path.join(HERE, "_asan")
+HOST_SANITIZE = ("-Xarch_host -fsanitize=address -Xarch_host -fsanitize=undefined "
+                 "-Xarch_host -fno-omit-frame-pointer").split()
+
+
+def build_asan():
+    """_asan/libtal_bsw.so and _asan/tal-bench-bsw with sanitized host code."""
+    os.makedirs(ASAN_DIR, exist_ok=True)
+    lib = build_hip(os.path.join(ASAN_DIR, "libtal_bsw.so"), extra=HOST_SANITIZE)
+    cli = os.path.join(ASAN_DIR, "tal-bench-bsw")
+    clangxx = os.path.join(os.path.dirname(os.path.realpath(HIPCC)), "..", "llvm", "bin",
+                           "clang++")
+    san = ["-fsanitize=address,undefined", "-fno-omit-frame-pointer"]
+    _run([clangxx, "-O1", "-g", "-std=c++17", *san, f"-I{os.path.join(ROOT, 'include')}",
+          os.path.join(CSRC, "tal_bench_bsw.cpp"), "-o", cli, f"-L{ASAN_DIR}", "-ltal_bsw",
+          "-Wl,-rpath,$ORIGIN"])
+    return lib, cli
+
+
 def build_all():
     build_synth()
     build_hip()
@@ -115,6 +138,7 @@ def build_all():
     build_cli()
     build_sw_cli()
     build_chain_cli()
+    build_asan()
 
 
 if __name__ == "__main__":

@@ -537,7 +537,7 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                  * the registers its computation reads, so the wait for them
                  * falls just before that computation, a whole block after
                  * they were issued, and never covers later stores. */
-                if (from < to) {
+                if (from < to && !G) {
                     uint32_t A[8], Bv[8];
                     uint32_t qa = 0, qb = 0;
                     load(from, true, A, qa);
@@ -550,6 +550,31 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                         load(b + 8, b + 8 < to, A, qa);
                         flush();
                         do_block(b, Bv, qb);
+                        b += 8;
+                        if (b >= to) break;
+                    }
+                } else if (from < to) {
+                    /* global blocks: loads two blocks ahead (three buffers),
+                     * so that the memory latency is covered by two blocks of
+                     * computation */
+                    uint32_t A[8], Bv[8], C[8];
+                    uint32_t qa = 0, qb = 0, qc = 0;
+                    load(from, true, A, qa);
+                    load(from + 8, from + 8 < to, Bv, qb);
+                    for (int b = from;;) {
+                        load(b + 16, b + 16 < to, C, qc);
+                        flush();
+                        do_block(b, A, qa);
+                        b += 8;
+                        if (b >= to) break;
+                        load(b + 16, b + 16 < to, A, qa);
+                        flush();
+                        do_block(b, Bv, qb);
+                        b += 8;
+                        if (b >= to) break;
+                        load(b + 16, b + 16 < to, Bv, qb);
+                        flush();
+                        do_block(b, C, qc);
                         b += 8;
                         if (b >= to) break;
                     }

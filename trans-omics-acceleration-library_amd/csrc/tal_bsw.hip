@@ -157,6 +157,9 @@ struct tal_bsw {
      * no-allocation call in progress (null in every other call) */
     uint8_t *scratch = nullptr;
     Arena *arena = nullptr;
+    /* fault injection (TAL_BSW_FAIL_ALLOC=k at creation): the k-th scratch
+     * request of the handle fails as if hipMalloc had run out of memory */
+    long fail_alloc = 0, allocs = 0;
     /* per-call timing */
     /* 16-bit start/end, 8-bit start/end, rerun start/end */
     hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
@@ -198,6 +201,8 @@ int hip_fail(tal_bsw *h, hipError_t e, const char *what)
 template <class T> int take(tal_bsw *h, DevBuf &b, size_t bytes, const char *what, T **out)
 {
     if (!h->arena) {
+        if (h->fail_alloc > 0 && ++h->allocs == h->fail_alloc)
+            HIPCHK(h, hipErrorOutOfMemory, what);
         HIPCHK(h, b.ensure(bytes), what);
         *out = (T *)b.p;
         return TAL_OK;
@@ -896,9 +901,10 @@ int host_call(tal_bsw *h, int mode, tal_seqpair *pairs, const uint8_t *ref, size
     if (n == 0) return TAL_OK;
     HIPCHK(h, hipSetDevice(h->device), "set device");
     hipStream_t s = h->stream;
-    HIPCHK(h, h->d_pairs.ensure(sizeof(tal_seqpair) * n), "alloc pairs");
-    HIPCHK(h, h->d_ref.ensure(std::max<size_t>(ref_len, 1)), "alloc ref");
-    HIPCHK(h, h->d_qer.ensure(std::max<size_t>(qer_len, 1)), "alloc qer");
+    void *unused;
+    TAKE(h, h->d_pairs, sizeof(tal_seqpair) * n, "alloc pairs", &unused);
+    TAKE(h, h->d_ref, std::max<size_t>(ref_len, 1), "alloc ref", &unused);
+    TAKE(h, h->d_qer, std::max<size_t>(qer_len, 1), "alloc qer", &unused);
     HIPCHK(h, hipMemcpyAsync(h->d_pairs.p, pairs, sizeof(tal_seqpair) * n, hipMemcpyHostToDevice, s),
            "copy pairs");
     if (ref_len)
@@ -987,6 +993,7 @@ int tal_bsw_create(const tal_bsw_params *params, int device, tal_bsw **out)
         if (set_main_lds_limit(h->lds_per_block) != hipSuccess) { rc = TAL_EDEVICE; break; }
         if (const char *e = getenv("TAL_BSW_WAVES_PER_CU")) h->waves_per_cu = std::max(1, atoi(e));
         if (const char *e = getenv("TAL_BSW_ORDER")) h->order = atoi(e) ? 1 : 0;
+        if (const char *e = getenv("TAL_BSW_FAIL_ALLOC")) h->fail_alloc = atol(e);
         if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) { rc = TAL_EDEVICE; break; }
         bool evok = true;
         for (hipEvent_t &e : h->ev) evok = evok && hipEventCreate(&e) == hipSuccess;
