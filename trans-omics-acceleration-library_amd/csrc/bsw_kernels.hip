@@ -57,6 +57,9 @@ typedef short s2v __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ s2v as_s2(uint32_t x) { return __builtin_bit_cast(s2v, x); }
 __device__ __forceinline__ uint32_t as_u32(s2v x) { return __builtin_bit_cast(uint32_t, x); }
 __device__ __forceinline__ s2v max2(s2v a, s2v b) { return __builtin_elementwise_max(a, b); }
+typedef unsigned short u2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ u2v as_u2(uint32_t x) { return __builtin_bit_cast(u2v, x); }
+__device__ __forceinline__ uint32_t u2_u32(u2v x) { return __builtin_bit_cast(uint32_t, x); }
 
 /* Reductions over the lanes of one group (xor offsets stay inside it). */
 template <int GL> __device__ __forceinline__ int group_max(int x)
@@ -87,10 +90,6 @@ __device__ __forceinline__ int16_t score_of(int s1, int s2, const KParams &K)
     return amb ? K.ambig : s;
 }
 
-/* One dword of state per (column, lane): columns below `split` in LDS
- * (lds[c * 64]), the rest in the block's global slab
- * (glob[(c - split) * 64]).  split is a multiple of 8, so an aligned
- * 8-column block lies wholly in one of the two. */
 typedef __attribute__((address_space(3))) uint32_t lds_u32;
 typedef __attribute__((address_space(1))) uint32_t glb_u32;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -99,33 +98,80 @@ typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
 
 /* One dword of state per (column, lane), in 8-column blocks of each lane:
  * [block][64 lanes][8 columns], so that a lane's block is 32 contiguous
- * bytes.  Blocks below `split` live in LDS, the rest in the wave's global
- * slab; split is a multiple of 8, so an aligned block lies wholly in one
- * place.  The row sweep runs over the LDS blocks and then over the global
- * blocks, each with its own kind of memory instruction.  In the global part a
- * lane that skips a block is pointed at a zero block (loads) or its own
- * scratch block (stores) instead of being branched around: every lane then
- * issues the same memory instructions, so the waits for them stay exact, and
- * lanes outside their band touch no state cache lines. */
+ * bytes.  Every block has a home in the wave's global slab; a window of `nb`
+ * consecutive blocks starting at block `lb` lives in LDS instead, as a ring
+ * (block B in slot (B - lb + r0) mod nb).  The window follows the start of
+ * the row's column range: columns below it are never stored again, so the
+ * LDS holds the part of every row's range that fits, instead of a fixed set
+ * of columns that later rows leave behind.  The row sweep runs over the LDS
+ * blocks and then over the global blocks, each with its own kind of memory
+ * instruction.  In the global part a lane that skips a block is pointed at a
+ * zero block (loads) or its own scratch block (stores) instead of being
+ * branched around: every lane then issues the same memory instructions, so
+ * the waits for them stay exact, and lanes outside their band touch no state
+ * cache lines. */
 struct StateRef {
-    lds_u32 *lds;          /* this lane's 8 dwords of LDS block 0 */
-    glb_u32 *glob;         /* this lane's 8 dwords of the global block at split */
+    lds_u32 *lds;          /* this lane's 8 dwords of LDS slot 0 */
+    glb_u32 *glob;         /* this lane's 8 dwords of global block 0 */
     const glb_u32 *zero;   /* 32 zero bytes */
     glb_u32 *trash;        /* 32 bytes of this lane that nothing reads */
-    int split;
+    int nb;                /* LDS slots (blocks) */
+    int nblk;              /* state blocks in the slab */
+    int lb = 0;            /* first block of the LDS window */
+    int r0 = 0;            /* its slot */
+    __device__ __forceinline__ bool in_lds(int blk) const
+    {
+        return (unsigned)(blk - lb) < (unsigned)nb;
+    }
+    __device__ __forceinline__ int slot(int blk) const
+    {
+        const int s = blk - lb + r0;
+        return s >= nb ? s - nb : s;
+    }
     /* generic address of column c (single accesses outside the sweep) */
     __device__ __forceinline__ uint32_t *at(int c) const
     {
-        const int b = c & ~7;
-        return b < split ? (uint32_t *)(lds + ((uint32_t)b << 6) + (c & 7))
-                         : (uint32_t *)(glob + ((uint32_t)(b - split) << 6) + (c & 7));
+        const int blk = c >> 3;
+        return in_lds(blk) ? (uint32_t *)(lds + ((uint32_t)slot(blk) << 9) + (c & 7))
+                           : (uint32_t *)(glob + ((uint32_t)blk << 9) + (c & 7));
     }
     __device__ __forceinline__ uint32_t ld(int c) const { return *at(c); }
     __device__ __forceinline__ void st(int c, uint32_t v) const { *at(c) = v; }
-    __device__ __forceinline__ lds_u32 *lblk(int b) const { return lds + ((uint32_t)b << 6); }
-    __device__ __forceinline__ glb_u32 *gblk(int b) const
+    /* the block at column b (a multiple of 8) in LDS / in the slab */
+    __device__ __forceinline__ lds_u32 *lblk(int b) const
     {
-        return glob + ((uint32_t)(b - split) << 6);
+        return lds + ((uint32_t)slot(b >> 3) << 9);
+    }
+    __device__ __forceinline__ glb_u32 *gblk(int b) const { return glob + ((uint32_t)b << 6); }
+    /* column where the LDS window ends */
+    __device__ __forceinline__ int wend() const { return (lb + nb) << 3; }
+    /* the window back to block 0 (a new group) */
+    __device__ __forceinline__ void reset()
+    {
+        lb = 0;
+        r0 = 0;
+    }
+    /* Moves the window start up to block `to`: each block leaving the
+     * window is written to its home in the slab, and its slot takes the
+     * block entering at the window end. */
+    __device__ void advance(int to)
+    {
+        while (lb < to) {
+            lds_u32x4 *p = (lds_u32x4 *)(lds + ((uint32_t)r0 << 9));
+            glb_u32x4 *g = (glb_u32x4 *)(glob + ((uint32_t)lb << 9));
+            const u32x4 x = p[0], y = p[1];
+            g[0] = x;
+            g[1] = y;
+            const int in = lb + nb;
+            if (in < nblk) {
+                const glb_u32x4 *h = (const glb_u32x4 *)(glob + ((uint32_t)in << 9));
+                const u32x4 a = h[0], c = h[1];
+                p[0] = a;
+                p[1] = c;
+            }
+            lb++;
+            r0 = r0 + 1 == nb ? 0 : r0 + 1;
+        }
     }
     /* the 8 columns of the aligned block at b; G: the block is global */
     template <bool G> __device__ __forceinline__ void ld8(int b, uint32_t *v, bool need) const
@@ -189,7 +235,7 @@ template <int BITS> __device__ __forceinline__ int table_score(int s1, int c, in
  * null when the launch carries none.
  */
 template <int BITS>
-__device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRef S, uint32_t *Q)
+__device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRef &S, uint32_t *Q)
 {
     typedef Width<BITS> Wd;
     constexpr int GL = Wd::GL;
@@ -281,6 +327,7 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
     const uint8_t *rseq = B.ref + (int64_t)idr;
     const uint8_t *qseq = B.qer + (int64_t)idq;
 
+    S.reset();
     /* ---- H_h row -1 and F (:2867-2879, :3034) ---- */
     const bool stale = BITS == 16 && B.stale_in && gvalid && ((B.stale_in[g] >> orig) & 1u);
     /* nzhi: the highest state column of this lane that may hold a nonzero
@@ -301,9 +348,11 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
     }
     /* ---- query codes for the fast path ---- */
     /* Every int8 score fits the per-row byte table.  The fast path needs
-     * query bytes 0-4 and nonnegative insertion penalties (it relies on a
-     * run of zero cells keeping E at zero). */
-    bool fast = oe_ins_s >= 0 && e_ins_s >= 0;
+     * query bytes 0-4, nonnegative insertion penalties (it relies on a run of
+     * zero cells keeping E at zero) and nonnegative gap extensions (E and F
+     * are nonnegative, so e - e_ins and f - e_del then never wrap and their
+     * max with 0 is an unsigned saturating subtraction). */
+    bool fast = oe_ins_s >= 0 && e_ins_s >= 0 && e_del_s >= 0;
     {
         bool ok = true;
         const int nwords = (ncol + 1 + 7) / 8 + 1;
@@ -352,6 +401,7 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
         if (beg < i - wband) beg = i - wband;
         if (end > i + wband + 1) end = i + wband + 1;
         if (end > ncol) end = ncol;
+        S.advance(beg >> 3); /* the LDS window starts at the range */
         const int16_t i1 = Wd::v(i + 1);
         int16_t e11 = 0, maxRS = 0, y1 = 0;
         int16_t h10 = 0;
@@ -417,6 +467,8 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
             const s2v OE = {oe_ins_s, oe_del_s};
             const s2v EE = {e_ins_s, e_del_s};
             const s2v ZERO2 = {0, 0};
+            const s2v OEd2 = {oe_del_s, oe_del_s}, OEi2 = {oe_ins_s, oe_ins_s};
+            const u2v EDu = {(uint16_t)e_del_s, (uint16_t)e_del_s};
             uint32_t ez = 0;                       /* low half: E entering the column */
             uint32_t hprev = (uint32_t)(uint16_t)h10; /* low half: H of the previous column */
             /* sbt: the score in the low half of a dword (sign-extended);
@@ -471,7 +523,7 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
              * and memory above nzhi already holds the masked zeros. */
             const int lo = min(hd, pend + 1);
             const int sthi = max(tl, nzhi);
-            /* the row's blocks: [b0, split) in LDS, then the global ones */
+            /* the row's blocks: those in the LDS window, then the global ones */
             uint32_t pr[8];
             int pb = -1;
             bool pneed = false;
@@ -488,13 +540,54 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                     const uint32_t rel0 = (uint32_t)(b - hd);
                     uint32_t any = 0, kb = 0;
                     if (b > beg && b + 8 <= end) {
+                        /* Two columns at a time in packed 16-bit halves:
+                         * the diagonal term m and F' of a column depend only
+                         * on the state above it, so columns k and k+1 share
+                         * every instruction; only E runs column by column.
+                         * m = H + s, or 0 when H == 0, as t*s + H with
+                         * t = min(H, 1). */
+                        uint32_t e = ez, hp = hprev;
 #pragma unroll
-                        for (int k = 0; k < 8; k++) {
-                            const uint32_t st = cell(sv[k], score((k & 1) ? sco : sce, k >> 1));
-                            pr[k] = rel0 + k < lim1 ? st : 0u;
-                            kb = max(kb, (pr[k] << 16) | (uint32_t)k);
-                            any |= pr[k];
+                        for (int q = 0; q < 4; q++) {
+                            const int k = 2 * q;
+                            /* scores of columns k, k+1 in the high bytes of
+                             * the two halves (value << 8) */
+                            const uint32_t sel = 0x000C000Cu | ((uint32_t)q << 8) |
+                                                 ((uint32_t)(4 + q) << 24);
+                            const uint32_t sp8 = __builtin_amdgcn_perm(sco, sce, sel);
+                            const s2v S = BITS == 16 ? (as_s2(sp8) >> 8) : as_s2(sp8);
+                            const u2v H = as_u2(__builtin_amdgcn_perm(sv[k + 1], sv[k], 0x05040100u));
+                            const u2v F = as_u2(__builtin_amdgcn_perm(sv[k + 1], sv[k], 0x07060302u));
+                            uint32_t t, mw;
+                            asm("v_pk_min_u16 %0, %1, %2" : "=v"(t) : "v"(u2_u32(H)), "v"(0x00010001u));
+                            asm("v_pk_mad_u16 %0, %1, %2, %3"
+                                : "=v"(mw)
+                                : "v"(t), "v"(as_u32(S)), "v"(u2_u32(H)));
+                            const s2v m = as_s2(mw);
+                            const s2v fn = max2(m - OEd2, as_s2(u2_u32(__builtin_elementwise_sub_sat(F, EDu))));
+                            const s2v am = m - OEi2;
+                            const uint16_t e0 = (uint16_t)e;
+                            const uint16_t e1 = (uint16_t)smax16(am.x, (int16_t)__builtin_elementwise_sub_sat(e0, (uint16_t)e_ins_s));
+                            const uint16_t e2 = (uint16_t)smax16(am.y, (int16_t)__builtin_elementwise_sub_sat(e1, (uint16_t)e_ins_s));
+                            const s2v E2 = as_s2((uint32_t)e0 | ((uint32_t)e1 << 16));
+                            const uint32_t h2 = as_u32(max2(max2(m, as_s2(u2_u32(F))), E2));
+                            const uint32_t st0 = __builtin_amdgcn_perm(as_u32(fn), hp, 0x05040100u);
+                            const uint32_t st1 = __builtin_amdgcn_perm(as_u32(fn), h2, 0x07060100u);
+                            pr[k] = rel0 + k < lim1 ? st0 : 0u;
+                            pr[k + 1] = rel0 + k + 1 < lim1 ? st1 : 0u;
+                            hp = h2 >> 16;
+                            e = e2;
                         }
+                        ez = e;
+                        hprev = hp;
+                        /* row-max key (store << 16 | column) and nonzero
+                         * flag of the block, three operands at a time */
+                        uint32_t kk[8];
+#pragma unroll
+                        for (int k = 0; k < 8; k++) kk[k] = (pr[k] << 16) | (uint32_t)k;
+                        kb = max(max(max(kk[0], kk[1]), max(kk[2], kk[3])),
+                                 max(max(kk[4], kk[5]), max(kk[6], kk[7])));
+                        any = (pr[0] | pr[1] | pr[2]) | (pr[3] | pr[4] | pr[5]) | (pr[6] | pr[7]);
                         pb = b;
                         pneed = b <= sthi && b + 7 >= lo;
                     } else {
@@ -583,8 +676,9 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
             };
             if (beg < end) {
                 const int b0 = beg & ~7;
-                sweep(std::false_type{}, b0, min(end, S.split));
-                sweep(std::true_type{}, max(b0, S.split), end);
+                const int we = S.wend();
+                sweep(std::false_type{}, b0, min(end, we));
+                sweep(std::true_type{}, max(b0, we), end);
                 j = end;
             }
             h10 = (int16_t)hprev;
@@ -825,15 +919,17 @@ __global__ __launch_bounds__(64) void bsw_main(KParams K, KBuffers B)
     typedef Width<BITS> Wd;
     const int lane = threadIdx.x;
     const int half = lane / Wd::GL;
-    /* LDS: state columns [0, lds_cols).  The block's global slab holds the
-     * query-code words, then state columns [lds_cols, state_cols). */
-    /* slab: query words [word][64 lanes], the global state blocks, and one
-     * scratch block (the target of skipped stores) */
+    /* LDS: a window of lds_cols state columns.  The block's global slab
+     * holds the query-code words ([word][64 lanes]), the home of every state
+     * block, and one scratch block (the target of skipped stores). */
     uint32_t *slab = B.slabs + (size_t)blockIdx.x * K.slab_cols * 64;
-    StateRef S{(lds_u32 *)lds_state + lane * 8,
-               (glb_u32 *)(slab + (size_t)K.query_words * 64 + (size_t)lane * 8),
-               (const glb_u32 *)B.zero,
-               (glb_u32 *)(slab + (size_t)(K.slab_cols - 8) * 64 + (size_t)lane * 8), K.lds_cols};
+    StateRef S;
+    S.lds = (lds_u32 *)lds_state + lane * 8;
+    S.glob = (glb_u32 *)(slab + (size_t)K.query_words * 64 + (size_t)lane * 8);
+    S.zero = (const glb_u32 *)B.zero;
+    S.trash = (glb_u32 *)(slab + (size_t)(K.slab_cols - 8) * 64 + (size_t)lane * 8);
+    S.nb = K.lds_cols >> 3;
+    S.nblk = (K.state_cols + 7) >> 3;
     uint32_t *Q = slab + lane;
     if (B.abort_flag && *B.abort_flag != 0) return;
     const int ntasks = B.ntasks_dev ? *B.ntasks_dev : K.ntasks;

@@ -379,9 +379,9 @@ int grid_for(const tal_bsw *h, bool lds, size_t lds_bytes, int ntasks)
 /* Placement of a wave's state: columns 0..max_ncol plus 17 columns of
  * slack for the 8-column block loads that run past a group's last column,
  * and the fast path's query-code words (8 columns per word, one word of
- * prefetch slack).  The first lds_cols state columns live in LDS, sized so
- * that `waves_per_cu` waves fit on a CU; the query words and the remaining
- * state columns go to a per-block slab in global memory. */
+ * prefetch slack).  A window of lds_cols state columns lives in LDS, sized so
+ * that `waves_per_cu` waves fit on a CU; the query words and the home of
+ * every state column go to a per-block slab in global memory. */
 struct Sizing {
     int state_cols, qwords, lds_cols, slab_cols;
     size_t lds_bytes;
@@ -400,8 +400,8 @@ Sizing sizing(const tal_bsw *h, int max_ncol)
     const size_t max_cols = h->lds_per_block / 256;
     if ((size_t)lds_cols > max_cols) lds_cols = (int)(max_cols & ~(size_t)7);
     z.lds_cols = lds_cols;
-    /* global state in whole 8-column blocks, then one scratch block */
-    z.slab_cols = z.qwords + ((std::max(0, z.state_cols - lds_cols) + 7) & ~7) + 8;
+    /* every state column in whole 8-column blocks, then one scratch block */
+    z.slab_cols = z.qwords + ((z.state_cols + 7) & ~7) + 8;
     z.lds_bytes = (size_t)lds_cols * 64 * sizeof(uint32_t);
     return z;
 }
@@ -590,7 +590,7 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
         h->st.ms_rerun16 = ms_rerun;
         h->st.groups16 = G;
         h->st.reruns16 = reruns;
-        h->st.lds_state16 = slab_cols == qwords ? 1 : 0;
+        h->st.lds_state16 = lds_cols >= ((state_cols + 7) & ~7) ? 1 : 0;
         if (!fresh) {
             HIPCHK(h, hipMemsetAsync(last, 0xFF, sizeof(int32_t) * IMAGE_COLS, s), "clear last");
             HIPCHK(h, launch_image_update(G, max_ncol + 1, shapes, B.wrote_out, last, B.fmask,
