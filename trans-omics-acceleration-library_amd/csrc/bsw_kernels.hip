@@ -694,20 +694,19 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
             /* exact first/last nonzero column of [beg, end) from the two
              * flagged blocks (one 8-column read each) */
             if (bfirst != NONE_COL) {
-                uint32_t nzm = 0;
+                /* both blocks whole (each lies in LDS or in the slab) */
+                const u32x4 *pf = (const u32x4 *)S.at(bfirst), *pl = (const u32x4 *)S.at(blast);
+                const u32x4 f0 = pf[0], f1 = pf[1], l0 = pl[0], l1 = pl[1];
+                const uint32_t vf[8] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w};
+                const uint32_t vl[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
+                uint32_t nzf = 0, nzl = 0;
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
-                    const int c = bfirst + k;
-                    if (c >= beg && c < end && S.ld(c) != 0u) nzm |= 1u << k;
+                    if (bfirst + k >= beg && bfirst + k < end && vf[k] != 0u) nzf |= 1u << k;
+                    if (blast + k >= beg && blast + k < end && vl[k] != 0u) nzl |= 1u << k;
                 }
-                fz = bfirst + __builtin_ctz(nzm);
-                nzm = 0;
-#pragma unroll
-                for (int k = 0; k < 8; k++) {
-                    const int c = blast + k;
-                    if (c >= beg && c < end && S.ld(c) != 0u) nzm |= 1u << k;
-                }
-                lz = blast + 31 - __builtin_clz(nzm);
+                fz = bfirst + __builtin_ctz(nzf);
+                lz = blast + 31 - __builtin_clz(nzl);
             }
         } else {
             const int s1b = s1;
@@ -869,9 +868,22 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
             bool dec = false;
             if (live && (S.ld(ncol) & 0xFFFFu) == 0u) {
                 const int16_t tnz = smin16(add16(tail, TWO), qlen);
-                int c = ncol - 1;
-                while (c >= bs && S.ld(c) == 0u) c--;
-                const int16_t tz = smin16(add16(Wd::v(c >= bs ? c : bs - 1), TWO), qlen);
+                /* the last nonzero column of [bs, ncol) ("zero at ncol"):
+                 * here end < ncol (F[ncol] not yet stored) and bs <= end.
+                 * Above the range only columns up to nzhi can be nonzero;
+                 * in [beg, end] it is lz, and below beg the stale stretch's
+                 * last nonzero. */
+                int c = min(ncol - 1, nzhi);
+                while (c > end && S.ld(c) == 0u) c--;
+                if (c <= end) {
+                    if (lz >= max(beg, bs))
+                        c = lz;
+                    else if (bs < beg && st_last >= bs)
+                        c = st_last;
+                    else
+                        c = bs - 1;
+                }
+                const int16_t tz = smin16(add16(Wd::v(c), TWO), qlen);
                 dec = tnz != tz;
             }
             /* bit of the reference's lane: read `dec` from the lane holding it */
