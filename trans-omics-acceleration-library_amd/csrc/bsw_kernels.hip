@@ -334,16 +334,25 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
      * value; blocks above it and above the band need no store */
     int nzhi = -1;
     {
+        /* whole 8-column blocks (columns past ncol get zeros; nothing reads
+         * them as values) */
         int16_t hv = h0w;
-        for (int c = 0; c <= ncol; c++) {
-            if (c == 1)
-                hv = h0w > oe_ins_s ? sub16(h0w, oe_ins_s) : (int16_t)0;
-            else if (c >= 2)
-                hv = (c < ncol) ? smax16(sub16(hv, e_ins_s), 0) : (int16_t)0;
-            const int16_t f = (c < ncol) ? (int16_t)0 : (int16_t)(stale ? 1 : 0);
-            const uint32_t v = pack(hv, f);
-            S.st(c, v);
-            if (v != 0u) nzhi = c;
+        for (int b = 0; b <= ncol; b += 8) {
+            uint32_t v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int c = b + k;
+                if (c == 1)
+                    hv = h0w > oe_ins_s ? sub16(h0w, oe_ins_s) : (int16_t)0;
+                else if (c >= 2)
+                    hv = (c < ncol) ? smax16(sub16(hv, e_ins_s), 0) : (int16_t)0;
+                const int16_t f = (c < ncol) ? (int16_t)0 : (int16_t)(stale ? 1 : 0);
+                v[k] = c <= ncol ? pack(hv, f) : 0u;
+                if (v[k] != 0u) nzhi = c;
+            }
+            u32x4 *p = (u32x4 *)S.at(b);
+            p[0] = (u32x4){v[0], v[1], v[2], v[3]};
+            p[1] = (u32x4){v[4], v[5], v[6], v[7]};
         }
     }
     /* ---- query codes for the fast path ---- */
@@ -744,16 +753,26 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                 }
             }
         }
+        uint32_t rj; /* the boundary store */
         { /* boundary store at j (end, or beg when the loop was empty) */
             const bool out1 = (jj > tail) || (head > jj);
-            const uint32_t r = pack(out1 ? (int16_t)0 : h10, 0);
-            S.st(j, r);
+            rj = pack(out1 ? (int16_t)0 : h10, 0);
+            S.st(j, rj);
             if (j == ncol) fwritten = true;
             /* fz: first nonzero in [beg, end) (end if none);
              * lz: last nonzero in [beg, end] (beg - 1 if none) */
             fz = fz == NONE_COL ? end : fz;
-            lz = (r != 0u && j >= beg) ? j : (lz >= beg ? lz : beg - 1);
+            lz = (rj != 0u && j >= beg) ? j : (lz >= beg ? lz : beg - 1);
         }
+        /* Loaded now, so that their latency overlaps the reductions below:
+         * columns j+1 and j+2 (the head and tail scans look at most two
+         * columns past the row's end, es <= end + 2) and the last query
+         * column for gscore. */
+        const bool gq = live && tail == qlen && qcol - 1 >= beg && qcol - 1 < end;
+        const uint32_t vj1 = S.ld(j + 1), vj2 = S.ld(j + 2), vq = S.ld(gq ? qcol : j);
+        auto colval = [&](int c) -> uint32_t {
+            return c == j ? rj : c == j + 1 ? vj1 : c == j + 2 ? vj2 : S.ld(c);
+        };
         pend = j;
         phd = hd;
         ptl = tl;
@@ -762,8 +781,8 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
          * when it was computed this row, the lane is live and tail == qlen;
          * then H_h[qlen] holds it (head <= qlen, or the lane would not be
          * live, so the store at column qlen was not masked). */
-        if (live && tail == qlen && qcol - 1 >= beg && qcol - 1 < end) {
-            const int16_t hq = (int16_t)(S.ld(qcol) & 0xFFFFu);
+        if (gq) {
+            const int16_t hq = (int16_t)(vq & 0xFFFFu);
             if (!(gscore > hq)) max_ie = i1;
             gscore = smax16(gscore, hq);
         }
@@ -842,14 +861,14 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
             fzs = fz;
         } else {
             int c = max(max(bs, beg), end);
-            while (c < es && S.ld(c) == 0u) c++;
+            while (c < es && colval(c) == 0u) c++;
             fzs = c < es ? c : es;
         }
         /* last nonzero <= es in [bs, es] (bs - 1 if none) */
         int lzs;
         {
             int c = es;
-            while (c > end && c >= bs && S.ld(c) == 0u) c--;
+            while (c > end && c >= bs && colval(c) == 0u) c--;
             if (c > end && c >= bs)
                 lzs = c;
             else if (lz >= beg && lz >= bs && lz <= es)
@@ -915,10 +934,16 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
             B.wrote_out[g] = wr ? 1 : 0;
         }
         uint32_t *fm = B.fmask + B.fmask_off[g];
-        for (int c = 0; c < ncol; c++) {
-            const int nz_o = __builtin_amdgcn_ds_bpermute(inv * 4, (S.ld(c) >> 16) != 0u ? 1 : 0);
-            const uint64_t nz = ballot(nz_o != 0);
-            if (gl == 0) fm[c] = (uint32_t)(nz >> (32 * half));
+        for (int b = 0; b < ncol; b += 8) {
+            const u32x4 *p = (const u32x4 *)S.at(b);
+            const u32x4 x = p[0], y = p[1];
+            const uint32_t v[8] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int nz_o = __builtin_amdgcn_ds_bpermute(inv * 4, (v[k] >> 16) != 0u ? 1 : 0);
+                const uint64_t nz = ballot(nz_o != 0);
+                if (gl == 0 && b + k < ncol) fm[b + k] = (uint32_t)(nz >> (32 * half));
+            }
         }
     }
 }
