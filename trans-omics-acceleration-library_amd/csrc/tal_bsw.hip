@@ -630,6 +630,11 @@ struct tal_bsw_plan {
     DevBuf blk, sel8, sel16, pairs8, pairs16, ctl;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    /* the 8-bit branch of the graph: captured on its own stream, forked
+     * after the 16-bit main launch and joined at the end, so that it runs
+     * beside the 16-bit rerun rounds */
+    hipStream_t s8 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 namespace {
@@ -638,6 +643,9 @@ void plan_release(tal_bsw_plan *P)
 {
     if (P->exec) (void)hipGraphExecDestroy(P->exec);
     if (P->graph) (void)hipGraphDestroy(P->graph);
+    if (P->fork) (void)hipEventDestroy(P->fork);
+    if (P->join) (void)hipEventDestroy(P->join);
+    if (P->s8) (void)hipStreamDestroy(P->s8);
     DevBuf *all[] = {&P->blk, &P->sel8, &P->sel16, &P->pairs8, &P->pairs16, &P->ctl};
     for (DevBuf *b : all) b->release();
     for (PathBufs *pb : {&P->p8.pb, &P->p16.pb}) {
@@ -755,18 +763,14 @@ int plan_enqueue(tal_bsw *h, tal_bsw_plan *P, hipStream_t s)
         HIPCHK(h, launch_gather(P->pairs, P->sel16.as<int32_t>(), P->n16,
                                 P->pairs16.as<tal_seqpair>(), s),
                "gather16");
-    if (P->n8) {
-        WidthPlan &W = P->p8;
-        HIPCHK(h, launch_plan_pad8(P->pairs8.as<tal_seqpair>(), P->n8, ctl + 1, ctl, s), "pad8");
-        hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, s, W.B.task_counter, 1, 0);
-        HIPCHK(h, launch_main(8, true, W.grid, W.z.lds_bytes, W.K, W.B, s), "main8");
-    }
+    /* 16-bit main launch first; the 8-bit branch then runs beside the
+     * 16-bit resolution and rerun rounds (they share no data) */
+    WidthPlan &W = P->p16;
+    const int32_t G = W.G;
+    GroupShape *shapes = W.pb.shapes.as<GroupShape>();
+    int32_t *cnt = W.B.task_counter + 1;
+    int32_t *list = W.pb.rerun.as<int32_t>();
     if (P->n16) {
-        WidthPlan &W = P->p16;
-        const int32_t G = W.G;
-        GroupShape *shapes = W.pb.shapes.as<GroupShape>();
-        int32_t *cnt = W.B.task_counter + 1;
-        int32_t *list = W.pb.rerun.as<int32_t>();
         HIPCHK(h, launch_shapes(16, P->pairs16.as<tal_seqpair>(), W.n, G, 0, shapes, nullptr, s),
                "shapes16");
         HIPCHK(h, launch_plan_check16(shapes, G, W.max_ncol, ctl, s), "check16");
@@ -774,6 +778,24 @@ int plan_enqueue(tal_bsw *h, tal_bsw_plan *P, hipStream_t s)
                            (int32_t *)W.pb.stale.p, G, 0);
         hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, s, W.B.task_counter, 2, 0);
         HIPCHK(h, launch_main(16, true, W.grid, W.z.lds_bytes, W.K, W.B, s), "main16");
+    }
+    hipStream_t s8 = s;
+    if (P->n8 && P->n16 && P->s8) {
+        HIPCHK(h, hipEventRecord(P->fork, s), "fork");
+        HIPCHK(h, hipStreamWaitEvent(P->s8, P->fork, 0), "fork wait");
+        s8 = P->s8;
+    }
+    if (P->n8) {
+        WidthPlan &W8 = P->p8;
+        HIPCHK(h, launch_plan_pad8(P->pairs8.as<tal_seqpair>(), P->n8, ctl + 1, ctl, s8), "pad8");
+        hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, s8, W8.B.task_counter, 1, 0);
+        HIPCHK(h, launch_main(8, true, W8.grid, W8.z.lds_bytes, W8.K, W8.B, s8), "main8");
+        HIPCHK(h, launch_scatter_out(P->pairs8.as<tal_seqpair>(), P->sel8.as<int32_t>(), P->n8,
+                                     P->pairs, s8),
+               "scatter8");
+    }
+    if (s8 != s) HIPCHK(h, hipEventRecord(P->join, s8), "join");
+    if (P->n16) {
         HIPCHK(h, launch_resolve(G, shapes, W.B.dep_out, W.B.wrote_out, W.B.fmask,
                                  W.B.fmask_off, h->image.as<uint32_t>(), nullptr,
                                  W.pb.stale.as<uint32_t>(), list, cnt, s, ctl),
@@ -801,15 +823,11 @@ int plan_enqueue(tal_bsw *h, tal_bsw_plan *P, hipStream_t s)
                                       W.pb.last.as<int32_t>(), W.B.fmask, W.B.fmask_off,
                                       h->image.as<uint32_t>(), s, ctl + 3),
                "image update");
-    }
-    if (P->n8)
-        HIPCHK(h, launch_scatter_out(P->pairs8.as<tal_seqpair>(), P->sel8.as<int32_t>(), P->n8,
-                                     P->pairs, s),
-               "scatter8");
-    if (P->n16)
         HIPCHK(h, launch_scatter_out(P->pairs16.as<tal_seqpair>(), P->sel16.as<int32_t>(),
                                      P->n16, P->pairs, s),
                "scatter16");
+    }
+    if (s8 != s) HIPCHK(h, hipStreamWaitEvent(s, P->join, 0), "join wait");
     return TAL_OK;
 }
 
@@ -1253,6 +1271,11 @@ int tal_bsw_plan_create(tal_bsw *h, tal_seqpair *pairs, const uint8_t *ref, cons
     rc = plan_width(h, P, 16, s);
     if (rc) return rc;
     HIPCHK(h, hipStreamSynchronize(s), "sync plan");
+    if (P->n8 && P->n16) {
+        HIPCHK(h, hipStreamCreateWithFlags(&P->s8, hipStreamNonBlocking), "branch stream");
+        HIPCHK(h, hipEventCreateWithFlags(&P->fork, hipEventDisableTiming), "fork event");
+        HIPCHK(h, hipEventCreateWithFlags(&P->join, hipEventDisableTiming), "join event");
+    }
     /* capture (kernel events are not recorded inside the graph: the replay
      * is timed as a whole by tal_bsw_plan_launch) */
     HIPCHK(h, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
