@@ -950,7 +950,7 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
 
 /* Persistent grid: each 64-thread block is one wave that pulls tasks. */
 template <int BITS>
-__global__ __launch_bounds__(64) void bsw_main(KParams K, KBuffers B)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bsw_main(KParams K, KBuffers B)
 {
     extern __shared__ uint32_t lds_state[];
     typedef Width<BITS> Wd;
