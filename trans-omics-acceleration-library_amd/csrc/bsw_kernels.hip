@@ -105,16 +105,13 @@ typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
  * LDS holds the part of every row's range that fits, instead of a fixed set
  * of columns that later rows leave behind.  The row sweep runs over the LDS
  * blocks and then over the global blocks, each with its own kind of memory
- * instruction.  In the global part a lane that skips a block is pointed at a
- * zero block (loads) or its own scratch block (stores) instead of being
- * branched around: every lane then issues the same memory instructions, so
- * the waits for them stay exact, and lanes outside their band touch no state
- * cache lines. */
+ * instruction.  In the global part a lane that skips a block loads a shared
+ * zero block and stores nothing, so lanes outside their band fetch no state
+ * lines and dirty none. */
 struct StateRef {
     lds_u32 *lds;          /* this lane's 8 dwords of LDS slot 0 */
     glb_u32 *glob;         /* this lane's 8 dwords of global block 0 */
     const glb_u32 *zero;   /* 32 zero bytes */
-    glb_u32 *trash;        /* 32 bytes of this lane that nothing reads */
     int nb;                /* LDS slots (blocks) */
     int nblk;              /* state blocks in the slab */
     int lb = 0;            /* first block of the LDS window */
@@ -194,9 +191,13 @@ struct StateRef {
     {
         const u32x4 x = {v[0], v[1], v[2], v[3]}, y = {v[4], v[5], v[6], v[7]};
         if (G) {
-            glb_u32x4 *p = (glb_u32x4 *)(need ? gblk(b) : trash);
-            p[0] = x;
-            p[1] = y;
+            /* lanes with nothing to store issue no store: their lines are
+             * never dirtied */
+            if (need) {
+                glb_u32x4 *p = (glb_u32x4 *)gblk(b);
+                p[0] = x;
+                p[1] = y;
+            }
         } else {
             lds_u32x4 *p = (lds_u32x4 *)lblk(b);
             p[0] = x;
@@ -626,8 +627,8 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                     S.ld8<G>(more ? b : from, v, more && b <= tl && b + 7 >= lo);
                     q = Q[(size_t)((more ? b : from) >> 3) * 64];
                 };
-                /* the pending stores (to the scratch block when there are
-                 * none; LDS stores are issued only when pending) */
+                /* the pending stores, if any (a global block only by the
+                 * lanes that need it) */
                 auto flush = [&]() {
                     if (G)
                         S.st8<true>(pb, pr, pb >= 0 && pneed);
@@ -957,14 +958,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bs
     const int lane = threadIdx.x;
     const int half = lane / Wd::GL;
     /* LDS: a window of lds_cols state columns.  The block's global slab
-     * holds the query-code words ([word][64 lanes]), the home of every state
-     * block, and one scratch block (the target of skipped stores). */
+     * holds the query-code words ([word][64 lanes]) and the home of every
+     * state block. */
     uint32_t *slab = B.slabs + (size_t)blockIdx.x * K.slab_cols * 64;
     StateRef S;
     S.lds = (lds_u32 *)lds_state + lane * 8;
     S.glob = (glb_u32 *)(slab + (size_t)K.query_words * 64 + (size_t)lane * 8);
     S.zero = (const glb_u32 *)B.zero;
-    S.trash = (glb_u32 *)(slab + (size_t)(K.slab_cols - 8) * 64 + (size_t)lane * 8);
     S.nb = K.lds_cols >> 3;
     S.nblk = (K.state_cols + 7) >> 3;
     uint32_t *Q = slab + lane;

@@ -400,8 +400,8 @@ Sizing sizing(const tal_bsw *h, int max_ncol)
     const size_t max_cols = h->lds_per_block / 256;
     if ((size_t)lds_cols > max_cols) lds_cols = (int)(max_cols & ~(size_t)7);
     z.lds_cols = lds_cols;
-    /* every state column in whole 8-column blocks, then one scratch block */
-    z.slab_cols = z.qwords + ((z.state_cols + 7) & ~7) + 8;
+    /* every state column in whole 8-column blocks */
+    z.slab_cols = z.qwords + ((z.state_cols + 7) & ~7);
     z.lds_bytes = (size_t)lds_cols * 64 * sizeof(uint32_t);
     return z;
 }
