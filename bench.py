@@ -664,13 +664,20 @@ def _end_to_end(W, stream, torch, reps=3):
     h_qer = torch.from_numpy(W["qer"]).pin_memory()
     h_out = [torch.empty(h.shape, dtype=h.dtype, pin_memory=True) for h in h_pairs]
 
+    # one explicit stream for the copies and the replays, so that each replay
+    # runs after its inputs arrive and before its outputs leave (torch's
+    # default stream has handle 0, which the library reads as "the handle's
+    # own stream")
+    s = torch.cuda.Stream()
+
     def once():
-        W["d_ref"].copy_(h_ref, non_blocking=True)
-        W["d_qer"].copy_(h_qer, non_blocking=True)
-        for c, hp, ho in zip(cs, h_pairs, h_out):
-            c["d_pairs"].copy_(hp, non_blocking=True)
-            c["plan"].launch(stream.cuda_stream)
-            ho.copy_(c["d_pairs"], non_blocking=True)
+        with torch.cuda.stream(s):
+            W["d_ref"].copy_(h_ref, non_blocking=True)
+            W["d_qer"].copy_(h_qer, non_blocking=True)
+            for c, hp, ho in zip(cs, h_pairs, h_out):
+                c["d_pairs"].copy_(hp, non_blocking=True)
+                c["plan"].launch(s.cuda_stream)
+                ho.copy_(c["d_pairs"], non_blocking=True)
 
     once()
     torch.cuda.synchronize()
