@@ -47,7 +47,6 @@ struct KBuffers {
                                    null = identity order */
     int32_t *task_counter;      /* persistent-grid work counter */
     uint32_t *slabs;            /* per-wave slabs: query words, state past LDS */
-    const uint32_t *zero;       /* 32 zero bytes (the read target of skipped blocks) */
     /* 16-bit stale-column bookkeeping (all null for the 8-bit path) */
     const uint32_t *stale_in; /* per group: lanes whose F[ncol] is nonzero */
     uint32_t *dep_out;        /* per group: lanes whose tail depended on it */

@@ -61,18 +61,36 @@ typedef unsigned short u2v __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ u2v as_u2(uint32_t x) { return __builtin_bit_cast(u2v, x); }
 __device__ __forceinline__ uint32_t u2_u32(u2v x) { return __builtin_bit_cast(uint32_t, x); }
 
-/* Reductions over the lanes of one group (xor offsets stay inside it). */
+/* Reductions over the lanes of one group, result in every lane of it: DPP
+ * steps make each 16-lane row uniform (swap neighbours, swap pairs, mirror
+ * the half-rows, mirror the row), then the row-pair swap (32-lane groups)
+ * and the half-wave swap (64-lane groups) bring in the other rows.  Every
+ * lane of a group is active or none is. */
+template <int GL, typename Op> __device__ __forceinline__ int group_reduce(int x, Op op)
+{
+    /* (every source lane of these patterns is in the row, so bound_ctrl
+     * never applies; it lets the move fold into the operation) */
+    x = op(x, __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, true));  /* quad_perm 1,0,3,2 */
+    x = op(x, __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, true));  /* quad_perm 2,3,0,1 */
+    x = op(x, __builtin_amdgcn_mov_dpp(x, 0x141, 0xF, 0xF, true)); /* row_half_mirror */
+    x = op(x, __builtin_amdgcn_mov_dpp(x, 0x140, 0xF, 0xF, true)); /* row_mirror */
+    {
+        const auto r = __builtin_amdgcn_permlane16_swap((unsigned)x, (unsigned)x, false, false);
+        x = op((int)r[0], (int)r[1]);
+    }
+    if (GL == 64) {
+        const auto r = __builtin_amdgcn_permlane32_swap((unsigned)x, (unsigned)x, false, false);
+        x = op((int)r[0], (int)r[1]);
+    }
+    return x;
+}
 template <int GL> __device__ __forceinline__ int group_max(int x)
 {
-#pragma unroll
-    for (int o = GL / 2; o > 0; o >>= 1) x = max(x, __shfl_xor(x, o, 64));
-    return x;
+    return group_reduce<GL>(x, [](int a, int b) { return max(a, b); });
 }
 template <int GL> __device__ __forceinline__ int group_min(int x)
 {
-#pragma unroll
-    for (int o = GL / 2; o > 0; o >>= 1) x = min(x, __shfl_xor(x, o, 64));
-    return x;
+    return group_reduce<GL>(x, [](int a, int b) { return min(a, b); });
 }
 
 /* Score of ref code s1 against query code s2 (raw bytes; pads are 99/100).
@@ -96,6 +114,31 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) u32x4 glb_u32x4;
 typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
 
+/* Buffer resource of one wave's slab.  Slab accesses take 32-bit byte
+ * offsets; an offset of OOB lies past every slab, where the hardware's range
+ * check turns a load into zeros and drops a store, so a lane with nothing to
+ * read or write in a block gives that offset instead of branching. */
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+constexpr uint32_t OOB = 0x80000000u;
+constexpr int RSRC_WORD3 = 0x00020000; /* raw dword buffer (gfx9 family) */
+
+__device__ __forceinline__ uint32_t bld32(rsrc_t r, uint32_t off)
+{
+    return __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void bst32(rsrc_t r, uint32_t off, uint32_t v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(v, r, (int)off, 0, 0);
+}
+__device__ __forceinline__ u32x4 bld128(rsrc_t r, uint32_t off)
+{
+    return __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void bst128(rsrc_t r, uint32_t off, u32x4 v)
+{
+    __builtin_amdgcn_raw_buffer_store_b128(v, r, (int)off, 0, 0);
+}
+
 /* One dword of state per (column, lane), in 8-column blocks of each lane:
  * [block][64 lanes][8 columns], so that a lane's block is 32 contiguous
  * bytes.  Every block has a home in the wave's global slab; a window of `nb`
@@ -105,17 +148,17 @@ typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
  * LDS holds the part of every row's range that fits, instead of a fixed set
  * of columns that later rows leave behind.  The row sweep runs over the LDS
  * blocks and then over the global blocks, each with its own kind of memory
- * instruction.  In the global part a lane that skips a block loads a shared
- * zero block and stores nothing, so lanes outside their band fetch no state
+ * instruction.  In the global part a lane that skips a block reads zeros and
+ * stores nothing (offset OOB), so lanes outside their band fetch no state
  * lines and dirty none. */
 struct StateRef {
-    lds_u32 *lds;          /* this lane's 8 dwords of LDS slot 0 */
-    glb_u32 *glob;         /* this lane's 8 dwords of global block 0 */
-    const glb_u32 *zero;   /* 32 zero bytes */
-    int nb;                /* LDS slots (blocks) */
-    int nblk;              /* state blocks in the slab */
-    int lb = 0;            /* first block of the LDS window */
-    int r0 = 0;            /* its slot */
+    lds_u32 *lds;   /* this lane's 8 dwords of LDS slot 0 */
+    rsrc_t slab;    /* the wave's slab */
+    uint32_t gbase; /* byte offset of this lane's 8 dwords of global block 0 */
+    int nb;         /* LDS slots (blocks) */
+    int nblk;       /* state blocks in the slab */
+    int lb = 0;     /* first block of the LDS window */
+    int r0 = 0;     /* its slot */
     __device__ __forceinline__ bool in_lds(int blk) const
     {
         return (unsigned)(blk - lb) < (unsigned)nb;
@@ -125,21 +168,55 @@ struct StateRef {
         const int s = blk - lb + r0;
         return s >= nb ? s - nb : s;
     }
-    /* generic address of column c (single accesses outside the sweep) */
-    __device__ __forceinline__ uint32_t *at(int c) const
+    /* byte offset of the global block at column b (a multiple of 8) */
+    __device__ __forceinline__ uint32_t goff(int b) const { return gbase + ((uint32_t)b << 8); }
+    /* single columns outside the sweep */
+    __device__ __forceinline__ uint32_t ld(int c) const
     {
         const int blk = c >> 3;
-        return in_lds(blk) ? (uint32_t *)(lds + ((uint32_t)slot(blk) << 9) + (c & 7))
-                           : (uint32_t *)(glob + ((uint32_t)blk << 9) + (c & 7));
+        if (in_lds(blk)) return lds[((uint32_t)slot(blk) << 9) + (c & 7)];
+        return bld32(slab, goff(c & ~7) + ((uint32_t)(c & 7) << 2));
     }
-    __device__ __forceinline__ uint32_t ld(int c) const { return *at(c); }
-    __device__ __forceinline__ void st(int c, uint32_t v) const { *at(c) = v; }
-    /* the block at column b (a multiple of 8) in LDS / in the slab */
+    __device__ __forceinline__ void st(int c, uint32_t v) const
+    {
+        const int blk = c >> 3;
+        if (in_lds(blk))
+            lds[((uint32_t)slot(blk) << 9) + (c & 7)] = v;
+        else
+            bst32(slab, goff(c & ~7) + ((uint32_t)(c & 7) << 2), v);
+    }
+    /* the block at column b (a multiple of 8) in LDS */
     __device__ __forceinline__ lds_u32 *lblk(int b) const
     {
         return lds + ((uint32_t)slot(b >> 3) << 9);
     }
-    __device__ __forceinline__ glb_u32 *gblk(int b) const { return glob + ((uint32_t)b << 6); }
+    /* whole blocks wherever they live (outside the sweep) */
+    __device__ __forceinline__ void ld8any(int b, uint32_t *v) const
+    {
+        u32x4 x, y;
+        if (in_lds(b >> 3)) {
+            const lds_u32x4 *p = (const lds_u32x4 *)lblk(b);
+            x = p[0];
+            y = p[1];
+        } else {
+            x = bld128(slab, goff(b));
+            y = bld128(slab, goff(b) + 16);
+        }
+        v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+        v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
+    }
+    __device__ __forceinline__ void st8any(int b, const uint32_t *v) const
+    {
+        const u32x4 x = {v[0], v[1], v[2], v[3]}, y = {v[4], v[5], v[6], v[7]};
+        if (in_lds(b >> 3)) {
+            lds_u32x4 *p = (lds_u32x4 *)lblk(b);
+            p[0] = x;
+            p[1] = y;
+        } else {
+            bst128(slab, goff(b), x);
+            bst128(slab, goff(b) + 16, y);
+        }
+    }
     /* column where the LDS window ends */
     __device__ __forceinline__ int wend() const { return (lb + nb) << 3; }
     /* the window back to block 0 (a new group) */
@@ -155,14 +232,13 @@ struct StateRef {
     {
         while (lb < to) {
             lds_u32x4 *p = (lds_u32x4 *)(lds + ((uint32_t)r0 << 9));
-            glb_u32x4 *g = (glb_u32x4 *)(glob + ((uint32_t)lb << 9));
-            const u32x4 x = p[0], y = p[1];
-            g[0] = x;
-            g[1] = y;
+            const uint32_t o = goff(lb << 3);
+            bst128(slab, o, p[0]);
+            bst128(slab, o + 16, p[1]);
             const int in = lb + nb;
             if (in < nblk) {
-                const glb_u32x4 *h = (const glb_u32x4 *)(glob + ((uint32_t)in << 9));
-                const u32x4 a = h[0], c = h[1];
+                const uint32_t oi = goff(in << 3);
+                const u32x4 a = bld128(slab, oi), c = bld128(slab, oi + 16);
                 p[0] = a;
                 p[1] = c;
             }
@@ -170,14 +246,15 @@ struct StateRef {
             r0 = r0 + 1 == nb ? 0 : r0 + 1;
         }
     }
-    /* the 8 columns of the aligned block at b; G: the block is global */
+    /* the 8 columns of the aligned block at b; G: the block is global, read
+     * only where `need` (zeros elsewhere) */
     template <bool G> __device__ __forceinline__ void ld8(int b, uint32_t *v, bool need) const
     {
         u32x4 x, y;
         if (G) {
-            const glb_u32x4 *p = (const glb_u32x4 *)(need ? (const glb_u32 *)gblk(b) : zero);
-            x = p[0];
-            y = p[1];
+            const uint32_t o = need ? goff(b) : OOB;
+            x = bld128(slab, o);
+            y = bld128(slab, o + 16);
         } else {
             const lds_u32x4 *p = (const lds_u32x4 *)lblk(b);
             x = p[0];
@@ -186,18 +263,15 @@ struct StateRef {
         v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
         v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
     }
-    /* `need` false: the block already holds these values for this lane */
+    /* a global block is written only where `need`: lanes with nothing to
+     * store issue a dropped store, so their lines are never dirtied */
     template <bool G> __device__ __forceinline__ void st8(int b, const uint32_t *v, bool need) const
     {
         const u32x4 x = {v[0], v[1], v[2], v[3]}, y = {v[4], v[5], v[6], v[7]};
         if (G) {
-            /* lanes with nothing to store issue no store: their lines are
-             * never dirtied */
-            if (need) {
-                glb_u32x4 *p = (glb_u32x4 *)gblk(b);
-                p[0] = x;
-                p[1] = y;
-            }
+            const uint32_t o = need ? goff(b) : OOB;
+            bst128(slab, o, x);
+            bst128(slab, o + 16, y);
         } else {
             lds_u32x4 *p = (lds_u32x4 *)lblk(b);
             p[0] = x;
@@ -232,11 +306,11 @@ template <int BITS> __device__ __forceinline__ int table_score(int s1, int c, in
  * the reference's smithWaterman512_{16,8} row loop with its batch wrapper's
  * per-group setup folded in.
  *
- * Q is the LDS area of the fast path's query codes ([word][64 lanes]), or
- * null when the launch carries none.
+ * qb is the byte offset of this lane's first query-code word in the slab
+ * (words [word][64 lanes], at the head of the slab).
  */
 template <int BITS>
-__device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRef &S, uint32_t *Q)
+__device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRef &S, uint32_t qb)
 {
     typedef Width<BITS> Wd;
     constexpr int GL = Wd::GL;
@@ -351,9 +425,7 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                 v[k] = c <= ncol ? pack(hv, f) : 0u;
                 if (v[k] != 0u) nzhi = c;
             }
-            u32x4 *p = (u32x4 *)S.at(b);
-            p[0] = (u32x4){v[0], v[1], v[2], v[3]};
-            p[1] = (u32x4){v[4], v[5], v[6], v[7]};
+            S.st8any(b, v);
         }
     }
     /* ---- query codes for the fast path ---- */
@@ -378,7 +450,7 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                 }
                 word |= (code & 15u) << (4 * k);
             }
-            Q[(size_t)wd * 64] = word;
+            bst32(S.slab, qb + ((uint32_t)wd << 8), word);
         }
         fast = (ballot(!ok) & gmask) == 0;
     }
@@ -470,19 +542,21 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
             for (int c = 4; c < 6; c++)
                 thi |= ((uint32_t)table_score<BITS>(s1, c, raw_match, raw_mism) & 255u)
                        << (8 * (c - 4));
-            /* The E and F recurrences have the same form,
-             *   e' = max(m - oe_ins, 0, e - e_ins),  f' = max(m - oe_del, 0, f - e_del),
-             * so they run as one packed pair: ez holds (e, f') and the state
-             * dword (H_h, F). */
+            /* E and F:  e' = max(m - oe_ins, 0, e - e_ins),
+             *           f' = max(m - oe_del, 0, f - e_del),
+             * the max with 0 folded into an unsigned saturating subtraction
+             * (e, f >= 0). */
+            const s2v OEd2 = {oe_del_s, oe_del_s}, OEi2 = {oe_ins_s, oe_ins_s};
+            const u2v EDu = {(uint16_t)e_del_s, (uint16_t)e_del_s};
             const s2v OE = {oe_ins_s, oe_del_s};
             const s2v EE = {e_ins_s, e_del_s};
             const s2v ZERO2 = {0, 0};
-            const s2v OEd2 = {oe_del_s, oe_del_s}, OEi2 = {oe_ins_s, oe_ins_s};
-            const u2v EDu = {(uint16_t)e_del_s, (uint16_t)e_del_s};
             uint32_t ez = 0;                       /* low half: E entering the column */
             uint32_t hprev = (uint32_t)(uint16_t)h10; /* low half: H of the previous column */
-            /* sbt: the score in the low half of a dword (sign-extended);
-             * only low halves are meaningful in m, h11 and the adds below */
+            /* one column of the edge blocks; sbt: the score in the low half
+             * of a dword (sign-extended); only low halves are meaningful in
+             * m, h11 and the adds below.  E and F run as one packed pair: ez
+             * holds (e, f'), the state dword (H_h, F). */
             auto cell = [&](uint32_t sv, uint32_t sbt) -> uint32_t {
                 const uint32_t m = (sv & 0xFFFFu) == 0u ? 0u : sv + sbt;
                 /* (e, f) of this column: e from ez, f from the state */
@@ -522,8 +596,7 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
             /* first and last 8-column block holding a nonzero store */
             int bfirst = NONE_COL, blast = -1;
             /* 8-column blocks aligned to multiples of 8 (one query word,
-             * static code positions).  Edge blocks guard each column with
-             * beg <= j < end; middle blocks run unguarded. */
+             * static code positions). */
             /* This lane reads a block iff it meets [lo, tl] and stores it iff
              * it meets [lo, max(tl, nzhi)], lo = min(hd, pend + 1).  Below
              * lo the state is zero: the previous row stored [pbeg, pend] with
@@ -601,6 +674,10 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                         pb = b;
                         pneed = b <= sthi && b + 7 >= lo;
                     } else {
+                        /* The row's first and last block: column by column,
+                         * only the columns in [beg, end), each stored at
+                         * once.  A lane that skipped loading a global block
+                         * has it outside [lo, tl] and stores masked zeros. */
 #pragma unroll
                         for (int k = 0; k < 8; k++)
                             if (b + k >= beg && b + k < end) {
@@ -608,7 +685,7 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                                     cell(sv[k], score((k & 1) ? sco : sce, k >> 1));
                                 const uint32_t r = rel0 + k < lim1 ? st : 0u;
                                 if (G)
-                                    S.gblk(b)[k] = r;
+                                    bst32(S.slab, S.goff(b) + 4u * k, r);
                                 else
                                     S.lblk(b)[k] = r;
                                 any |= r;
@@ -621,11 +698,16 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                         blast = b;
                     }
                 };
-                /* the next block's loads, issued for every lane (a lane past
-                 * its part reads the zero block or its first LDS block) */
-                auto load = [&](int b, bool more, uint32_t *v, uint32_t &q) {
-                    S.ld8<G>(more ? b : from, v, more && b <= tl && b + 7 >= lo);
-                    q = Q[(size_t)((more ? b : from) >> 3) * 64];
+                /* a block's loads, issued for every lane: in the slab a lane
+                 * reads only a block it may need (zeros elsewhere, and past
+                 * the sweep); in LDS a block one past the window reads the
+                 * window's first slot, a valid address */
+                auto load = [&](int b, uint32_t *v, uint32_t &q) {
+                    if (G)
+                        S.ld8<true>(b, v, b < to && b + 7 >= lo && b <= tl);
+                    else
+                        S.ld8<false>(b, v, true);
+                    q = bld32(S.slab, qb + ((uint32_t)b << 5));
                 };
                 /* the pending stores, if any (a global block only by the
                  * lanes that need it) */
@@ -642,17 +724,17 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                  * they were issued, and never covers later stores. */
                 if (from < to && !G) {
                     uint32_t A[8], Bv[8];
-                    uint32_t qa = 0, qb = 0;
-                    load(from, true, A, qa);
+                    uint32_t qa = 0, qbv = 0;
+                    load(from, A, qa);
                     for (int b = from;;) {
-                        load(b + 8, b + 8 < to, Bv, qb);
+                        load(b + 8, Bv, qbv);
                         flush();
                         do_block(b, A, qa);
                         b += 8;
                         if (b >= to) break;
-                        load(b + 8, b + 8 < to, A, qa);
+                        load(b + 8, A, qa);
                         flush();
-                        do_block(b, Bv, qb);
+                        do_block(b, Bv, qbv);
                         b += 8;
                         if (b >= to) break;
                     }
@@ -661,21 +743,21 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                      * so that the memory latency is covered by two blocks of
                      * computation */
                     uint32_t A[8], Bv[8], C[8];
-                    uint32_t qa = 0, qb = 0, qc = 0;
-                    load(from, true, A, qa);
-                    load(from + 8, from + 8 < to, Bv, qb);
+                    uint32_t qa = 0, qbv = 0, qc = 0;
+                    load(from, A, qa);
+                    load(from + 8, Bv, qbv);
                     for (int b = from;;) {
-                        load(b + 16, b + 16 < to, C, qc);
+                        load(b + 16, C, qc);
                         flush();
                         do_block(b, A, qa);
                         b += 8;
                         if (b >= to) break;
-                        load(b + 16, b + 16 < to, A, qa);
+                        load(b + 16, A, qa);
                         flush();
-                        do_block(b, Bv, qb);
+                        do_block(b, Bv, qbv);
                         b += 8;
                         if (b >= to) break;
-                        load(b + 16, b + 16 < to, Bv, qb);
+                        load(b + 16, Bv, qbv);
                         flush();
                         do_block(b, C, qc);
                         b += 8;
@@ -704,17 +786,22 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
             /* exact first/last nonzero column of [beg, end) from the two
              * flagged blocks (one 8-column read each) */
             if (bfirst != NONE_COL) {
-                /* both blocks whole (each lies in LDS or in the slab) */
-                const u32x4 *pf = (const u32x4 *)S.at(bfirst), *pl = (const u32x4 *)S.at(blast);
-                const u32x4 f0 = pf[0], f1 = pf[1], l0 = pl[0], l1 = pl[1];
-                const uint32_t vf[8] = {f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, f1.z, f1.w};
-                const uint32_t vl[8] = {l0.x, l0.y, l0.z, l0.w, l1.x, l1.y, l1.z, l1.w};
+                uint32_t vf[8], vl[8];
+                S.ld8any(bfirst, vf);
+                S.ld8any(blast, vl);
                 uint32_t nzf = 0, nzl = 0;
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
-                    if (bfirst + k >= beg && bfirst + k < end && vf[k] != 0u) nzf |= 1u << k;
-                    if (blast + k >= beg && blast + k < end && vl[k] != 0u) nzl |= 1u << k;
+                    nzf |= min(vf[k], 1u) << k;
+                    nzl |= min(vl[k], 1u) << k;
                 }
+                /* columns [beg, end) of each block as a bit range */
+                auto range = [&](int bb) -> uint32_t {
+                    const int l = min(max(beg - bb, 0), 8), h = min(max(end - bb, 0), 8);
+                    return ((1u << h) - 1u) & ~((1u << l) - 1u);
+                };
+                nzf &= range(bfirst);
+                nzl &= range(blast);
                 fz = bfirst + __builtin_ctz(nzf);
                 lz = blast + 31 - __builtin_clz(nzl);
             }
@@ -936,9 +1023,8 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
         }
         uint32_t *fm = B.fmask + B.fmask_off[g];
         for (int b = 0; b < ncol; b += 8) {
-            const u32x4 *p = (const u32x4 *)S.at(b);
-            const u32x4 x = p[0], y = p[1];
-            const uint32_t v[8] = {x.x, x.y, x.z, x.w, y.x, y.y, y.z, y.w};
+            uint32_t v[8];
+            S.ld8any(b, v);
 #pragma unroll
             for (int k = 0; k < 8; k++) {
                 const int nz_o = __builtin_amdgcn_ds_bpermute(inv * 4, (v[k] >> 16) != 0u ? 1 : 0);
@@ -963,11 +1049,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bs
     uint32_t *slab = B.slabs + (size_t)blockIdx.x * K.slab_cols * 64;
     StateRef S;
     S.lds = (lds_u32 *)lds_state + lane * 8;
-    S.glob = (glb_u32 *)(slab + (size_t)K.query_words * 64 + (size_t)lane * 8);
-    S.zero = (const glb_u32 *)B.zero;
+    S.slab = __builtin_amdgcn_make_buffer_rsrc(slab, 0, K.slab_cols * 64 * (int)sizeof(uint32_t),
+                                               RSRC_WORD3);
+    S.gbase = ((uint32_t)K.query_words * 64u + (uint32_t)lane * 8u) * 4u;
     S.nb = K.lds_cols >> 3;
     S.nblk = (K.state_cols + 7) >> 3;
-    uint32_t *Q = slab + lane;
+    const uint32_t qb = (uint32_t)lane * 4u;
     if (B.abort_flag && *B.abort_flag != 0) return;
     const int ntasks = B.ntasks_dev ? *B.ntasks_dev : K.ntasks;
     for (;;) {
@@ -982,7 +1069,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bs
             g = t * Wd::GPW + half;
             if (g >= K.ngroups) g = -1;
         }
-        run_group<BITS>(K, B, g, S, Q);
+        run_group<BITS>(K, B, g, S, qb);
     }
 }
 

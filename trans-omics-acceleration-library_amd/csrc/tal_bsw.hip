@@ -144,8 +144,7 @@ struct tal_bsw {
     int order = 1;         /* task order: 0 = input order, 1 = longest groups first */
     int plan_rounds = 2;   /* stale-column rerun rounds captured in a plan's graph */
     DevBuf image; /* IMAGE_COLS u32: F[c] nonzero lanes of the 16-bit scratch */
-    DevBuf zero_image; /* the image of a freshly constructed object; its first
-                          bytes double as the kernels' zero block */
+    DevBuf zero_image; /* the image of a freshly constructed object */
     PathBufs b16, b8;
     /* dispatcher scratch */
     DevBuf flag, idx_all, idx_sel, nsel, pairs8, cubtmp;
@@ -517,7 +516,6 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
         B.task_groups = order;
     }
     TAKE(h, pb.counter, sizeof(int32_t) * 4, "alloc counter", &B.task_counter);
-    B.zero = h->zero_image.as<uint32_t>();
     int32_t *list = nullptr, *tasks = nullptr, *last = nullptr;
     uint32_t *stale = nullptr;
     if (bits == 16) {
@@ -720,7 +718,6 @@ int plan_width(tal_bsw *h, tal_bsw_plan *P, int bits, hipStream_t s)
     HIPCHK(h, pb.slabs.ensure((size_t)W.grid * W.z.slab_cols * 64 * sizeof(uint32_t)),
            "alloc state slabs");
     B.slabs = pb.slabs.as<uint32_t>();
-    B.zero = h->zero_image.as<uint32_t>();
     if (bits == 8) {
         B.last_len2_dev = P->ctl.as<int32_t>() + 1;
     } else {
