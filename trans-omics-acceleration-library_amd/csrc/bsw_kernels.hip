@@ -246,37 +246,22 @@ struct StateRef {
             r0 = r0 + 1 == nb ? 0 : r0 + 1;
         }
     }
-    /* the 8 columns of the aligned block at b; G: the block is global, read
-     * only where `need` (zeros elsewhere) */
-    template <bool G> __device__ __forceinline__ void ld8(int b, uint32_t *v, bool need) const
+    /* the 8 columns of the global block at b (a multiple of 8), read only
+     * where `need` (zeros elsewhere) */
+    __device__ __forceinline__ void gld8(int b, uint32_t *v, bool need) const
     {
-        u32x4 x, y;
-        if (G) {
-            const uint32_t o = need ? goff(b) : OOB;
-            x = bld128(slab, o);
-            y = bld128(slab, o + 16);
-        } else {
-            const lds_u32x4 *p = (const lds_u32x4 *)lblk(b);
-            x = p[0];
-            y = p[1];
-        }
+        const uint32_t o = need ? goff(b) : OOB;
+        const u32x4 x = bld128(slab, o), y = bld128(slab, o + 16);
         v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
         v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
     }
-    /* a global block is written only where `need`: lanes with nothing to
-     * store issue a dropped store, so their lines are never dirtied */
-    template <bool G> __device__ __forceinline__ void st8(int b, const uint32_t *v, bool need) const
+    /* written only where `need`: lanes with nothing to store issue a
+     * dropped store, so their lines are never dirtied */
+    __device__ __forceinline__ void gst8(int b, const uint32_t *v, bool need) const
     {
-        const u32x4 x = {v[0], v[1], v[2], v[3]}, y = {v[4], v[5], v[6], v[7]};
-        if (G) {
-            const uint32_t o = need ? goff(b) : OOB;
-            bst128(slab, o, x);
-            bst128(slab, o + 16, y);
-        } else {
-            lds_u32x4 *p = (lds_u32x4 *)lblk(b);
-            p[0] = x;
-            p[1] = y;
-        }
+        const uint32_t o = need ? goff(b) : OOB;
+        bst128(slab, o, (u32x4){v[0], v[1], v[2], v[3]});
+        bst128(slab, o + 16, (u32x4){v[4], v[5], v[6], v[7]});
     }
 };
 
@@ -430,11 +415,11 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
     }
     /* ---- query codes for the fast path ---- */
     /* Every int8 score fits the per-row byte table.  The fast path needs
-     * query bytes 0-4, nonnegative insertion penalties (it relies on a run of
-     * zero cells keeping E at zero) and nonnegative gap extensions (E and F
+     * query bytes 0-4, nonnegative gap open+extend penalties (a run of zero
+     * cells keeps E and F at zero) and nonnegative gap extensions (E and F
      * are nonnegative, so e - e_ins and f - e_del then never wrap and their
      * max with 0 is an unsigned saturating subtraction). */
-    bool fast = oe_ins_s >= 0 && e_ins_s >= 0 && e_del_s >= 0;
+    bool fast = oe_ins_s >= 0 && oe_del_s >= 0 && e_ins_s >= 0 && e_del_s >= 0;
     {
         bool ok = true;
         const int nwords = (ncol + 1 + 7) / 8 + 1;
@@ -615,7 +600,7 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                 /* one block: compute and store (a middle block's stores are
                  * left pending in pr/pb), fold its row-max key and nonzero
                  * flag into the row's */
-                auto do_block = [&](int b, const uint32_t *sv, uint32_t qw) {
+                auto do_block = [&](int b, const uint32_t *sv, uint32_t qw, lds_u32 *lp) {
                     /* scores of columns b+0,2,4,6 (sce) and b+1,3,5,7 (sco) */
                     const uint32_t sce = __builtin_amdgcn_perm(thi, tlo, qw & 0x0F0F0F0Fu);
                     const uint32_t sco =
@@ -629,6 +614,16 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                          * every instruction; only E runs column by column.
                          * m = H + s, or 0 when H == 0, as t*s + H with
                          * t = min(H, 1). */
+                        /* The band mask.  Inside [beg, end) the state
+                         * left of a lane's head is zero, except at column
+                         * pend+1 (not stored last row; end <= pend + 2, so it
+                         * can only be column end-1), and zero inputs give
+                         * zero stores here: E and F stay zero (oe_ins,
+                         * oe_del >= 0) and the block is not the row's first,
+                         * so no boundary H enters it.  Columns b..b+6 are
+                         * thus masked at the tail only, column b+7 at both
+                         * ends. */
+                        const int tb = tl - b;
                         uint32_t e = ez, hp = hprev;
 #pragma unroll
                         for (int q = 0; q < 4; q++) {
@@ -656,8 +651,8 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                             const uint32_t h2 = as_u32(max2(max2(m, as_s2(u2_u32(F))), E2));
                             const uint32_t st0 = __builtin_amdgcn_perm(as_u32(fn), hp, 0x05040100u);
                             const uint32_t st1 = __builtin_amdgcn_perm(as_u32(fn), h2, 0x07060100u);
-                            pr[k] = rel0 + k < lim1 ? st0 : 0u;
-                            pr[k + 1] = rel0 + k + 1 < lim1 ? st1 : 0u;
+                            pr[k] = k <= tb ? st0 : 0u;
+                            pr[k + 1] = (q < 3 ? k + 1 <= tb : rel0 + 7 < lim1) ? st1 : 0u;
                             hp = h2 >> 16;
                             e = e2;
                         }
@@ -687,7 +682,7 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                                 if (G)
                                     bst32(S.slab, S.goff(b) + 4u * k, r);
                                 else
-                                    S.lblk(b)[k] = r;
+                                    lp[k] = r;
                                 any |= r;
                                 if (b + k > beg) kb = max(kb, (r << 16) | (uint32_t)k);
                             }
@@ -698,24 +693,28 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                         blast = b;
                     }
                 };
-                /* a block's loads, issued for every lane: in the slab a lane
+                /* a global block's loads, issued for every lane: a lane
                  * reads only a block it may need (zeros elsewhere, and past
-                 * the sweep); in LDS a block one past the window reads the
-                 * window's first slot, a valid address */
+                 * the sweep) */
                 auto load = [&](int b, uint32_t *v, uint32_t &q) {
-                    if (G)
-                        S.ld8<true>(b, v, b < to && b + 7 >= lo && b <= tl);
-                    else
-                        S.ld8<false>(b, v, true);
+                    S.gld8(b, v, b < to && b + 7 >= lo && b <= tl);
                     q = bld32(S.slab, qb + ((uint32_t)b << 5));
                 };
+                /* LDS part: the blocks from the window start on sit in
+                 * consecutive ring slots, so their addresses step by one
+                 * slot (a block one past the window reads the window's first
+                 * slot, a valid address) */
+                lds_u32 *pp = nullptr; /* LDS address of the pending block */
                 /* the pending stores, if any (a global block only by the
                  * lanes that need it) */
                 auto flush = [&]() {
-                    if (G)
-                        S.st8<true>(pb, pr, pb >= 0 && pneed);
-                    else if (pb >= 0)
-                        S.st8<false>(pb, pr, true);
+                    if (G) {
+                        S.gst8(pb, pr, pb >= 0 && pneed);
+                    } else if (pb >= 0) {
+                        lds_u32x4 *p = (lds_u32x4 *)pp;
+                        p[0] = (u32x4){pr[0], pr[1], pr[2], pr[3]};
+                        p[1] = (u32x4){pr[4], pr[5], pr[6], pr[7]};
+                    }
                     pb = -1;
                 };
                 /* Two buffers used in turn: a block's loads land straight in
@@ -725,16 +724,29 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                 if (from < to && !G) {
                     uint32_t A[8], Bv[8];
                     uint32_t qa = 0, qbv = 0;
-                    load(from, A, qa);
+                    int sl = S.r0; /* slot of the next block to load (from = the window start) */
+                    auto lload = [&](int b, uint32_t *v, uint32_t &q) -> lds_u32 * {
+                        lds_u32 *p = S.lds + ((uint32_t)sl << 9);
+                        sl = sl + 1 == S.nb ? 0 : sl + 1;
+                        const lds_u32x4 *p4 = (const lds_u32x4 *)p;
+                        const u32x4 x = p4[0], y = p4[1];
+                        v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+                        v[4] = y.x; v[5] = y.y; v[6] = y.z; v[7] = y.w;
+                        q = bld32(S.slab, qb + ((uint32_t)b << 5));
+                        return p;
+                    };
+                    lds_u32 *pa = lload(from, A, qa), *pbv;
                     for (int b = from;;) {
-                        load(b + 8, Bv, qbv);
+                        pbv = lload(b + 8, Bv, qbv);
                         flush();
-                        do_block(b, A, qa);
+                        do_block(b, A, qa, pa);
+                        pp = pa;
                         b += 8;
                         if (b >= to) break;
-                        load(b + 8, A, qa);
+                        pa = lload(b + 8, A, qa);
                         flush();
-                        do_block(b, Bv, qbv);
+                        do_block(b, Bv, qbv, pbv);
+                        pp = pbv;
                         b += 8;
                         if (b >= to) break;
                     }
@@ -749,17 +761,17 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                     for (int b = from;;) {
                         load(b + 16, C, qc);
                         flush();
-                        do_block(b, A, qa);
+                        do_block(b, A, qa, nullptr);
                         b += 8;
                         if (b >= to) break;
                         load(b + 16, A, qa);
                         flush();
-                        do_block(b, Bv, qbv);
+                        do_block(b, Bv, qbv, nullptr);
                         b += 8;
                         if (b >= to) break;
                         load(b + 16, Bv, qbv);
                         flush();
-                        do_block(b, C, qc);
+                        do_block(b, C, qc, nullptr);
                         b += 8;
                         if (b >= to) break;
                     }
