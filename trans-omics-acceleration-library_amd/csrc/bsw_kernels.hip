@@ -423,19 +423,28 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
     {
         bool ok = true;
         const int nwords = (ncol + 1 + 7) / 8 + 1;
-        for (int wd = 0; wd < nwords; wd++) {
-            uint32_t word = 0;
-            for (int k = 0; k < 8; k++) {
+        /* 16 byte loads in flight at a time; columns past the query read a
+         * byte of the pair array (any valid address) and become the pad */
+        const uint8_t *qsafe = (const uint8_t *)B.pairs;
+        for (int wd = 0; wd < nwords; wd += 2) {
+            uint32_t by[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
                 const int c = wd * 8 + k;
-                uint32_t code = QPAD;
-                if (c < len2) {
-                    const uint32_t b = qseq[c];
-                    ok = ok && b <= 4u;
-                    code = b;
-                }
-                word |= (code & 15u) << (4 * k);
+                by[k] = *(c < len2 ? qseq + c : qsafe);
             }
-            bst32(S.slab, qb + ((uint32_t)wd << 8), word);
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                uint32_t word = 0;
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    const int c = (wd + h) * 8 + k;
+                    const bool in = c < len2;
+                    ok = ok && (!in || by[8 * h + k] <= 4u);
+                    word |= ((in ? by[8 * h + k] : QPAD) & 15u) << (4 * k);
+                }
+                if (wd + h < nwords) bst32(S.slab, qb + ((uint32_t)(wd + h) << 8), word);
+            }
         }
         fast = (ballot(!ok) & gmask) == 0;
     }
@@ -624,16 +633,29 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                          * thus masked at the tail only, column b+7 at both
                          * ends. */
                         const int tb = tl - b;
+                        /* 16-bit: the scores shifted up a byte, so that
+                         * every score has an odd byte position in one of
+                         * the two sources (the permute's sign selectors
+                         * 8-11 replicate the top bit of bytes 1, 3, 5, 7) */
+                        const uint32_t sce1 = sce << 8, sco1 = sco << 8;
                         uint32_t e = ez, hp = hprev;
 #pragma unroll
                         for (int q = 0; q < 4; q++) {
                             const int k = 2 * q;
-                            /* scores of columns k, k+1 in the high bytes of
-                             * the two halves (value << 8) */
-                            const uint32_t sel = 0x000C000Cu | ((uint32_t)q << 8) |
-                                                 ((uint32_t)(4 + q) << 24);
-                            const uint32_t sp8 = __builtin_amdgcn_perm(sco, sce, sel);
-                            const s2v S = BITS == 16 ? (as_s2(sp8) >> 8) : as_s2(sp8);
+                            /* scores of columns k, k+1 in the two halves:
+                             * 16-bit sign-extended (each byte with its sign
+                             * byte), 8-bit as value << 8 (each byte above a
+                             * zero byte) */
+                            s2v S;
+                            if (BITS == 16) {
+                                const uint32_t sel = q < 2 ? 0x0A050801u : 0x0B070903u;
+                                S = as_s2((q & 1) ? __builtin_amdgcn_perm(sco, sce, sel)
+                                                  : __builtin_amdgcn_perm(sco1, sce1, sel));
+                            } else {
+                                const uint32_t sel = 0x000C000Cu | ((uint32_t)q << 8) |
+                                                     ((uint32_t)(4 + q) << 24);
+                                S = as_s2(__builtin_amdgcn_perm(sco, sce, sel));
+                            }
                             const u2v H = as_u2(__builtin_amdgcn_perm(sv[k + 1], sv[k], 0x05040100u));
                             const u2v F = as_u2(__builtin_amdgcn_perm(sv[k + 1], sv[k], 0x07060302u));
                             uint32_t t, mw;
