@@ -269,21 +269,36 @@ int tal_bsw_set_tuning(tal_bsw *h, int32_t waves_per_cu, int32_t order);
  * call on a freshly constructed handle: its groups never mix with other
  * requests' pairs and its stale-column state starts from zero, so its outputs
  * do not depend on what it was batched with.
+ *
+ * With resident = 1 no batch is launched: one persistent kernel stays on the
+ * GPU and takes tasks (a pair of 16-bit groups, an 8-bit group, or a
+ * request's stale-column resolution) from a ring in pinned host memory.
+ * Admission copies a request's sequences into device memory under the same
+ * budget (its device footprint is the sequences plus its 16-bit F masks)
+ * and publishes its tasks at once, so a request admitted while others run is
+ * taken up by waves as they free up.  Queries longer than max_query_len are
+ * TAL_ERANGE in this mode.  The kernel leaves after 50 ms without work and
+ * is launched again when work arrives; the same independence holds.
  */
 typedef struct tal_bsw_server tal_bsw_server;
 
 typedef struct tal_bsw_server_config {
     size_t hbm_budget;       /* device bytes for admitted requests, < 2 GiB
                                 (sequence offsets are int32); 0 = 1 GiB */
-    int32_t workers;         /* batches on the GPU at once; 0 = 2 */
-    int32_t max_batch_pairs; /* pairs per batch at most; 0 = 4194304 */
+    int32_t workers;         /* batches on the GPU at once; 0 = 2 (host-batched mode) */
+    int32_t max_batch_pairs; /* pairs per batch at most; 0 = 4194304 (host-batched mode) */
+    int32_t resident;        /* 1: the device-resident loop; 0: host-batched */
+    int32_t max_query_len;   /* resident mode: longest query; 0 = 1024 */
 } tal_bsw_server_config;
 
 /* Per-request record filled by tal_bsw_server_wait. */
 typedef struct tal_bsw_request_info {
     int32_t status;     /* TAL_OK or the request's error code */
-    int32_t batch;      /* batch the request ran in (-1 if it did not run) */
-    int32_t batch_requests; /* requests in that batch */
+    int32_t batch;      /* batch the request ran in (-1 if it did not run;
+                           resident mode: 0 if it ran) */
+    int32_t batch_requests; /* requests in that batch (resident mode: requests
+                               on the device when it was admitted, itself
+                               included) */
     int32_t pad;
     double queue_ms;    /* submit -> admitted (copied to the device) */
     double latency_ms;  /* submit -> outputs written into the caller's pairs */
@@ -292,9 +307,11 @@ typedef struct tal_bsw_request_info {
 typedef struct tal_bsw_server_stats {
     int64_t requests;   /* completed requests */
     int64_t pairs;      /* pairs of the completed requests */
-    int64_t batches;    /* batches run */
+    int64_t batches;    /* batches run (resident mode: kernel launches) */
     int64_t peak_bytes; /* peak device bytes held by admitted requests */
-    double busy_ms;     /* summed wall time of the batches */
+    double busy_ms;     /* summed wall time of the batches (host-batched mode) */
+    int64_t reruns;     /* resident mode: 16-bit groups run again after a
+                           stale-column resolution */
 } tal_bsw_server_stats;
 
 int tal_bsw_server_create(const tal_bsw_params *params, int device,

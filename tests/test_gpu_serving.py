@@ -1,16 +1,19 @@
-"""GPU tests of the serving loop (tal_bsw_server_*): every request's outputs
-are the same alone and in the mix, bit-exact, whatever batch it ran in.
+"""GPU tests of the serving loop (tal_bsw_server_*), host-batched and
+device-resident: every request's outputs are the same alone and in the mix,
+bit-exact, whatever batch or kernel launch it ran in.
 
 "Alone" is the request on a fresh engine handle through the 8/16-bit
 dispatcher; the oracle (a fresh oracle object per request, 8-bit pairs then
 16-bit pairs) checks both.
 """
 import threading
+import time
 
 import numpy as np
 import pytest
 
 import oracle as O
+import stale_batch
 import synth as S
 import tal_bsw as T
 from helpers import assert_same_outputs
@@ -52,10 +55,14 @@ def _oracle(pairs, ref, qer, w=100):
     return want
 
 
-def test_each_request_same_alone_and_in_mix():
-    print("[gpu] serving: 12 requests, 2 workers, 32 MiB budget", flush=True)
+MODES = [pytest.param(False, id="host-batched"), pytest.param(True, id="resident")]
+
+
+@pytest.mark.parametrize("resident", MODES)
+def test_each_request_same_alone_and_in_mix(resident):
+    print(f"[gpu] serving (resident={resident}): 12 requests, 32 MiB budget", flush=True)
     reqs = _requests()
-    srv = T.Server(hbm_budget=32 << 20, workers=2)
+    srv = T.Server(hbm_budget=32 << 20, workers=2, resident=resident)
     try:
         mine = [r[0].copy() for r in reqs]
         tickets = [srv.submit(p, r[1], r[2], 100) for p, r in zip(mine, reqs)]
@@ -73,12 +80,14 @@ def test_each_request_same_alone_and_in_mix():
             assert_same_outputs(got, _oracle(*r), f"request {k} mix vs oracle")
 
 
-def test_concurrent_submitters_and_admission_waits():
+@pytest.mark.parametrize("resident", MODES)
+def test_concurrent_submitters_and_admission_waits(resident):
     """Four threads submit at once into a budget smaller than the offered
-    requests, so admission has to wait for finished batches."""
-    print("[gpu] serving: 4 submitter threads, 6 MiB budget", flush=True)
+    requests, so admission has to wait for finished requests and reuses the
+    device memory they held."""
+    print(f"[gpu] serving (resident={resident}): 4 submitter threads, 6 MiB budget", flush=True)
     reqs = _requests(SPECS * 2)
-    srv = T.Server(hbm_budget=6 << 20, workers=3)
+    srv = T.Server(hbm_budget=6 << 20, workers=3, resident=resident)
     mine = [r[0].copy() for r in reqs]
     results = [None] * len(reqs)
 
@@ -106,16 +115,17 @@ def test_concurrent_submitters_and_admission_waits():
         assert_same_outputs(mine[k], cache[key], f"request {k}")
 
 
-def test_bad_requests_fail_alone():
+@pytest.mark.parametrize("resident", MODES)
+def test_bad_requests_fail_alone(resident):
     """A request over the 16-bit limit and one larger than the budget fail
     with their own status; the requests around them are served."""
-    print("[gpu] serving: failing requests", flush=True)
+    print(f"[gpu] serving (resident={resident}): failing requests", flush=True)
     good = _requests([(1, 500, 0), (2, 800, 0)])
     bad_pairs, bad_ref, bad_qer, _ = S.generate(1, 10)
     bad_pairs = bad_pairs.copy()
     bad_pairs["len2"][3] = 40000  # beyond MAX_SEQ_LEN16 - 1
     big = S.generate(2, 30000)
-    srv = T.Server(hbm_budget=2 << 20, workers=1)
+    srv = T.Server(hbm_budget=2 << 20, workers=1, resident=resident)
     try:
         p0 = good[0][0].copy()
         p1 = good[1][0].copy()
@@ -131,3 +141,91 @@ def test_bad_requests_fail_alone():
     assert i0["status"] == T.TAL_OK and i1["status"] == T.TAL_OK
     assert_same_outputs(p0, _alone(*good[0]), "good request 0")
     assert_same_outputs(p1, _alone(*good[1]), "good request 1")
+
+
+def test_resident_stale_column_reruns():
+    """A request whose 16-bit groups read nonzero stale F[ncol] values: its
+    resolution task finds them, the reruns run in the resident kernel, and the
+    outputs equal a fresh object's (oracle), next to ordinary requests."""
+    print("[gpu] serving (resident): stale-column batch", flush=True)
+    pairs, ref, qer = stale_batch.make(32, 29)
+    w2, r2, q2, _ = S.generate(2, 3000)
+    w2 = w2[T.is_8bit(w2)].copy()
+    w2["idr"] += len(ref)
+    w2["idq"] += len(qer)
+    sp, sr, sq = (np.concatenate([pairs, w2]), np.concatenate([ref, r2]),
+                  np.concatenate([qer, q2]))
+    others = _requests([(1, 900, 0), (2, 2000, 100)])
+    srv = T.Server(hbm_budget=64 << 20, resident=True)
+    try:
+        mine = sp.copy()
+        t = srv.submit(mine, sr, sq, 100)
+        oth = [r[0].copy() for r in others]
+        ot = [srv.submit(p, r[1], r[2], 100) for p, r in zip(oth, others)]
+        info = srv.wait(t)
+        oinfo = [srv.wait(x) for x in ot]
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert info["status"] == T.TAL_OK and all(i["status"] == T.TAL_OK for i in oinfo)
+    assert st["reruns"] > 0, "no stale-column rerun ran"
+    assert_same_outputs(mine, _oracle(sp, sr, sq), "stale-column request vs oracle")
+    for k, (r, got) in enumerate(zip(others, oth)):
+        assert_same_outputs(got, _alone(*r), f"request {k} next to it")
+
+
+def test_resident_request_joins_running_kernel():
+    """Requests submitted while a long request runs are taken by the waves it
+    leaves idle: they finish first, in the same kernel launch."""
+    print("[gpu] serving (resident): short requests overtake a long one", flush=True)
+    pairs, ref, qer, w = S.generate(3, 20000)
+    sel = np.where(w == 500)[0][:2000]
+    long_req = (pairs[sel].copy(), ref, qer)
+    shorts = _requests([(1, 400, k * 400) for k in range(4)])
+    srv = T.Server(hbm_budget=256 << 20, resident=True, max_query_len=2048)
+    try:
+        lp = long_req[0].copy()
+        tl = srv.submit(lp, long_req[1], long_req[2], 500)
+        time.sleep(0.005)
+        sp = [r[0].copy() for r in shorts]
+        ts = [srv.submit(p, r[1], r[2], 100) for p, r in zip(sp, shorts)]
+        done_at = {}
+        while len(done_at) < 1 + len(ts):
+            for t in [tl] + ts:
+                if t not in done_at:
+                    info = srv.wait(t, 0)
+                    if info is not None:
+                        done_at[t] = (time.perf_counter(), info)
+            time.sleep(0.0005)
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert all(v[1]["status"] == T.TAL_OK for v in done_at.values())
+    assert all(done_at[t][0] < done_at[tl][0] for t in ts), "a short request waited for the long one"
+    assert max(done_at[t][1]["batch_requests"] for t in ts) > 1
+    assert st["batches"] == 1, f"{st['batches']} kernel launches"
+    assert_same_outputs(lp, _alone(*long_req, w=500), "long request")
+    for k, (r, got) in enumerate(zip(shorts, sp)):
+        assert_same_outputs(got, _alone(*r), f"short request {k}")
+
+
+def test_resident_kernel_leaves_when_idle_and_returns():
+    """After a stretch without work the kernel leaves; the next request
+    launches it again and is served the same."""
+    print("[gpu] serving (resident): idle exit and relaunch", flush=True)
+    reqs = _requests([(1, 300, 0), (2, 500, 0)])
+    srv = T.Server(hbm_budget=16 << 20, resident=True)
+    try:
+        got = []
+        for r in reqs:
+            p = r[0].copy()
+            info = srv.wait(srv.submit(p, r[1], r[2], 100))
+            assert info["status"] == T.TAL_OK
+            got.append(p)
+            time.sleep(0.2)
+        st = srv.stats()
+    finally:
+        srv.close()
+    assert st["batches"] == 2, f"{st['batches']} kernel launches"
+    for k, (r, p) in enumerate(zip(reqs, got)):
+        assert_same_outputs(p, _alone(*r), f"request {k}")

@@ -80,6 +80,46 @@ struct GroupShape {
     int32_t nrow, ncol;
 };
 
+/* ---- device-resident serving (bsw_server.hip, resident mode) ----
+ * Everything below lives in pinned host memory, written by the host and read
+ * by the persistent serving kernel, except the outputs, the stale-column
+ * bookkeeping and the done flags, which the kernel writes with plain stores
+ * (no atomics on host memory). */
+
+/* One task of request slot `req`: bits 16, two 16-bit groups (one per
+ * half-wave, g1 = -1 for none); bits 8, one 8-bit group g0; bits 0, the
+ * stale-column resolution of the request's 16-bit groups. */
+struct ServeTask {
+    int32_t req, bits, g0, g1;
+};
+
+/* A request slot.  Group g of the 16-bit (8-bit) block holds pairs
+ * [32g, 32g+32) ([64g, 64g+64)) of pairs16 (pairs8); idr/idq of every record
+ * are offsets into `seq`.  Pinned host memory: the pair blocks (outputs
+ * written in place), fmask_off, ncol16 and reruns; device memory: seq and
+ * the 16-bit stale-column bookkeeping. */
+struct ServeReq {
+    tal_seqpair *pairs16, *pairs8;
+    const uint8_t *seq;
+    uint32_t *stale;   /* per 16-bit group: lanes whose F[ncol] starts nonzero */
+    uint32_t *dep;     /* per 16-bit group */
+    uint8_t *wrote;    /* per 16-bit group */
+    uint32_t *fmask;   /* final F nonzero masks, ncol16[g] words per group */
+    const int64_t *fmask_off;
+    const int32_t *ncol16; /* columns of each 16-bit group */
+    int32_t *reruns;   /* a resolve task's result: count, then the groups */
+    int32_t n16, n8, w, pad;
+};
+
+/* Control words: the host publishes tasks [0, tail) of the ring and sets
+ * stop when the server closes.  Each finished task t writes (int32)(t + 1)
+ * into done[t % ring capacity]. */
+struct ServeCtl {
+    int64_t tail;
+    int32_t stop;
+    int32_t pad[13];
+};
+
 /* Host-side entry points shared by the C ABI (tal_bsw.hip) and the serving
  * loop (bsw_server.hip); not exported from the library. */
 #define TAL_INTERNAL __attribute__((visibility("hidden")))
@@ -96,6 +136,24 @@ TAL_INTERNAL hipStream_t handle_stream(const tal_bsw *h);
 TAL_INTERNAL int score_bound(const tal_bsw_params &p);
 /* The dispatcher rule: 1 if the pair goes to the 8-bit path. */
 TAL_INTERNAL int goes_8bit(const tal_seqpair &p, int smax);
+/* The scoring parameters of a handle as the kernels take them for one lane
+ * width (n, w and the sizing fields left zero). */
+TAL_INTERNAL KParams params_for(const tal_bsw_params &p, int bits);
+/* State placement of a wave for groups of up to max_ncol columns with
+ * waves_per_cu waves sharing a CU's LDS (tal_bsw.hip). */
+struct Sizing {
+    int state_cols, qwords, lds_cols, slab_cols;
+    size_t lds_bytes;
+};
+TAL_INTERNAL Sizing state_sizing(size_t lds_per_cu, size_t lds_per_block, int waves_per_cu,
+                                 int max_ncol);
+/* The persistent serving kernel (bsw_kernels.hip). */
+TAL_INTERNAL hipError_t launch_serve(int grid, size_t lds_bytes, const KParams &K16,
+                                     const KParams &K8, const ServeTask *ring, int32_t ring_mask,
+                                     const ServeReq *reqs, const ServeCtl *ctl, int32_t *done,
+                                     int64_t *head, int64_t *tail_seen, uint32_t *slabs,
+                                     int64_t idle_ticks, int32_t pollers, hipStream_t s);
+TAL_INTERNAL hipError_t set_serve_lds_limit(size_t bytes);
 
 } // namespace talbsw
 

@@ -1107,6 +1107,191 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bs
     }
 }
 
+/* ------------------------------------------------ device-resident serving */
+
+/* A value the host wrote into pinned memory: a system-scope load, which
+ * always reads memory (never a cached copy). */
+template <class T> __device__ __forceinline__ T hload(const T *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+template <class T> __device__ __forceinline__ T *hload_ptr(T *const *p)
+{
+    return (T *)(uintptr_t)hload((const uint64_t *)p);
+}
+
+/*
+ * The persistent serving kernel.  Each 64-thread block is one wave that
+ * claims tasks [head, tail) of the host's ring in order, runs them and marks
+ * them done.  It exits when the host sets stop or after idle_ticks of the
+ * 100 MHz real-time clock without a task; the host relaunches it when work
+ * arrives while it is gone.
+ *
+ * Claims are a compare-and-swap on `head` (device memory) below tail_seen,
+ * the device's copy of the host's tail, so a wave never takes a task that is
+ * not published and no task is taken twice or lost.  Blocks below
+ * `pollers` read the host's tail (over the bus) on every idle turn, the
+ * others on every 16th; in between they wait on tail_seen.
+ *
+ * Coherence.  The host writes a task's descriptor, its request's pinned
+ * records and (by DMA) its sequence bytes before it raises tail.  After a
+ * claim the wave acquires at system scope, which invalidates its L1 and L2,
+ * so it reads all of these, and the device bookkeeping written by earlier
+ * tasks of the request, as written.  Before marking the task done it
+ * releases at system scope, which waits for its stores to host memory and
+ * writes its device stores back from L2.
+ */
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bsw_serve(
+    KParams K16, KParams K8, const ServeTask *ring, int32_t ring_mask, const ServeReq *reqs,
+    const ServeCtl *ctl, int32_t *done, int64_t *head, int64_t *tail_seen, uint32_t *slabs,
+    int64_t idle_ticks, int32_t pollers)
+{
+    extern __shared__ uint32_t lds_state[];
+    const int lane = threadIdx.x;
+    uint32_t *slab = slabs + (size_t)blockIdx.x * K16.slab_cols * 64;
+    StateRef S;
+    S.lds = (lds_u32 *)lds_state + lane * 8;
+    S.slab = __builtin_amdgcn_make_buffer_rsrc(slab, 0, K16.slab_cols * 64 * (int)sizeof(uint32_t),
+                                               RSRC_WORD3);
+    S.gbase = ((uint32_t)K16.query_words * 64u + (uint32_t)lane * 8u) * 4u;
+    S.nb = K16.lds_cols >> 3;
+    S.nblk = (K16.state_cols + 7) >> 3;
+    const uint32_t qb = (uint32_t)lane * 4u;
+    const bool poller = (int)blockIdx.x < pollers;
+    uint32_t polls = 0;
+    uint64_t idle_from = __builtin_amdgcn_s_memrealtime();
+    for (;;) {
+        int64_t t = -1; /* the claimed task; -1: leave */
+        if (lane == 0) {
+            for (;;) {
+                const int64_t h = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const int64_t ts =
+                    __hip_atomic_load(tail_seen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                if (h < ts) {
+                    int64_t e = h;
+                    if (__hip_atomic_compare_exchange_strong(head, &e, h + 1, __ATOMIC_RELAXED,
+                                                             __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT)) {
+                        t = h;
+                        break;
+                    }
+                    continue;
+                }
+                if (poller || (++polls & 15) == 0) {
+                    const int64_t tl =
+                        __hip_atomic_load(&ctl->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+                    if (tl > ts) {
+                        __hip_atomic_fetch_max(tail_seen, tl, __ATOMIC_RELEASE,
+                                               __HIP_MEMORY_SCOPE_AGENT);
+                        continue;
+                    }
+                }
+                if (hload(&ctl->stop) != 0 ||
+                    (int64_t)(__builtin_amdgcn_s_memrealtime() - idle_from) > idle_ticks)
+                    break;
+                if (poller)
+                    __builtin_amdgcn_s_sleep(32);
+                else
+                    __builtin_amdgcn_s_sleep(127);
+            }
+        }
+        {
+            const int lo = __shfl((int)(uint32_t)t, 0, 64), hi = __shfl((int)(t >> 32), 0, 64);
+            t = (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+        }
+        if (t < 0) break;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+        const int slot = (int)(t & ring_mask);
+        const ServeTask *tk = ring + slot;
+        const int bits = hload(&tk->bits), rq = hload(&tk->req);
+        const int g0 = hload(&tk->g0), g1 = hload(&tk->g1);
+        const ServeReq *R = reqs + rq;
+        const int32_t n16 = hload(&R->n16), n8 = hload(&R->n8), w = hload(&R->w);
+        if (bits == 0) {
+            /* Stale-column resolution of the request's 16-bit groups from the
+             * zero image of a fresh object (bsw_resolve with one segment):
+             * lanes take groups, the groups to rerun go to the rerun list. */
+            const uint32_t *dep = hload_ptr(&R->dep), *fmask = hload_ptr(&R->fmask);
+            const uint8_t *wrote = hload_ptr(&R->wrote);
+            uint32_t *stale = hload_ptr(&R->stale);
+            const int64_t *foff = hload_ptr(&R->fmask_off);
+            const int32_t *ncol = hload_ptr(&R->ncol16);
+            int32_t *reruns = hload_ptr(&R->reruns);
+            const int G = n16 / 32;
+            int count = 0;
+            for (int base = 0; base < G; base += 64) {
+                const int g = base + lane;
+                bool rr = false;
+                uint32_t truth = 0;
+                if (g < G) {
+                    const uint32_t d = dep[g];
+                    if (d != 0u) {
+                        const int c = hload(ncol + g);
+                        for (int h = g - 1; h >= 0; h--) {
+                            const int nh = hload(ncol + h);
+                            if (nh > c) {
+                                truth = fmask[hload(foff + h) + c];
+                                break;
+                            }
+                            if (nh == c && wrote[h]) break;
+                        }
+                        rr = (truth & d) != (stale[g] & d);
+                    }
+                }
+                if (rr) stale[g] = truth;
+                const uint64_t m = ballot(rr);
+                if (rr) reruns[1 + count + __popcll(m & ((1ull << lane) - 1))] = g;
+                count += __popcll(m);
+            }
+            if (lane == 0) reruns[0] = count;
+        } else if (bits == 16) {
+            KParams K = K16;
+            K.n = n16;
+            K.ngroups = n16 / 32;
+            K.w = w;
+            KBuffers B{};
+            B.pairs = hload_ptr(&R->pairs16);
+            B.ref = B.qer = hload_ptr(&R->seq);
+            B.stale_in = hload_ptr(&R->stale);
+            B.dep_out = hload_ptr(&R->dep);
+            B.wrote_out = hload_ptr(&R->wrote);
+            B.fmask = hload_ptr(&R->fmask);
+            B.fmask_off = hload_ptr(&R->fmask_off);
+            run_group<16>(K, B, lane < 32 ? g0 : g1, S, qb);
+        } else {
+            KParams K = K8;
+            K.n = n8;
+            K.ngroups = n8 / 64;
+            K.w = w;
+            KBuffers B{};
+            B.pairs = hload_ptr(&R->pairs8);
+            B.ref = B.qer = hload_ptr(&R->seq);
+            run_group<8>(K, B, g0, S, qb);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        if (lane == 0)
+            __hip_atomic_store(done + slot, (int32_t)(t + 1), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_SYSTEM);
+        idle_from = __builtin_amdgcn_s_memrealtime();
+    }
+}
+
+hipError_t launch_serve(int grid, size_t lds_bytes, const KParams &K16, const KParams &K8,
+                        const ServeTask *ring, int32_t ring_mask, const ServeReq *reqs,
+                        const ServeCtl *ctl, int32_t *done, int64_t *head, int64_t *tail_seen,
+                        uint32_t *slabs, int64_t idle_ticks, int32_t pollers, hipStream_t s)
+{
+    hipLaunchKernelGGL(bsw_serve, dim3(grid), dim3(64), lds_bytes, s, K16, K8, ring, ring_mask,
+                       reqs, ctl, done, head, tail_seen, slabs, idle_ticks, pollers);
+    return hipGetLastError();
+}
+
+hipError_t set_serve_lds_limit(size_t bytes)
+{
+    return hipFuncSetAttribute((const void *)bsw_serve, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes);
+}
+
 /* Group shapes: rows/columns of each group and the 8-bit padding length. */
 template <int GL>
 __global__ void bsw_shapes(const tal_seqpair *pairs, int32_t n, int32_t ngroups,

@@ -18,6 +18,20 @@
  * 8-bit len2 -- the padding of one getScores call), so a batch's groups never
  * span requests; the 16-bit stale-column search runs per request from the
  * zero image (run_batch with seg_first).
+ *
+ * Resident mode (config.resident = 1).  One persistent kernel (bsw_serve)
+ * stays on the GPU and pulls tasks from a ring in pinned host memory; the
+ * host never launches a batch.  The admission thread copies a request's
+ * sequences into the device arena (by DMA, under the same budget), writes
+ * its padded pair blocks into a pinned arena and publishes one task per
+ * 16-bit group pair and per 8-bit group, longest first, by raising the
+ * ring's tail; waves that free up take them at once, so a request admitted
+ * while others run joins the running kernel as slots free up.  A monitor
+ * thread watches the tasks' done flags; when a request's groups are done it
+ * publishes the request's stale-column resolution as one more task, then any
+ * reruns that resolution asks for, and finally copies the outputs into the
+ * caller's array.  The kernel leaves after a stretch without work and the
+ * monitor launches it again when work is pending.
  */
 #include <algorithm>
 #include <chrono>
@@ -105,6 +119,15 @@ struct Request {
     size_t off = 0, bytes = 0;        /* arena slot */
     size_t ref_off = 0, qer_off = 0, p8_off = 0, p16_off = 0; /* absolute arena offsets */
     bool admitted = false;
+    /* resident mode */
+    int slot = -1;                  /* ServeReq slot */
+    size_t pin_off = 0, pin_bytes = 0; /* pinned-arena slot */
+    int32_t G16 = 0, G8 = 0;        /* groups of each width */
+    int64_t t_first = 0, t_end = 0; /* the tasks being waited for: [t_first, t_end) */
+    int64_t cursor = 0;             /* first of them not seen done */
+    int phase = 0;                  /* 0 groups, 1 resolution, 2 reruns */
+    int in_flight = 0;              /* requests on the device when admitted (itself included) */
+    bool keep = false;              /* its tasks may still be in the ring: never reuse its memory */
     /* result */
     int status = TAL_OK;
     int32_t batch = -1, batch_requests = 0;
@@ -114,8 +137,47 @@ struct Request {
 
 } // namespace
 
+namespace {
+
+/* Resident-mode state: the task ring and request slots in pinned host
+ * memory, the claim counters and state slabs of the persistent kernel in
+ * device memory. */
+struct Resident {
+    int64_t cap = 0;           /* ring capacity (a power of two) */
+    ServeTask *ring = nullptr; /* pinned */
+    int32_t *done = nullptr;   /* pinned, one flag per ring entry */
+    ServeReq *reqs = nullptr;  /* pinned */
+    ServeCtl *ctl = nullptr;   /* pinned */
+    uint8_t *pin = nullptr;    /* pinned arena: pair blocks, group records, staging */
+    size_t pin_size = 0;
+    Arena pin_alloc;
+    int64_t *head = nullptr, *tail_seen = nullptr; /* device */
+    uint32_t *slabs = nullptr;                     /* device */
+    int grid = 0;
+    size_t lds_bytes = 0;
+    KParams K16{}, K8{};
+    int max_qlen = 0;
+    int64_t tail = 0; /* tasks published */
+    std::vector<int> free_slots;
+    std::vector<Request *> live; /* requests with tasks on the device */
+    hipStream_t kstream = nullptr;
+    hipEvent_t kdone = nullptr;
+    bool launched = false;
+    int64_t idle_ticks = 0;
+    std::thread monitor;
+};
+
+constexpr int RES_SLOTS = 4096;
+constexpr int64_t RES_RING = (int64_t)1 << 20;
+constexpr size_t RES_PINNED = (size_t)512 << 20;
+constexpr int RES_WAVES_PER_CU = 8; /* two per SIMD: room for other kernels' waves */
+
+} // namespace
+
 struct tal_bsw_server {
     tal_bsw_params prm{};
+    bool resident = false;
+    Resident *res = nullptr;
     int device = 0;
     int smax = 0;
     size_t budget = 0;
@@ -160,7 +222,7 @@ int prepare(tal_bsw_server *s, Request *r)
                        align_up(std::max<size_t>(r->qer_len, 1));
     r->bytes = seq + align_up(sizeof(tal_seqpair) * (size_t)r->n8p) +
                align_up(sizeof(tal_seqpair) * (size_t)r->n16p);
-    return r->bytes > s->budget ? TAL_ERANGE : TAL_OK;
+    return TAL_OK;
 }
 
 /* The padded blocks with offsets moved into the arena. */
@@ -190,7 +252,7 @@ void finish(tal_bsw_server *s, Request *r, int status)
     r->status = status;
     r->t_done = now_ms();
     r->done = true;
-    if (r->admitted) s->alloc.give(r->off, r->bytes);
+    if (r->admitted && !r->keep) s->alloc.give(r->off, r->bytes);
     if (status == TAL_OK) {
         s->st.requests++;
         s->st.pairs += r->n;
@@ -211,6 +273,7 @@ void admit_loop(tal_bsw_server *s)
             s->admitting = true;
         }
         int rc = prepare(s, r);
+        if (rc == TAL_OK && r->bytes > s->budget) rc = TAL_ERANGE;
         {
             std::unique_lock<std::mutex> lk(s->mu);
             if (rc == TAL_OK) /* arrival order: wait for room for the head */
@@ -403,6 +466,417 @@ void work_loop(tal_bsw_server *s, int k)
     }
 }
 
+/* ---------------------------------------------------------- resident mode */
+
+bool res_running(const Resident *R)
+{
+    return R->launched && hipEventQuery(R->kdone) == hipErrorNotReady;
+}
+
+/* Launches the persistent kernel unless it is running (s->mu held). */
+int res_ensure_kernel(tal_bsw_server *s)
+{
+    Resident *R = s->res;
+    if (res_running(R)) return TAL_OK;
+    if (launch_serve(R->grid, R->lds_bytes, R->K16, R->K8, R->ring, (int32_t)(R->cap - 1), R->reqs,
+                     R->ctl, R->done, R->head, R->tail_seen, R->slabs, R->idle_ticks, 32,
+                     R->kstream) != hipSuccess ||
+        hipEventRecord(R->kdone, R->kstream) != hipSuccess)
+        return TAL_EDEVICE;
+    R->launched = true;
+    s->st.batches++;
+    return TAL_OK;
+}
+
+/* The first task index still on the device (every earlier ring entry may be
+ * written again). */
+int64_t res_oldest(const Resident *R)
+{
+    int64_t m = R->tail;
+    for (const Request *r : R->live)
+        if (r->cursor < r->t_end) m = std::min(m, r->cursor);
+    return m;
+}
+
+/* Publishes tasks for request r (s->mu held): they become the range the
+ * monitor waits for.  Returns false, publishing nothing, if the ring has no
+ * room for them yet. */
+bool res_publish(tal_bsw_server *s, Request *r, const std::vector<ServeTask> &tasks)
+{
+    Resident *R = s->res;
+    const int64_t n = (int64_t)tasks.size();
+    if (R->tail + n - res_oldest(R) > R->cap) return false;
+    for (int64_t k = 0; k < n; k++) R->ring[(R->tail + k) & (R->cap - 1)] = tasks[(size_t)k];
+    r->t_first = r->cursor = R->tail;
+    r->t_end = R->tail + n;
+    R->tail += n;
+    __atomic_store_n(&R->ctl->tail, R->tail, __ATOMIC_RELEASE);
+    return true;
+}
+
+/* Group shapes and longest-first tasks of a prepared request whose blocks
+ * are in place. */
+std::vector<ServeTask> res_group_tasks(const Request *r, const tal_seqpair *p16,
+                                       const tal_seqpair *p8, int32_t *ncol16)
+{
+    struct G {
+        int64_t cost;
+        int bits, g;
+    };
+    std::vector<G> gs;
+    auto cost = [&](const tal_seqpair *p, int lanes, int w, int *ncol_out) {
+        int nr = 0, nc = 0;
+        for (int l = 0; l < lanes; l++) {
+            nr = std::max(nr, p[l].len1);
+            nc = std::max(nc, p[l].len2);
+        }
+        if (ncol_out) *ncol_out = nc;
+        return (int64_t)nr * std::min(nc, 2 * w + 2);
+    };
+    for (int g = 0; g < r->G16; g++) gs.push_back({cost(p16 + 32 * g, 32, r->w, &ncol16[g]), 16, g});
+    for (int g = 0; g < r->G8; g++) gs.push_back({cost(p8 + 64 * g, 64, r->w, nullptr), 8, g});
+    std::stable_sort(gs.begin(), gs.end(), [](const G &a, const G &b) { return a.cost > b.cost; });
+    std::vector<ServeTask> t;
+    int pend16 = -1; /* a 16-bit group waiting for its partner */
+    for (const G &x : gs) {
+        if (x.bits == 8) {
+            t.push_back({r->slot, 8, x.g, -1});
+        } else if (pend16 < 0) {
+            pend16 = (int)t.size();
+            t.push_back({r->slot, 16, x.g, -1});
+        } else {
+            t[(size_t)pend16].g1 = x.g;
+            pend16 = -1;
+        }
+    }
+    return t;
+}
+
+/* Outputs into the caller's array, slots and memory released (s->mu held). */
+void res_finish(tal_bsw_server *s, Request *r, int status)
+{
+    Resident *R = s->res;
+    if (status == TAL_OK) {
+        const tal_seqpair *p16 = R->reqs[r->slot].pairs16, *p8 = R->reqs[r->slot].pairs8;
+        auto put = [&](const std::vector<int32_t> &idx, const tal_seqpair *blk) {
+            for (size_t t = 0; t < idx.size(); t++) {
+                const tal_seqpair &q = blk[t];
+                tal_seqpair &d = r->pairs[idx[t]];
+                d.score = q.score;
+                d.tle = q.tle;
+                d.gtle = q.gtle;
+                d.qle = q.qle;
+                d.gscore = q.gscore;
+                d.max_off = q.max_off;
+            }
+        };
+        put(r->idx16, p16);
+        put(r->idx8, p8);
+    }
+    if (!r->keep) {
+        if (r->slot >= 0) R->free_slots.push_back(r->slot);
+        if (r->pin_bytes) R->pin_alloc.give(r->pin_off, r->pin_bytes);
+    }
+    R->live.erase(std::remove(R->live.begin(), R->live.end(), r), R->live.end());
+    r->batch = status == TAL_OK ? 0 : -1;
+    r->batch_requests = r->in_flight;
+    finish(s, r, status);
+    s->cv.notify_all();
+}
+
+/* Pinned-arena layout of a request: pair blocks, per-group records, then the
+ * staging copy of its sequences. */
+struct PinLayout {
+    size_t p16, p8, ncol, foff, stale, reruns, seq, bytes;
+};
+PinLayout pin_layout(const Request *r)
+{
+    PinLayout L{};
+    size_t o = 0;
+    auto take = [&](size_t b) {
+        const size_t at = o;
+        o += align_up(std::max<size_t>(b, 1));
+        return at;
+    };
+    L.p16 = take(sizeof(tal_seqpair) * (size_t)r->n16p);
+    L.p8 = take(sizeof(tal_seqpair) * (size_t)r->n8p);
+    L.ncol = take(sizeof(int32_t) * (size_t)r->G16);
+    L.foff = take(sizeof(int64_t) * (size_t)r->G16);
+    L.stale = take(sizeof(uint32_t) * (size_t)r->G16);
+    L.reruns = take(sizeof(int32_t) * ((size_t)r->G16 + 1));
+    L.seq = take(r->ref_len + r->qer_len);
+    L.bytes = o;
+    return L;
+}
+
+void res_admit_loop(tal_bsw_server *s)
+{
+    (void)hipSetDevice(s->device);
+    Resident *R = s->res;
+    std::vector<tal_seqpair> blk;
+    for (;;) {
+        Request *r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            s->cv.wait(lk, [&] { return s->stop || !s->pending.empty(); });
+            if (s->pending.empty()) return; /* stop with nothing queued */
+            r = s->pending.front();
+            s->admitting = true;
+        }
+        int rc = prepare(s, r);
+        /* the kernel's state slabs hold max_qlen columns */
+        for (int32_t k = 0; k < r->n && rc == TAL_OK; k++)
+            if (r->pairs[k].len2 > R->max_qlen) rc = TAL_ERANGE;
+        r->G16 = r->n16p / 32;
+        r->G8 = r->n8p / 64;
+        PinLayout L{};
+        size_t fcols = 0; /* F-mask words of the 16-bit groups */
+        if (rc == TAL_OK) {
+            L = pin_layout(r);
+            for (int32_t g = 0; g < r->G16; g++) {
+                int nc = 0;
+                for (int l = 0; l < 32 && 32 * g + l < (int32_t)r->idx16.size(); l++)
+                    nc = std::max(nc, r->pairs[r->idx16[32 * g + l]].len2);
+                fcols += (size_t)nc;
+            }
+            /* device footprint: sequences plus the 16-bit bookkeeping */
+            r->bytes = align_up(std::max<size_t>(r->ref_len, 1)) +
+                       align_up(std::max<size_t>(r->qer_len, 1)) +
+                       align_up(sizeof(uint32_t) * (size_t)std::max(r->G16, 1)) +
+                       align_up((size_t)std::max(r->G16, 1)) +
+                       align_up(sizeof(uint32_t) * std::max<size_t>(fcols, 1));
+            if (r->bytes > s->budget || L.bytes > R->pin_size) rc = TAL_ERANGE;
+        }
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            if (rc == TAL_OK) /* arrival order: wait for room for the head */
+                s->cv.wait(lk, [&] {
+                    if (R->free_slots.empty()) return false;
+                    if (R->tail - res_oldest(R) + r->G16 + r->G8 + 2 > R->cap / 2) return false;
+                    size_t po = 0, dof = 0;
+                    if (!R->pin_alloc.take(L.bytes, &po)) return false;
+                    if (!s->alloc.take(r->bytes, &dof)) {
+                        R->pin_alloc.give(po, L.bytes);
+                        return false;
+                    }
+                    r->pin_off = po;
+                    r->pin_bytes = L.bytes;
+                    r->off = dof;
+                    return true;
+                });
+            s->pending.pop_front();
+            if (rc != TAL_OK) {
+                finish(s, r, rc);
+                s->admitting = false;
+                s->cv.notify_all();
+                continue;
+            }
+            r->admitted = true;
+            r->slot = R->free_slots.back();
+            R->free_slots.pop_back();
+        }
+        /* device arena: ref, qer, dep, wrote, fmask */
+        r->ref_off = r->off;
+        r->qer_off = r->ref_off + align_up(std::max<size_t>(r->ref_len, 1));
+        const size_t dep_off = r->qer_off + align_up(std::max<size_t>(r->qer_len, 1));
+        const size_t wrote_off = dep_off + align_up(sizeof(uint32_t) * (size_t)std::max(r->G16, 1));
+        const size_t fmask_off = wrote_off + align_up((size_t)std::max(r->G16, 1));
+        uint8_t *pin = R->pin + r->pin_off;
+        tal_seqpair *p16 = (tal_seqpair *)(pin + L.p16), *p8 = (tal_seqpair *)(pin + L.p8);
+        int32_t *ncol = (int32_t *)(pin + L.ncol);
+        int64_t *foff = (int64_t *)(pin + L.foff);
+        uint32_t *stale = (uint32_t *)(pin + L.stale);
+        int32_t *reruns = (int32_t *)(pin + L.reruns);
+        if (r->n16p) {
+            build_block(r, r->idx16, r->n16p, false, blk);
+            std::memcpy(p16, blk.data(), sizeof(tal_seqpair) * (size_t)r->n16p);
+        }
+        if (r->n8p) {
+            build_block(r, r->idx8, r->n8p, true, blk);
+            std::memcpy(p8, blk.data(), sizeof(tal_seqpair) * (size_t)r->n8p);
+        }
+        std::vector<ServeTask> tasks = res_group_tasks(r, p16, p8, ncol);
+        int64_t fo = 0;
+        for (int32_t g = 0; g < r->G16; g++) {
+            foff[g] = fo;
+            fo += ncol[g];
+        }
+        std::memset(stale, 0, sizeof(uint32_t) * (size_t)r->G16);
+        reruns[0] = 0;
+        /* the sequences: staged in pinned memory, then copied by DMA */
+        std::memcpy(pin + L.seq, r->ref, r->ref_len);
+        std::memcpy(pin + L.seq + r->ref_len, r->qer, r->qer_len);
+        hipError_t e = hipSuccess;
+        if (r->ref_len)
+            e = hipMemcpyAsync(s->arena + r->ref_off, pin + L.seq, r->ref_len,
+                               hipMemcpyHostToDevice, s->copy_stream);
+        if (e == hipSuccess && r->qer_len)
+            e = hipMemcpyAsync(s->arena + r->qer_off, pin + L.seq + r->ref_len, r->qer_len,
+                               hipMemcpyHostToDevice, s->copy_stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(s->copy_stream);
+        ServeReq q{};
+        q.pairs16 = p16;
+        q.pairs8 = p8;
+        q.seq = s->arena;
+        q.stale = stale;
+        q.dep = (uint32_t *)(s->arena + dep_off);
+        q.wrote = s->arena + wrote_off;
+        q.fmask = (uint32_t *)(s->arena + fmask_off);
+        q.fmask_off = foff;
+        q.ncol16 = ncol;
+        q.reruns = reruns;
+        q.n16 = r->n16p;
+        q.n8 = r->n8p;
+        q.w = r->w;
+        std::unique_lock<std::mutex> lk(s->mu);
+        if (e != hipSuccess) {
+            res_finish(s, r, TAL_EDEVICE);
+        } else {
+            R->reqs[r->slot] = q;
+            r->t_admit = now_ms();
+            R->live.push_back(r);
+            r->in_flight = (int)R->live.size();
+            /* room was checked at admission; should the monitor's tasks have
+             * taken it meanwhile, wait for the device to drain some */
+            while (!res_publish(s, r, tasks)) {
+                lk.unlock();
+                std::this_thread::sleep_for(std::chrono::microseconds(50));
+                lk.lock();
+            }
+            const int krc = res_ensure_kernel(s);
+            if (krc != TAL_OK) {
+                r->keep = true;
+                res_finish(s, r, krc);
+            }
+        }
+        s->admitting = false;
+        s->cv.notify_all();
+    }
+}
+
+/* Advances each live request through its phases as its tasks finish. */
+void res_monitor_loop(tal_bsw_server *s)
+{
+    (void)hipSetDevice(s->device);
+    Resident *R = s->res;
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        if (s->stop && s->pending.empty() && !s->admitting && R->live.empty()) return;
+        bool moved = false; /* a request changed phase or finished */
+        std::vector<Request *> live = R->live;
+        for (Request *r : live) {
+            while (r->cursor < r->t_end &&
+                   __atomic_load_n(&R->done[r->cursor & (R->cap - 1)], __ATOMIC_ACQUIRE) ==
+                       (int32_t)(r->cursor + 1))
+                r->cursor++;
+            if (r->cursor < r->t_end) continue;
+            if (r->phase == 1) { /* a resolution finished: reruns, or done */
+                const int32_t *rr = R->reqs[r->slot].reruns;
+                const int32_t nrr = rr[0];
+                if (nrr == 0) {
+                    res_finish(s, r, TAL_OK);
+                    moved = true;
+                    continue;
+                }
+                std::vector<ServeTask> t;
+                for (int32_t k = 0; k < nrr; k += 2)
+                    t.push_back({r->slot, 16, rr[1 + k], k + 1 < nrr ? rr[2 + k] : -1});
+                /* (no room in the ring yet: the next round tries again) */
+                if (res_publish(s, r, t)) {
+                    r->phase = 2;
+                    s->st.reruns += nrr;
+                    moved = true;
+                }
+            } else if (r->G16 > 0) { /* groups or reruns finished: resolve */
+                if (res_publish(s, r, {ServeTask{r->slot, 0, 0, -1}})) {
+                    r->phase = 1;
+                    moved = true;
+                }
+            } else {
+                res_finish(s, r, TAL_OK);
+                moved = true;
+            }
+        }
+        if (!R->live.empty() && !res_running(R)) {
+            const int krc = res_ensure_kernel(s);
+            if (krc != TAL_OK)
+                for (Request *r : std::vector<Request *>(R->live)) {
+                    r->keep = true;
+                    res_finish(s, r, krc);
+                }
+        }
+        if (!moved) {
+            lk.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+            lk.lock();
+        }
+    }
+}
+
+void res_release(tal_bsw_server *s)
+{
+    Resident *R = s->res;
+    if (!R) return;
+    if (R->ctl) __atomic_store_n(&R->ctl->stop, 1, __ATOMIC_RELEASE);
+    if (R->launched) (void)hipEventSynchronize(R->kdone);
+    if (R->kdone) (void)hipEventDestroy(R->kdone);
+    if (R->kstream) (void)hipStreamDestroy(R->kstream);
+    for (void *p : {(void *)R->head, (void *)R->slabs})
+        if (p) (void)hipFree(p);
+    for (void *p : {(void *)R->ring, (void *)R->done, (void *)R->reqs, (void *)R->ctl, (void *)R->pin})
+        if (p) (void)hipHostFree(p);
+    delete R;
+    s->res = nullptr;
+}
+
+int res_create(tal_bsw_server *s, int max_qlen)
+{
+    Resident *R = new (std::nothrow) Resident();
+    if (!R) return TAL_ENOMEM;
+    s->res = R;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, s->device) != hipSuccess) return TAL_EDEVICE;
+    const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
+                                                                : prop.sharedMemPerBlock;
+    const Sizing z = state_sizing(lds_cu, prop.sharedMemPerBlock, RES_WAVES_PER_CU, max_qlen);
+    R->max_qlen = max_qlen;
+    R->grid = prop.multiProcessorCount * RES_WAVES_PER_CU;
+    R->lds_bytes = z.lds_bytes;
+    for (int bits : {16, 8}) {
+        KParams &K = bits == 16 ? R->K16 : R->K8;
+        K = params_for(s->prm, bits);
+        K.state_cols = z.state_cols;
+        K.lds_cols = z.lds_cols;
+        K.slab_cols = z.slab_cols;
+        K.query_words = z.qwords;
+    }
+    R->cap = RES_RING;
+    R->pin_size = RES_PINNED;
+    R->idle_ticks = 5000000; /* 50 ms of the 100 MHz clock */
+    const unsigned fl = hipHostMallocCoherent | hipHostMallocMapped;
+    if (hipHostMalloc((void **)&R->ring, sizeof(ServeTask) * (size_t)R->cap, fl) != hipSuccess ||
+        hipHostMalloc((void **)&R->done, sizeof(int32_t) * (size_t)R->cap, fl) != hipSuccess ||
+        hipHostMalloc((void **)&R->reqs, sizeof(ServeReq) * RES_SLOTS, fl) != hipSuccess ||
+        hipHostMalloc((void **)&R->ctl, sizeof(ServeCtl), fl) != hipSuccess ||
+        hipHostMalloc((void **)&R->pin, R->pin_size, fl) != hipSuccess)
+        return TAL_ENOMEM;
+    std::memset(R->done, 0, sizeof(int32_t) * (size_t)R->cap);
+    std::memset(R->reqs, 0, sizeof(ServeReq) * RES_SLOTS);
+    std::memset(R->ctl, 0, sizeof(ServeCtl));
+    R->pin_alloc.reset(R->pin_size);
+    for (int k = RES_SLOTS - 1; k >= 0; k--) R->free_slots.push_back(k);
+    const size_t slab_bytes = (size_t)z.slab_cols * 64 * sizeof(uint32_t);
+    if (hipMalloc((void **)&R->head, 2 * sizeof(int64_t)) != hipSuccess) return TAL_ENOMEM;
+    R->tail_seen = R->head + 1;
+    if (hipMemset(R->head, 0, 2 * sizeof(int64_t)) != hipSuccess) return TAL_EDEVICE;
+    if (hipMalloc((void **)&R->slabs, slab_bytes * (size_t)R->grid) != hipSuccess) return TAL_ENOMEM;
+    if (set_serve_lds_limit(prop.sharedMemPerBlock) != hipSuccess ||
+        hipStreamCreateWithFlags(&R->kstream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&R->kdone, hipEventDisableTiming) != hipSuccess)
+        return TAL_EDEVICE;
+    return TAL_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -417,7 +891,10 @@ int tal_bsw_server_create(const tal_bsw_params *params, int device,
     if (cfg.hbm_budget == 0) cfg.hbm_budget = (size_t)1 << 30;
     if (cfg.workers <= 0) cfg.workers = 2;
     if (cfg.max_batch_pairs <= 0) cfg.max_batch_pairs = 1 << 22;
-    if (cfg.hbm_budget > MAX_BUDGET || cfg.workers > 8) return TAL_EINVAL;
+    if (cfg.max_query_len <= 0) cfg.max_query_len = 1024;
+    if (cfg.hbm_budget > MAX_BUDGET || cfg.workers > 8 || cfg.max_query_len > MAX_LEN16 ||
+        (cfg.resident != 0 && cfg.resident != 1))
+        return TAL_EINVAL;
     tal_bsw_server *s = new (std::nothrow) tal_bsw_server();
     if (!s) return TAL_ENOMEM;
     s->prm = *params;
@@ -425,13 +902,15 @@ int tal_bsw_server_create(const tal_bsw_params *params, int device,
     s->smax = score_bound(*params);
     s->budget = cfg.hbm_budget;
     s->max_batch = cfg.max_batch_pairs;
+    s->resident = cfg.resident == 1;
     int rc = TAL_OK;
-    for (int k = 0; k < cfg.workers && rc == TAL_OK; k++) {
+    for (int k = 0; k < (s->resident ? 0 : cfg.workers) && rc == TAL_OK; k++) {
         tal_bsw *e = nullptr;
         rc = tal_bsw_create(params, device, &e);
         if (rc == TAL_OK) s->engines.push_back(e);
     }
     if (rc == TAL_OK && hipSetDevice(device) != hipSuccess) rc = TAL_EDEVICE;
+    if (rc == TAL_OK && s->resident) rc = res_create(s, cfg.max_query_len);
     if (rc == TAL_OK && hipMalloc((void **)&s->arena, s->budget) != hipSuccess) {
         s->arena = nullptr;
         rc = TAL_ENOMEM;
@@ -440,13 +919,20 @@ int tal_bsw_server_create(const tal_bsw_params *params, int device,
         rc = TAL_EDEVICE;
     if (rc != TAL_OK) {
         for (tal_bsw *e : s->engines) tal_bsw_destroy(e);
+        res_release(s);
+        if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
         if (s->arena) (void)hipFree(s->arena);
         delete s;
         return rc;
     }
     s->alloc.reset(s->budget);
-    s->admitter = std::thread(admit_loop, s);
-    for (int k = 0; k < cfg.workers; k++) s->workers.emplace_back(work_loop, s, k);
+    if (s->resident) {
+        s->admitter = std::thread(res_admit_loop, s);
+        s->res->monitor = std::thread(res_monitor_loop, s);
+    } else {
+        s->admitter = std::thread(admit_loop, s);
+        for (int k = 0; k < cfg.workers; k++) s->workers.emplace_back(work_loop, s, k);
+    }
     *out = s;
     return TAL_OK;
 }
@@ -531,6 +1017,8 @@ void tal_bsw_server_destroy(tal_bsw_server *s)
     }
     for (std::thread &t : s->workers)
         if (t.joinable()) t.join();
+    if (s->res && s->res->monitor.joinable()) s->res->monitor.join();
+    res_release(s);
     for (auto &kv : s->all) delete kv.second;
     for (tal_bsw *e : s->engines) tal_bsw_destroy(e);
     (void)hipSetDevice(s->device);

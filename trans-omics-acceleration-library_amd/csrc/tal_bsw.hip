@@ -234,28 +234,9 @@ int smax_of(const tal_bsw_params &p)
 
 KParams base_params(const tal_bsw *h, int bits, int32_t n, int32_t w)
 {
-    KParams K{};
-    const tal_bsw_params &p = h->prm;
-    const int sh = bits == 16 ? 0 : 8;
-    auto sc = [&](int v) -> int16_t {
-        return (int16_t)(uint16_t)((bits == 16 ? (uint32_t)(uint16_t)v : (uint32_t)(uint8_t)v) << sh);
-    };
+    KParams K = params_for(h->prm, bits);
     K.n = n;
     K.w = w;
-    K.o_del = p.o_del;
-    K.e_del = p.e_del;
-    K.o_ins = p.o_ins;
-    K.e_ins = p.e_ins;
-    K.end_bonus = p.end_bonus;
-    K.smax = smax_of(p);
-    K.match = sc((int8_t)p.w_match);
-    K.mism = sc((int8_t)(-(int)(int8_t)p.w_mismatch));
-    K.ambig = sc(-1);
-    K.e_del_s = sc(p.e_del);
-    K.oe_del_s = sc(p.o_del + p.e_del);
-    K.e_ins_s = sc(p.e_ins);
-    K.oe_ins_s = sc(p.o_ins + p.e_ins);
-    K.zdrop_s = sc(p.zdrop);
     return K;
 }
 
@@ -375,34 +356,10 @@ int grid_for(const tal_bsw *h, bool lds, size_t lds_bytes, int ntasks)
     return (int)std::max<long long>(1, std::min<long long>(g, ntasks));
 }
 
-/* Placement of a wave's state: columns 0..max_ncol plus 17 columns of
- * slack for the 8-column block loads that run past a group's last column,
- * and the fast path's query-code words (8 columns per word, one word of
- * prefetch slack).  A window of lds_cols state columns lives in LDS, sized so
- * that `waves_per_cu` waves fit on a CU; the query words and the home of
- * every state column go to a per-block slab in global memory. */
-struct Sizing {
-    int state_cols, qwords, lds_cols, slab_cols;
-    size_t lds_bytes;
-};
-
+/* The state placement for this handle's waves per CU (state_sizing). */
 Sizing sizing(const tal_bsw *h, int max_ncol)
 {
-    Sizing z{};
-    z.state_cols = max_ncol + 18;
-    z.qwords = (max_ncol >> 3) + 2;
-    const int wpc = std::max(1, h->waves_per_cu);
-    const long budget_cols = (long)(h->lds_per_cu / wpc) / 256;
-    const int fit_all = (z.state_cols + 7) & ~7;
-    int lds_cols = (int)std::max<long>(8, budget_cols) & ~7;
-    if (fit_all <= lds_cols) lds_cols = fit_all;
-    const size_t max_cols = h->lds_per_block / 256;
-    if ((size_t)lds_cols > max_cols) lds_cols = (int)(max_cols & ~(size_t)7);
-    z.lds_cols = lds_cols;
-    /* every state column in whole 8-column blocks */
-    z.slab_cols = z.qwords + ((z.state_cols + 7) & ~7);
-    z.lds_bytes = (size_t)lds_cols * 64 * sizeof(uint32_t);
-    return z;
+    return state_sizing(h->lds_per_cu, h->lds_per_block, h->waves_per_cu, max_ncol);
 }
 
 /* Task order: longest groups first, and the two groups of a 16-bit wave of
@@ -969,6 +926,55 @@ int run_batch(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, cons
 hipStream_t handle_stream(const tal_bsw *h) { return h->stream; }
 
 int score_bound(const tal_bsw_params &p) { return smax_of(p); }
+
+KParams params_for(const tal_bsw_params &p, int bits)
+{
+    KParams K{};
+    const int sh = bits == 16 ? 0 : 8;
+    auto sc = [&](int v) -> int16_t {
+        return (int16_t)(uint16_t)((bits == 16 ? (uint32_t)(uint16_t)v : (uint32_t)(uint8_t)v) << sh);
+    };
+    K.o_del = p.o_del;
+    K.e_del = p.e_del;
+    K.o_ins = p.o_ins;
+    K.e_ins = p.e_ins;
+    K.end_bonus = p.end_bonus;
+    K.smax = smax_of(p);
+    K.match = sc((int8_t)p.w_match);
+    K.mism = sc((int8_t)(-(int)(int8_t)p.w_mismatch));
+    K.ambig = sc(-1);
+    K.e_del_s = sc(p.e_del);
+    K.oe_del_s = sc(p.o_del + p.e_del);
+    K.e_ins_s = sc(p.e_ins);
+    K.oe_ins_s = sc(p.o_ins + p.e_ins);
+    K.zdrop_s = sc(p.zdrop);
+    return K;
+}
+
+/* Placement of a wave's state: columns 0..max_ncol plus 17 columns of
+ * slack for the 8-column block loads that run past a group's last column,
+ * and the fast path's query-code words (8 columns per word, one word of
+ * prefetch slack).  A window of lds_cols state columns lives in LDS, sized so
+ * that `waves_per_cu` waves fit on a CU; the query words and the home of
+ * every state column go to a per-block slab in global memory. */
+Sizing state_sizing(size_t lds_per_cu, size_t lds_per_block, int waves_per_cu, int max_ncol)
+{
+    Sizing z{};
+    z.state_cols = max_ncol + 18;
+    z.qwords = (max_ncol >> 3) + 2;
+    const int wpc = std::max(1, waves_per_cu);
+    const long budget_cols = (long)(lds_per_cu / wpc) / 256;
+    const int fit_all = (z.state_cols + 7) & ~7;
+    int lds_cols = (int)std::max<long>(8, budget_cols) & ~7;
+    if (fit_all <= lds_cols) lds_cols = fit_all;
+    const size_t max_cols = lds_per_block / 256;
+    if ((size_t)lds_cols > max_cols) lds_cols = (int)(max_cols & ~(size_t)7);
+    z.lds_cols = lds_cols;
+    /* every state column in whole 8-column blocks */
+    z.slab_cols = z.qwords + ((z.state_cols + 7) & ~7);
+    z.lds_bytes = (size_t)lds_cols * 64 * sizeof(uint32_t);
+    return z;
+}
 
 int goes_8bit(const tal_seqpair &p, int smax)
 {

@@ -60,7 +60,8 @@ class Tuning(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_size_t), ("workers", ctypes.c_int32),
-                ("max_batch_pairs", ctypes.c_int32)]
+                ("max_batch_pairs", ctypes.c_int32), ("resident", ctypes.c_int32),
+                ("max_query_len", ctypes.c_int32)]
 
 
 class RequestInfo(ctypes.Structure):
@@ -72,7 +73,7 @@ class RequestInfo(ctypes.Structure):
 class ServerStats(ctypes.Structure):
     _fields_ = [("requests", ctypes.c_int64), ("pairs", ctypes.c_int64),
                 ("batches", ctypes.c_int64), ("peak_bytes", ctypes.c_int64),
-                ("busy_ms", ctypes.c_double)]
+                ("busy_ms", ctypes.c_double), ("reruns", ctypes.c_int64)]
 
 
 class Pack(ctypes.Structure):
@@ -304,13 +305,17 @@ class Plan:
 class Server:
     """Streamed serving with continuous batching (tal_bsw_server_* of
     include/tal_bsw.h).  Each request is scored as one dispatcher call on a
-    fresh engine object, whatever it is batched with."""
+    fresh engine object, whatever it is batched with.  resident=True runs the
+    device-resident loop (one persistent kernel pulling tasks) instead of
+    host-launched batches."""
 
     def __init__(self, hbm_budget=0, workers=0, max_batch_pairs=0, device=0, o_del=6, e_del=1,
-                 o_ins=6, e_ins=1, zdrop=100, end_bonus=5, w_match=1, w_mismatch=4):
+                 o_ins=6, e_ins=1, zdrop=100, end_bonus=5, w_match=1, w_mismatch=4,
+                 resident=False, max_query_len=0):
         L = load()
         self.params = Params(o_del, e_del, o_ins, e_ins, zdrop, end_bonus, w_match, w_mismatch)
-        cfg = ServerConfig(int(hbm_budget), int(workers), int(max_batch_pairs))
+        cfg = ServerConfig(int(hbm_budget), int(workers), int(max_batch_pairs),
+                           1 if resident else 0, int(max_query_len))
         h = ctypes.c_void_p()
         rc = L.tal_bsw_server_create(ctypes.byref(self.params), int(device), ctypes.byref(cfg),
                                      ctypes.byref(h))
