@@ -98,6 +98,9 @@ def main():
                     help="streamed serving instead: Poisson arrivals of 1k-64k-pair requests, "
                          "p50/p99 latency at 50/80/95%% of the measured peak")
     ap.add_argument("--serve-requests", type=int, default=400, help="requests per load level")
+    ap.add_argument("--serve-mode", default="resident", choices=["resident", "host"],
+                    help="serving loop of --serve: the device-resident kernel or host-launched "
+                         "batches (the default run measures both)")
     ap.add_argument("--sw", default="", choices=["", *sorted(synth.SW_WORKLOADS)],
                     help="measure the full inter-task SW engine (PairWiseSW drop-in) on one of "
                          "its synthetic workloads instead")
@@ -813,7 +816,8 @@ def _batch(args):
         line["workloads"] = {str(k): _workload_summary(k, args, torch, dev, stream)
                              for k in (1, 3, 5)}
     if not args.no_serve and cfg == 2 and not args.pairs:
-        line["serving"] = _serve_levels(args)
+        line["serving"] = _serve_levels(args, resident=True)
+        line["serving_host_batched"] = _serve_levels(args, resident=False)
     print(json.dumps(line), flush=True)
 
 
@@ -829,12 +833,14 @@ def _slice_request(pairs, ref, qer, a, b):
     return p, ref[r0:r1], qer[q0:q1]
 
 
-def _serve_levels(args):
+def _serve_levels(args, resident=True):
     """Streamed serving: requests of log-uniform 1k-64k pairs drawn from the
     workload-1 and workload-2 generators arrive as a Poisson process at 50%,
     80% and 95% of the peak pairs/s measured by submitting a full set at
-    once; latency is submit -> outputs in the caller's array.  Returns the
-    per-level p50/p99 and the server's counters."""
+    once; latency is submit -> outputs in the caller's array.  resident: the
+    device-resident loop (one persistent kernel), else host-launched batches
+    on two workers.  Returns the per-level p50/p99 and the server's
+    counters."""
     import threading
     rng = np.random.default_rng(0xB5A0000000000004)
     pools = {1: synth.generate(1, 400_000), 2: synth.generate(2, 1_000_000)}
@@ -846,8 +852,10 @@ def _serve_levels(args):
         pairs, ref, qer, _ = pools[int(kd)]
         a = int(rng.integers(0, len(pairs) - sz))
         reqs.append(_slice_request(pairs, ref, qer, a, a + int(sz)))
-    log(f"[serve] {len(reqs)} requests, mean {sizes.mean():.0f} pairs")
-    srv = tal_bsw.Server(hbm_budget=1 << 30, workers=2, **PARAMS)
+    mode = "resident" if resident else "host-batched"
+    log(f"[serve] {mode}: {len(reqs)} requests, mean {sizes.mean():.0f} pairs")
+    srv = tal_bsw.Server(hbm_budget=1 << 30, workers=2, resident=resident, max_query_len=256,
+                         **PARAMS)
 
     def run_set(sub, rate):
         """Submits sub (open loop at `rate` requests/s, or all at once if None)
@@ -901,16 +909,17 @@ def _serve_levels(args):
         levels.append(lv)
     st = srv.stats()
     srv.close()
+    cfg = {"mode": mode, "request_pairs": "log-uniform 1k-64k", "arrivals": "Poisson",
+           "hbm_budget": 1 << 30, "w": 100}
+    cfg.update({"max_query_len": 256} if resident else {"workers": 2})
     return {"metric": "p50/p99 request latency, streamed serving (continuous batching)",
             "p99_ms_at_95pct": levels[-1]["p99_ms"], "peak_pairs_per_s": round(peak, 1),
             "levels": levels, "data": "synthetic (workloads 1 and 2 generators)",
-            "config": {"request_pairs": "log-uniform 1k-64k", "arrivals": "Poisson",
-                       "hbm_budget": 1 << 30, "workers": 2, "w": 100},
-            "server": st}
+            "config": cfg, "server": st}
 
 
 def _serve(args):
-    r = _serve_levels(args)
+    r = _serve_levels(args, resident=args.serve_mode != "host")
     line = {"metric": r["metric"], "value": r["p99_ms_at_95pct"], "unit": "ms (p99 at 95% load)",
             "n_gpus": 1, "higher_is_better": False, "peak_pairs_per_s": r["peak_pairs_per_s"],
             "levels": r["levels"], "data": r["data"],

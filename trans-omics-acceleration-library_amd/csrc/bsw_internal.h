@@ -148,11 +148,14 @@ struct Sizing {
 TAL_INTERNAL Sizing state_sizing(size_t lds_per_cu, size_t lds_per_block, int waves_per_cu,
                                  int max_ncol);
 /* The persistent serving kernel (bsw_kernels.hip). */
+/* dev: the kernel's device words -- the claim head, its copy of the host's
+ * tail and of the stop flag, the count of worker waves on the GPU (all zero
+ * before the first launch). */
 TAL_INTERNAL hipError_t launch_serve(int grid, size_t lds_bytes, const KParams &K16,
                                      const KParams &K8, const ServeTask *ring, int32_t ring_mask,
                                      const ServeReq *reqs, const ServeCtl *ctl, int32_t *done,
-                                     int64_t *head, int64_t *tail_seen, uint32_t *slabs,
-                                     int64_t idle_ticks, int32_t pollers, hipStream_t s);
+                                     int64_t *dev, uint32_t *slabs, int64_t idle_ticks,
+                                     int32_t pollers, hipStream_t s);
 TAL_INTERNAL hipError_t set_serve_lds_limit(size_t bytes);
 
 } // namespace talbsw

@@ -286,6 +286,39 @@ template <int BITS> __device__ __forceinline__ int table_score(int s1, int c, in
     return amb ? -1 : (s1 == qbyte ? match : mism);
 }
 
+/* Loads and stores of the caller's data.  SYS: the data may have been
+ * written by another agent (DMA, the host) or another wave since this wave
+ * last cached it, and results are read by them: system-scope accesses, which
+ * bypass the caches for these lines only. */
+template <bool SYS, class T> __device__ __forceinline__ T dld(const T *p)
+{
+    if (SYS) return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    return *p;
+}
+/* Sequence bytes through a buffer resource over a packed sequence buffer:
+ * offsets are the pair's offset plus the position (both below 2^31 + 2^15),
+ * and SEQ_OOB for a position past the sequence, which reads 0 with no memory
+ * access.  (OOB itself lies inside this range: it is past the end of the
+ * state slabs only.) */
+constexpr uint32_t SEQ_RANGE = 0x80008000u;
+constexpr uint32_t SEQ_OOB = 0xFFFFFF00u;
+static_assert(SEQ_OOB >= SEQ_RANGE, "the sequence pad offset must be out of range");
+__device__ __forceinline__ rsrc_t seq_rsrc(const uint8_t *base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)base, 0, (int)SEQ_RANGE, RSRC_WORD3);
+}
+template <bool SYS> __device__ __forceinline__ uint32_t seq_byte(rsrc_t r, uint32_t off)
+{
+    return __builtin_amdgcn_raw_buffer_load_b8(r, (int)off, 0, SYS ? 0x11 : 0); /* 0x11: sc0 sc1 */
+}
+template <bool SYS, class T> __device__ __forceinline__ void dst(T *p, T v)
+{
+    if (SYS)
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    else
+        *p = v;
+}
+
 /*
  * Runs group g (or nothing when g < 0) on this half-wave / wave.  The body is
  * the reference's smithWaterman512_{16,8} row loop with its batch wrapper's
@@ -294,7 +327,7 @@ template <int BITS> __device__ __forceinline__ int table_score(int s1, int c, in
  * qb is the byte offset of this lane's first query-code word in the slab
  * (words [word][64 lanes], at the head of the slab).
  */
-template <int BITS>
+template <int BITS, bool SYS = false>
 __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRef &S, uint32_t qb)
 {
     typedef Width<BITS> Wd;
@@ -324,11 +357,11 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
         real = gvalid && p < n_pairs;
         if (real) {
             const tal_seqpair *sp = B.pairs + p;
-            idr = sp->idr;
-            idq = sp->idq;
-            len1 = sp->len1;
-            len2 = sp->len2;
-            h0 = sp->h0;
+            idr = dld<SYS>(&sp->idr);
+            idq = dld<SYS>(&sp->idq);
+            len1 = dld<SYS>(&sp->len1);
+            len2 = dld<SYS>(&sp->len2);
+            h0 = dld<SYS>(&sp->h0);
         } else if (gvalid && BITS == 8) {
             len2 = B.last_len2_dev ? *B.last_len2_dev : Kref.last_len2; /* :2017 */
         }
@@ -384,12 +417,11 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
     int16_t maxScore = h0w, x = 0, y = 0, gscore = Wd::v(-1), max_off = 0, max_ie = 0;
     bool live = true; /* exit0 */
 
-    const uint8_t *rseq = B.ref + (int64_t)idr;
-    const uint8_t *qseq = B.qer + (int64_t)idq;
+    const rsrc_t rbuf = seq_rsrc(B.ref), qbuf = seq_rsrc(B.qer);
 
     S.reset();
     /* ---- H_h row -1 and F (:2867-2879, :3034) ---- */
-    const bool stale = BITS == 16 && B.stale_in && gvalid && ((B.stale_in[g] >> orig) & 1u);
+    const bool stale = BITS == 16 && B.stale_in && gvalid && ((dld<SYS>(B.stale_in + g) >> orig) & 1u);
     /* nzhi: the highest state column of this lane that may hold a nonzero
      * value; blocks above it and above the band need no store */
     int nzhi = -1;
@@ -423,15 +455,14 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
     {
         bool ok = true;
         const int nwords = (ncol + 1 + 7) / 8 + 1;
-        /* 16 byte loads in flight at a time; columns past the query read a
-         * byte of the pair array (any valid address) and become the pad */
-        const uint8_t *qsafe = (const uint8_t *)B.pairs;
+        /* 16 byte loads in flight at a time; columns past the query load
+         * nothing and become the pad */
         for (int wd = 0; wd < nwords; wd += 2) {
             uint32_t by[16];
 #pragma unroll
             for (int k = 0; k < 16; k++) {
                 const int c = wd * 8 + k;
-                by[k] = *(c < len2 ? qseq + c : qsafe);
+                by[k] = seq_byte<SYS>(qbuf, c < len2 ? (uint32_t)idq + (uint32_t)c : SEQ_OOB);
             }
 #pragma unroll
             for (int h = 0; h < 2; h++) {
@@ -460,7 +491,7 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
     bool gdone = !gvalid;
     bool fwritten = false;
     uint64_t dep = 0;
-    int s1_next = 0 < len1 ? (int)rseq[0] : 99;
+    int s1_next = 0 < len1 ? (int)seq_byte<SYS>(rbuf, (uint32_t)idr) : 99;
 
     for (int i = 0;; i++) {
         const bool ract = !gdone && i < nrow;
@@ -470,7 +501,11 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
             continue;
         }
         const int s1 = s1_next; /* ref code of this row; next row's load in flight */
-        s1_next = i + 1 < len1 ? (int)rseq[i + 1] : 99;
+        {
+            const bool more = i + 1 < len1;
+            const uint32_t b = seq_byte<SYS>(rbuf, more ? (uint32_t)idr + (uint32_t)(i + 1) : SEQ_OOB);
+            s1_next = more ? (int)b : 99;
+        }
         /* column range (:3059-3063) */
         beg = nbeg;
         end = nend;
@@ -846,7 +881,7 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                 jj = add16(jj, ONE);
                 const uint32_t sv = S.ld(j);
                 const int16_t h00 = (int16_t)(sv & 0xFFFFu), f11 = (int16_t)(sv >> 16);
-                const int s2 = j < len2 ? (int)qseq[j] : 100;
+                const int s2 = j < len2 ? (int)seq_byte<SYS>(qbuf, (uint32_t)idq + (uint32_t)j) : 100;
                 bool amb;
                 if (BITS == 16)
                     amb = (s1b == 4) | (s2 == 4);
@@ -1041,19 +1076,19 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
 
     if (real) { /* outputs (:3327-3353) */
         tal_seqpair *sp = B.pairs + p;
-        sp->score = Wd::out(maxScore);
-        sp->tle = Wd::out(x);
-        sp->qle = Wd::out(y);
-        sp->max_off = Wd::out(max_off);
-        sp->gscore = Wd::out(gscore);
-        sp->gtle = Wd::out(max_ie);
+        dst<SYS>(&sp->score, Wd::out(maxScore));
+        dst<SYS>(&sp->tle, Wd::out(x));
+        dst<SYS>(&sp->qle, Wd::out(y));
+        dst<SYS>(&sp->max_off, Wd::out(max_off));
+        dst<SYS>(&sp->gscore, Wd::out(gscore));
+        dst<SYS>(&sp->gtle, Wd::out(max_ie));
     }
     if (BITS == 16 && gvalid && B.dep_out) {
         const uint32_t d = (uint32_t)(dep >> (32 * half));
         const bool wr = fwritten;
         if (gl == 0) {
-            B.dep_out[g] = d;
-            B.wrote_out[g] = wr ? 1 : 0;
+            dst<SYS>(B.dep_out + g, d);
+            dst<SYS>(B.wrote_out + g, (uint8_t)(wr ? 1 : 0));
         }
         uint32_t *fm = B.fmask + B.fmask_off[g];
         for (int b = 0; b < ncol; b += 8) {
@@ -1063,7 +1098,7 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
             for (int k = 0; k < 8; k++) {
                 const int nz_o = __builtin_amdgcn_ds_bpermute(inv * 4, (v[k] >> 16) != 0u ? 1 : 0);
                 const uint64_t nz = ballot(nz_o != 0);
-                if (gl == 0 && b + k < ncol) fm[b + k] = (uint32_t)(nz >> (32 * half));
+                if (gl == 0 && b + k < ncol) dst<SYS>(fm + b + k, (uint32_t)(nz >> (32 * half)));
             }
         }
     }
@@ -1129,23 +1164,28 @@ template <class T> __device__ __forceinline__ T *hload_ptr(T *const *p)
  *
  * Claims are a compare-and-swap on `head` (device memory) below tail_seen,
  * the device's copy of the host's tail, so a wave never takes a task that is
- * not published and no task is taken twice or lost.  Blocks below
- * `pollers` read the host's tail (over the bus) on every idle turn, the
- * others on every 16th; in between they wait on tail_seen.
+ * not published and no task is taken twice or lost.  The blocks below
+ * `pollers` are watchers: the only waves that read host memory while idle,
+ * they copy the tail and the stop flag into tail_seen and stop_seen and run
+ * no tasks.  Idle workers watch those device words, sleeping longer on each
+ * idle turn (up to about 55 us), so that idle waves load neither the bus nor
+ * the busy waves' memory.
  *
  * Coherence.  The host writes a task's descriptor, its request's pinned
- * records and (by DMA) its sequence bytes before it raises tail.  After a
- * claim the wave acquires at system scope, which invalidates its L1 and L2,
- * so it reads all of these, and the device bookkeeping written by earlier
- * tasks of the request, as written.  Before marking the task done it
- * releases at system scope, which waits for its stores to host memory and
- * writes its device stores back from L2.
+ * records and (by DMA) its sequence bytes before it raises tail.  Every read
+ * of these, and of the bookkeeping that earlier tasks of the request wrote,
+ * is a system-scope load, and every result a system-scope store: these pass
+ * the caches, so no wave ever reads a line another agent has changed since,
+ * and nothing invalidates or writes back the caches as a whole (they hold
+ * the other waves' state slabs).  The done flag is written once the task's
+ * stores are complete.
  */
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bsw_serve(
     KParams K16, KParams K8, const ServeTask *ring, int32_t ring_mask, const ServeReq *reqs,
-    const ServeCtl *ctl, int32_t *done, int64_t *head, int64_t *tail_seen, uint32_t *slabs,
-    int64_t idle_ticks, int32_t pollers)
+    const ServeCtl *ctl, int32_t *done, int64_t *dev, uint32_t *slabs, int64_t idle_ticks,
+    int32_t pollers)
 {
+    int64_t *head = dev, *tail_seen = dev + 1, *stop_seen = dev + 2, *alive = dev + 3;
     extern __shared__ uint32_t lds_state[];
     const int lane = threadIdx.x;
     uint32_t *slab = slabs + (size_t)blockIdx.x * K16.slab_cols * 64;
@@ -1157,16 +1197,45 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bs
     S.nb = K16.lds_cols >> 3;
     S.nblk = (K16.state_cols + 7) >> 3;
     const uint32_t qb = (uint32_t)lane * 4u;
-    const bool poller = (int)blockIdx.x < pollers;
-    uint32_t polls = 0;
+    if ((int)blockIdx.x < pollers) {
+        /* A watcher: copies the host's tail and stop flag into the device
+         * words the workers watch, and runs no tasks, so that it is never
+         * busy when work arrives.  It stays while any worker stays (workers
+         * leave after idle_ticks without a task). */
+        if (lane == 0) {
+            const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+            for (;;) {
+                const int64_t tl = hload(&ctl->tail);
+                if (tl > __hip_atomic_load(tail_seen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                    __hip_atomic_fetch_max(tail_seen, tl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (hload(&ctl->stop) != 0) {
+                    __hip_atomic_store(stop_seen, (int64_t)1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    break;
+                }
+                /* (after a start-up grace period, in which workers count in) */
+                if (__hip_atomic_load(alive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
+                    (int64_t)(__builtin_amdgcn_s_memrealtime() - t0) > idle_ticks)
+                    break;
+                __builtin_amdgcn_s_sleep(127);
+            }
+        }
+        return;
+    }
+    if (lane == 0) __hip_atomic_fetch_add(alive, (int64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint64_t idle_from = __builtin_amdgcn_s_memrealtime();
     for (;;) {
         int64_t t = -1; /* the claimed task; -1: leave */
         if (lane == 0) {
+            int naps = 1; /* idle turns sleep longer and longer */
             for (;;) {
                 const int64_t h = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                /* (relaxed throughout: what a task reads is read with
+                 * system-scope loads, so no ordering is needed here, and
+                 * acquire/release would invalidate or write back the
+                 * caches that hold the busy waves' slabs) */
                 const int64_t ts =
-                    __hip_atomic_load(tail_seen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_load(tail_seen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (h < ts) {
                     int64_t e = h;
                     if (__hip_atomic_compare_exchange_strong(head, &e, h + 1, __ATOMIC_RELAXED,
@@ -1177,30 +1246,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bs
                     }
                     continue;
                 }
-                if (poller || (++polls & 15) == 0) {
-                    const int64_t tl =
-                        __hip_atomic_load(&ctl->tail, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
-                    if (tl > ts) {
-                        __hip_atomic_fetch_max(tail_seen, tl, __ATOMIC_RELEASE,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-                        continue;
-                    }
-                }
-                if (hload(&ctl->stop) != 0 ||
-                    (int64_t)(__builtin_amdgcn_s_memrealtime() - idle_from) > idle_ticks)
+                if (__hip_atomic_load(stop_seen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
                     break;
-                if (poller)
-                    __builtin_amdgcn_s_sleep(32);
-                else
-                    __builtin_amdgcn_s_sleep(127);
+                if ((int64_t)(__builtin_amdgcn_s_memrealtime() - idle_from) > idle_ticks) break;
+                for (int k = 0; k < naps; k++) __builtin_amdgcn_s_sleep(127);
+                naps = min(naps + 1, 16);
             }
         }
         {
             const int lo = __shfl((int)(uint32_t)t, 0, 64), hi = __shfl((int)(t >> 32), 0, 64);
             t = (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
         }
-        if (t < 0) break;
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "");
+        if (t < 0) {
+            if (lane == 0)
+                __hip_atomic_fetch_add(alive, (int64_t)-1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
         const int slot = (int)(t & ring_mask);
         const ServeTask *tk = ring + slot;
         const int bits = hload(&tk->bits), rq = hload(&tk->req);
@@ -1224,26 +1285,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bs
                 bool rr = false;
                 uint32_t truth = 0;
                 if (g < G) {
-                    const uint32_t d = dep[g];
+                    const uint32_t d = hload(dep + g);
                     if (d != 0u) {
                         const int c = hload(ncol + g);
                         for (int h = g - 1; h >= 0; h--) {
                             const int nh = hload(ncol + h);
                             if (nh > c) {
-                                truth = fmask[hload(foff + h) + c];
+                                truth = hload(fmask + hload(foff + h) + c);
                                 break;
                             }
-                            if (nh == c && wrote[h]) break;
+                            if (nh == c && hload(wrote + h)) break;
                         }
-                        rr = (truth & d) != (stale[g] & d);
+                        rr = (truth & d) != (hload(stale + g) & d);
                     }
                 }
-                if (rr) stale[g] = truth;
+                if (rr) dst<true>(stale + g, truth);
                 const uint64_t m = ballot(rr);
-                if (rr) reruns[1 + count + __popcll(m & ((1ull << lane) - 1))] = g;
+                if (rr) dst<true>(reruns + 1 + count + __popcll(m & ((1ull << lane) - 1)), g);
                 count += __popcll(m);
             }
-            if (lane == 0) reruns[0] = count;
+            if (lane == 0) dst<true>(reruns, count);
         } else if (bits == 16) {
             KParams K = K16;
             K.n = n16;
@@ -1257,7 +1318,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bs
             B.wrote_out = hload_ptr(&R->wrote);
             B.fmask = hload_ptr(&R->fmask);
             B.fmask_off = hload_ptr(&R->fmask_off);
-            run_group<16>(K, B, lane < 32 ? g0 : g1, S, qb);
+            run_group<16, true>(K, B, lane < 32 ? g0 : g1, S, qb);
         } else {
             KParams K = K8;
             K.n = n8;
@@ -1266,9 +1327,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bs
             KBuffers B{};
             B.pairs = hload_ptr(&R->pairs8);
             B.ref = B.qer = hload_ptr(&R->seq);
-            run_group<8>(K, B, g0, S, qb);
+            run_group<8, true>(K, B, g0, S, qb);
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
+        /* every result store of the task (all system-scope, past the
+         * caches) is complete before the done flag is written */
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         if (lane == 0)
             __hip_atomic_store(done + slot, (int32_t)(t + 1), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_SYSTEM);
@@ -1278,11 +1341,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bs
 
 hipError_t launch_serve(int grid, size_t lds_bytes, const KParams &K16, const KParams &K8,
                         const ServeTask *ring, int32_t ring_mask, const ServeReq *reqs,
-                        const ServeCtl *ctl, int32_t *done, int64_t *head, int64_t *tail_seen,
-                        uint32_t *slabs, int64_t idle_ticks, int32_t pollers, hipStream_t s)
+                        const ServeCtl *ctl, int32_t *done, int64_t *dev, uint32_t *slabs,
+                        int64_t idle_ticks, int32_t pollers, hipStream_t s)
 {
     hipLaunchKernelGGL(bsw_serve, dim3(grid), dim3(64), lds_bytes, s, K16, K8, ring, ring_mask,
-                       reqs, ctl, done, head, tail_seen, slabs, idle_ticks, pollers);
+                       reqs, ctl, done, dev, slabs, idle_ticks, pollers);
     return hipGetLastError();
 }
 

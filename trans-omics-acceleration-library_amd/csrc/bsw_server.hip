@@ -36,6 +36,7 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -151,7 +152,7 @@ struct Resident {
     uint8_t *pin = nullptr;    /* pinned arena: pair blocks, group records, staging */
     size_t pin_size = 0;
     Arena pin_alloc;
-    int64_t *head = nullptr, *tail_seen = nullptr; /* device */
+    int64_t *dev = nullptr;    /* device: claim head, tail seen, stop seen, workers alive */
     uint32_t *slabs = nullptr;                     /* device */
     int grid = 0;
     size_t lds_bytes = 0;
@@ -225,11 +226,10 @@ int prepare(tal_bsw_server *s, Request *r)
     return TAL_OK;
 }
 
-/* The padded blocks with offsets moved into the arena. */
-void build_block(const Request *r, const std::vector<int32_t> &idx, int32_t npad, bool eight,
-                 std::vector<tal_seqpair> &out)
+/* The padded blocks with offsets moved into the arena, into out[0, npad). */
+void build_block_at(const Request *r, const std::vector<int32_t> &idx, int32_t npad, bool eight,
+                    tal_seqpair *out)
 {
-    out.assign(npad, tal_seqpair{});
     const int32_t m = (int32_t)idx.size();
     for (int32_t t = 0; t < m; t++) {
         tal_seqpair p = r->pairs[idx[t]];
@@ -244,6 +244,12 @@ void build_block(const Request *r, const std::vector<int32_t> &idx, int32_t npad
         p.len2 = eight ? r->pairs[idx[m - 1]].len2 : 0;
         out[t] = p;
     }
+}
+void build_block(const Request *r, const std::vector<int32_t> &idx, int32_t npad, bool eight,
+                 std::vector<tal_seqpair> &out)
+{
+    out.assign(npad, tal_seqpair{});
+    build_block_at(r, idx, npad, eight, out.data());
 }
 
 void finish(tal_bsw_server *s, Request *r, int status)
@@ -479,8 +485,7 @@ int res_ensure_kernel(tal_bsw_server *s)
     Resident *R = s->res;
     if (res_running(R)) return TAL_OK;
     if (launch_serve(R->grid, R->lds_bytes, R->K16, R->K8, R->ring, (int32_t)(R->cap - 1), R->reqs,
-                     R->ctl, R->done, R->head, R->tail_seen, R->slabs, R->idle_ticks, 32,
-                     R->kstream) != hipSuccess ||
+                     R->ctl, R->done, R->dev, R->slabs, R->idle_ticks, 4, R->kstream) != hipSuccess ||
         hipEventRecord(R->kdone, R->kstream) != hipSuccess)
         return TAL_EDEVICE;
     R->launched = true;
@@ -584,10 +589,9 @@ void res_finish(tal_bsw_server *s, Request *r, int status)
     s->cv.notify_all();
 }
 
-/* Pinned-arena layout of a request: pair blocks, per-group records, then the
- * staging copy of its sequences. */
+/* Pinned-arena layout of a request: pair blocks and per-group records. */
 struct PinLayout {
-    size_t p16, p8, ncol, foff, stale, reruns, seq, bytes;
+    size_t p16, p8, ncol, foff, stale, reruns, bytes;
 };
 PinLayout pin_layout(const Request *r)
 {
@@ -604,7 +608,6 @@ PinLayout pin_layout(const Request *r)
     L.foff = take(sizeof(int64_t) * (size_t)r->G16);
     L.stale = take(sizeof(uint32_t) * (size_t)r->G16);
     L.reruns = take(sizeof(int32_t) * ((size_t)r->G16 + 1));
-    L.seq = take(r->ref_len + r->qer_len);
     L.bytes = o;
     return L;
 }
@@ -613,7 +616,6 @@ void res_admit_loop(tal_bsw_server *s)
 {
     (void)hipSetDevice(s->device);
     Resident *R = s->res;
-    std::vector<tal_seqpair> blk;
     for (;;) {
         Request *r = nullptr;
         {
@@ -687,14 +689,16 @@ void res_admit_loop(tal_bsw_server *s)
         int64_t *foff = (int64_t *)(pin + L.foff);
         uint32_t *stale = (uint32_t *)(pin + L.stale);
         int32_t *reruns = (int32_t *)(pin + L.reruns);
-        if (r->n16p) {
-            build_block(r, r->idx16, r->n16p, false, blk);
-            std::memcpy(p16, blk.data(), sizeof(tal_seqpair) * (size_t)r->n16p);
-        }
-        if (r->n8p) {
-            build_block(r, r->idx8, r->n8p, true, blk);
-            std::memcpy(p8, blk.data(), sizeof(tal_seqpair) * (size_t)r->n8p);
-        }
+        /* the sequences go by DMA while the host builds the rest */
+        hipError_t e = hipSuccess;
+        if (r->ref_len)
+            e = hipMemcpyAsync(s->arena + r->ref_off, r->ref, r->ref_len, hipMemcpyHostToDevice,
+                               s->copy_stream);
+        if (e == hipSuccess && r->qer_len)
+            e = hipMemcpyAsync(s->arena + r->qer_off, r->qer, r->qer_len, hipMemcpyHostToDevice,
+                               s->copy_stream);
+        if (r->n16p) build_block_at(r, r->idx16, r->n16p, false, p16);
+        if (r->n8p) build_block_at(r, r->idx8, r->n8p, true, p8);
         std::vector<ServeTask> tasks = res_group_tasks(r, p16, p8, ncol);
         int64_t fo = 0;
         for (int32_t g = 0; g < r->G16; g++) {
@@ -703,17 +707,10 @@ void res_admit_loop(tal_bsw_server *s)
         }
         std::memset(stale, 0, sizeof(uint32_t) * (size_t)r->G16);
         reruns[0] = 0;
-        /* the sequences: staged in pinned memory, then copied by DMA */
-        std::memcpy(pin + L.seq, r->ref, r->ref_len);
-        std::memcpy(pin + L.seq + r->ref_len, r->qer, r->qer_len);
-        hipError_t e = hipSuccess;
-        if (r->ref_len)
-            e = hipMemcpyAsync(s->arena + r->ref_off, pin + L.seq, r->ref_len,
-                               hipMemcpyHostToDevice, s->copy_stream);
-        if (e == hipSuccess && r->qer_len)
-            e = hipMemcpyAsync(s->arena + r->qer_off, pin + L.seq + r->ref_len, r->qer_len,
-                               hipMemcpyHostToDevice, s->copy_stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(s->copy_stream);
+        /* (also after a failed copy: one queued before it must end before
+         * the slot can be given back) */
+        const hipError_t es = hipStreamSynchronize(s->copy_stream);
+        if (e == hipSuccess) e = es;
         ServeReq q{};
         q.pairs16 = p16;
         q.pairs8 = p8;
@@ -821,7 +818,7 @@ void res_release(tal_bsw_server *s)
     if (R->launched) (void)hipEventSynchronize(R->kdone);
     if (R->kdone) (void)hipEventDestroy(R->kdone);
     if (R->kstream) (void)hipStreamDestroy(R->kstream);
-    for (void *p : {(void *)R->head, (void *)R->slabs})
+    for (void *p : {(void *)R->dev, (void *)R->slabs})
         if (p) (void)hipFree(p);
     for (void *p : {(void *)R->ring, (void *)R->done, (void *)R->reqs, (void *)R->ctl, (void *)R->pin})
         if (p) (void)hipHostFree(p);
@@ -838,9 +835,11 @@ int res_create(tal_bsw_server *s, int max_qlen)
     if (hipGetDeviceProperties(&prop, s->device) != hipSuccess) return TAL_EDEVICE;
     const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor
                                                                 : prop.sharedMemPerBlock;
-    const Sizing z = state_sizing(lds_cu, prop.sharedMemPerBlock, RES_WAVES_PER_CU, max_qlen);
+    int wpc = RES_WAVES_PER_CU;
+    if (const char *e = getenv("TAL_BSW_SERVE_WAVES_PER_CU")) wpc = std::max(1, std::min(atoi(e), 12));
+    const Sizing z = state_sizing(lds_cu, prop.sharedMemPerBlock, wpc, max_qlen);
     R->max_qlen = max_qlen;
-    R->grid = prop.multiProcessorCount * RES_WAVES_PER_CU;
+    R->grid = prop.multiProcessorCount * wpc;
     R->lds_bytes = z.lds_bytes;
     for (int bits : {16, 8}) {
         KParams &K = bits == 16 ? R->K16 : R->K8;
@@ -866,9 +865,8 @@ int res_create(tal_bsw_server *s, int max_qlen)
     R->pin_alloc.reset(R->pin_size);
     for (int k = RES_SLOTS - 1; k >= 0; k--) R->free_slots.push_back(k);
     const size_t slab_bytes = (size_t)z.slab_cols * 64 * sizeof(uint32_t);
-    if (hipMalloc((void **)&R->head, 2 * sizeof(int64_t)) != hipSuccess) return TAL_ENOMEM;
-    R->tail_seen = R->head + 1;
-    if (hipMemset(R->head, 0, 2 * sizeof(int64_t)) != hipSuccess) return TAL_EDEVICE;
+    if (hipMalloc((void **)&R->dev, 4 * sizeof(int64_t)) != hipSuccess) return TAL_ENOMEM;
+    if (hipMemset(R->dev, 0, 4 * sizeof(int64_t)) != hipSuccess) return TAL_EDEVICE;
     if (hipMalloc((void **)&R->slabs, slab_bytes * (size_t)R->grid) != hipSuccess) return TAL_ENOMEM;
     if (set_serve_lds_limit(prop.sharedMemPerBlock) != hipSuccess ||
         hipStreamCreateWithFlags(&R->kstream, hipStreamNonBlocking) != hipSuccess ||
