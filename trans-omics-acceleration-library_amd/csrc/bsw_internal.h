@@ -148,9 +148,15 @@ struct Sizing {
 TAL_INTERNAL Sizing state_sizing(size_t lds_per_cu, size_t lds_per_block, int waves_per_cu,
                                  int max_ncol);
 /* The persistent serving kernel (bsw_kernels.hip). */
-/* dev: the kernel's device words -- the claim head, its copy of the host's
- * tail and of the stop flag, the count of worker waves on the GPU (all zero
- * before the first launch). */
+constexpr int SERVE_DEV_WORDS = 8;
+constexpr int SERVE_CU_KEYS = 2048; /* per-CU rank counters past the words */
+constexpr size_t SERVE_DEV_BYTES = SERVE_DEV_WORDS * sizeof(int64_t) + SERVE_CU_KEYS * sizeof(int32_t);
+void serve_dev_init(int64_t words[SERVE_DEV_WORDS]);
+/* dev: the kernel's SERVE_DEV_WORDS device words -- the claim head, its copy
+ * of the host's tail and of the stop flag, the count of worker waves on the
+ * GPU, the real-time clock of the last claim or finish, the closing flag and
+ * the lowest handed-back ticket, set up by serve_dev_init before the first
+ * launch; then SERVE_CU_KEYS rank counters (SERVE_DEV_BYTES in all). */
 TAL_INTERNAL hipError_t launch_serve(int grid, size_t lds_bytes, const KParams &K16,
                                      const KParams &K8, const ServeTask *ring, int32_t ring_mask,
                                      const ServeReq *reqs, const ServeCtl *ctl, int32_t *done,

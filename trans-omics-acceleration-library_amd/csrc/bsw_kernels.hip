@@ -300,6 +300,10 @@ template <bool SYS, class T> __device__ __forceinline__ T dld(const T *p)
  * and SEQ_OOB for a position past the sequence, which reads 0 with no memory
  * access.  (OOB itself lies inside this range: it is past the end of the
  * state slabs only.) */
+/* s_getreg operands (id | offset << 6 | (size - 1) << 11): the wave's
+ * hardware ids (CU in bits 8-11, SH 12, SE 13-15) and its XCD */
+constexpr int HWREG_HW_ID = 4 | (31 << 11);
+constexpr int HWREG_XCC_ID = 20 | (31 << 11);
 constexpr uint32_t SEQ_RANGE = 0x80008000u;
 constexpr uint32_t SEQ_OOB = 0xFFFFFF00u;
 static_assert(SEQ_OOB >= SEQ_RANGE, "the sequence pad offset must be out of range");
@@ -1159,17 +1163,28 @@ template <class T> __device__ __forceinline__ T *hload_ptr(T *const *p)
  * The persistent serving kernel.  Each 64-thread block is one wave that
  * claims tasks [head, tail) of the host's ring in order, runs them and marks
  * them done.  It exits when the host sets stop or after idle_ticks of the
- * 100 MHz real-time clock without a task; the host relaunches it when work
- * arrives while it is gone.
+ * 100 MHz real-time clock in which no wave claimed or finished a task; the
+ * host relaunches it when work arrives while it is gone, clearing `closing`
+ * first.
  *
- * Claims are a compare-and-swap on `head` (device memory) below tail_seen,
- * the device's copy of the host's tail, so a wave never takes a task that is
- * not published and no task is taken twice or lost.  The blocks below
+ * Claims are tickets: a wave that sees head below tail_seen (the device's
+ * copy of the host's tail) takes the next ticket with a fetch-and-add on
+ * head, which always succeeds (a compare-and-swap loop would make each claim
+ * wait out a round of failed attempts by every idle wave).  A ticket past
+ * the tail is held until its task is published; a wave that leaves while
+ * holding one hands it back (the minimum goes to `handback`, from which the
+ * next launch resumes, skipping tasks whose done flag is set).  So no task
+ * is run twice or lost.  The blocks below
  * `pollers` are watchers: the only waves that read host memory while idle,
  * they copy the tail and the stop flag into tail_seen and stop_seen and run
  * no tasks.  Idle workers watch those device words, sleeping longer on each
  * idle turn (up to about 55 us), so that idle waves load neither the bus nor
  * the busy waves' memory.
+ *
+ * The kernel's own words (dev) are changed with system-scope atomics, which
+ * pass every cache.  Idle waves poll them with agent-scope loads, served by
+ * their XCD's L2 (so that polling puts no load on memory), and every few
+ * turns with a system-scope load, in case that copy is stale.
  *
  * Coherence.  The host writes a task's descriptor, its request's pinned
  * records and (by DMA) its sequence bytes before it raises tail.  Every read
@@ -1186,6 +1201,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bs
     int32_t pollers)
 {
     int64_t *head = dev, *tail_seen = dev + 1, *stop_seen = dev + 2, *alive = dev + 3;
+    int64_t *last_active = dev + 4, *closing = dev + 5, *handback = dev + 6;
+    int32_t *cu_rank = (int32_t *)(dev + SERVE_DEV_WORDS);
     extern __shared__ uint32_t lds_state[];
     const int lane = threadIdx.x;
     uint32_t *slab = slabs + (size_t)blockIdx.x * K16.slab_cols * 64;
@@ -1207,14 +1224,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bs
             for (;;) {
                 const int64_t tl = hload(&ctl->tail);
                 if (tl > __hip_atomic_load(tail_seen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-                    __hip_atomic_fetch_max(tail_seen, tl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_fetch_max(tail_seen, tl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                 if (hload(&ctl->stop) != 0) {
                     __hip_atomic_store(stop_seen, (int64_t)1, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
+                                       __HIP_MEMORY_SCOPE_SYSTEM);
                     break;
                 }
                 /* (after a start-up grace period, in which workers count in) */
-                if (__hip_atomic_load(alive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0 &&
+                if (__hip_atomic_load(alive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) == 0 &&
                     (int64_t)(__builtin_amdgcn_s_memrealtime() - t0) > idle_ticks)
                     break;
                 __builtin_amdgcn_s_sleep(127);
@@ -1222,35 +1239,80 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bs
         }
         return;
     }
-    if (lane == 0) __hip_atomic_fetch_add(alive, (int64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    uint64_t idle_from = __builtin_amdgcn_s_memrealtime();
+    if (lane == 0) __hip_atomic_fetch_add(alive, (int64_t)1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    const int64_t started = (int64_t)__builtin_amdgcn_s_memrealtime();
+    /* This wave's rank among the workers on its CU (by start order): a
+     * rank-r wave that sees work waits about r * 7 us before it draws a
+     * ticket, so that a small request spreads over the CUs before it doubles
+     * up on one. */
+    int rank = 0;
+    if (lane == 0) {
+        const uint32_t hw = __builtin_amdgcn_s_getreg(HWREG_HW_ID);
+        const uint32_t xcc = __builtin_amdgcn_s_getreg(HWREG_XCC_ID);
+        const uint32_t key = ((xcc & 7u) << 8) | ((hw >> 8) & 0xFFu); /* XCD, SE, SH, CU */
+        rank = __hip_atomic_fetch_add(cu_rank + key, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     for (;;) {
         int64_t t = -1; /* the claimed task; -1: leave */
         if (lane == 0) {
-            int naps = 1; /* idle turns sleep longer and longer */
+            /* (relaxed throughout: what a task reads is read with
+             * system-scope loads, so no ordering is needed here, and
+             * acquire/release would invalidate or write back the caches
+             * that hold the busy waves' slabs) */
+            int64_t my = -1; /* a ticket this wave holds */
+            int naps = 1;    /* idle turns sleep longer and longer */
+            int turn = 0;
+            bool primed = false; /* work seen; the rank's wait is done */
             for (;;) {
-                const int64_t h = __hip_atomic_load(head, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                /* (relaxed throughout: what a task reads is read with
-                 * system-scope loads, so no ordering is needed here, and
-                 * acquire/release would invalidate or write back the
-                 * caches that hold the busy waves' slabs) */
-                const int64_t ts =
-                    __hip_atomic_load(tail_seen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if (h < ts) {
-                    int64_t e = h;
-                    if (__hip_atomic_compare_exchange_strong(head, &e, h + 1, __ATOMIC_RELAXED,
-                                                             __ATOMIC_RELAXED,
-                                                             __HIP_MEMORY_SCOPE_AGENT)) {
-                        t = h;
-                        break;
+                /* polls read the XCD's L2; every 8th turn reads memory, so
+                 * a stale line there delays a wave by at most a few turns */
+                const bool fresh = (++turn & 7) == 0;
+                auto ld = [&](const int64_t *p) {
+                    return fresh ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM)
+                                 : __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                };
+                const int64_t ts = ld(tail_seen);
+                if (my >= 0 && my < ts) { /* the ticket's task is published */
+                    t = my;
+                    break;
+                }
+                const bool leave = ld(closing) != 0 || ld(stop_seen) != 0;
+                if (!leave && my < 0 && ld(head) < ts) {
+                    if (rank > 0 && !primed) {
+                        primed = true;
+                        for (int k = 0; k < 2 * min(rank, 16); k++) __builtin_amdgcn_s_sleep(127);
+                        continue;
                     }
+                    my = __hip_atomic_fetch_add(head, (int64_t)1, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_SYSTEM);
+                    naps = 1;
                     continue;
                 }
-                if (__hip_atomic_load(stop_seen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)
+                primed = false;
+                /* The grid leaves as a whole once no wave has claimed or
+                 * finished a task for idle_ticks: a wave that sees that
+                 * raises `closing` and every wave leaves at its next idle
+                 * turn (a wave that leaves alone would leave the kernel
+                 * running with fewer waves, and the host relaunches only a
+                 * kernel that has ended). */
+                bool go = leave;
+                if (!go && (int64_t)__builtin_amdgcn_s_memrealtime() - max(ld(last_active), started) >
+                               idle_ticks) {
+                    __hip_atomic_store(closing, (int64_t)1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_SYSTEM);
+                    go = true;
+                }
+                if (go) {
+                    /* a ticket past the tail goes back: the next launch
+                     * starts its claims there (bsw_serve_reset) */
+                    if (my >= 0)
+                        __hip_atomic_fetch_min(handback, my, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_SYSTEM);
                     break;
-                if ((int64_t)(__builtin_amdgcn_s_memrealtime() - idle_from) > idle_ticks) break;
+                }
+                /* (a ticket holder is next in line: it naps briefly) */
                 for (int k = 0; k < naps; k++) __builtin_amdgcn_s_sleep(127);
-                naps = min(naps + 1, 16);
+                naps = min(naps + 1, my >= 0 ? 4 : 16);
             }
         }
         {
@@ -1259,9 +1321,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bs
         }
         if (t < 0) {
             if (lane == 0)
-                __hip_atomic_fetch_add(alive, (int64_t)-1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_fetch_add(alive, (int64_t)-1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
             break;
         }
+        /* a task below a hand-back may have run already (by its ticket's
+         * holder, after the hand-back of an earlier ticket) */
+        if (hload(done + (t & ring_mask)) == (int32_t)(t + 1)) continue;
+        if (lane == 0)
+            __hip_atomic_fetch_max(last_active, (int64_t)__builtin_amdgcn_s_memrealtime(),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         const int slot = (int)(t & ring_mask);
         const ServeTask *tk = ring + slot;
         const int bits = hload(&tk->bits), rq = hload(&tk->req);
@@ -1332,11 +1400,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bs
         /* every result store of the task (all system-scope, past the
          * caches) is complete before the done flag is written */
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        if (lane == 0)
+        if (lane == 0) {
             __hip_atomic_store(done + slot, (int32_t)(t + 1), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_SYSTEM);
-        idle_from = __builtin_amdgcn_s_memrealtime();
+            __hip_atomic_fetch_max(last_active, (int64_t)__builtin_amdgcn_s_memrealtime(),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
     }
+}
+
+/* Before a launch of bsw_serve: claims resume from the lowest handed-back
+ * ticket, and closing is cleared. */
+__global__ __launch_bounds__(256) void bsw_serve_reset(int64_t *dev)
+{
+    int32_t *cu_rank = (int32_t *)(dev + SERVE_DEV_WORDS);
+    for (int k = threadIdx.x; k < SERVE_CU_KEYS; k += blockDim.x) cu_rank[k] = 0;
+    if (threadIdx.x == 0) {
+        if (dev[6] < dev[0]) dev[0] = dev[6];
+        dev[5] = 0;
+        dev[6] = INT64_MAX;
+    }
+}
+
+void serve_dev_init(int64_t words[SERVE_DEV_WORDS])
+{
+    for (int k = 0; k < SERVE_DEV_WORDS; k++) words[k] = 0;
+    words[6] = INT64_MAX;
 }
 
 hipError_t launch_serve(int grid, size_t lds_bytes, const KParams &K16, const KParams &K8,
@@ -1344,6 +1433,7 @@ hipError_t launch_serve(int grid, size_t lds_bytes, const KParams &K16, const KP
                         const ServeCtl *ctl, int32_t *done, int64_t *dev, uint32_t *slabs,
                         int64_t idle_ticks, int32_t pollers, hipStream_t s)
 {
+    hipLaunchKernelGGL(bsw_serve_reset, dim3(1), dim3(256), 0, s, dev);
     hipLaunchKernelGGL(bsw_serve, dim3(grid), dim3(64), lds_bytes, s, K16, K8, ring, ring_mask,
                        reqs, ctl, done, dev, slabs, idle_ticks, pollers);
     return hipGetLastError();

@@ -123,6 +123,7 @@ struct Request {
     /* resident mode */
     int slot = -1;                  /* ServeReq slot */
     size_t pin_off = 0, pin_bytes = 0; /* pinned-arena slot */
+    tal_seqpair *hp16 = nullptr, *hp8 = nullptr; /* pinned pair blocks (device copies at p16_off/p8_off) */
     int32_t G16 = 0, G8 = 0;        /* groups of each width */
     int64_t t_first = 0, t_end = 0; /* the tasks being waited for: [t_first, t_end) */
     int64_t cursor = 0;             /* first of them not seen done */
@@ -152,7 +153,7 @@ struct Resident {
     uint8_t *pin = nullptr;    /* pinned arena: pair blocks, group records, staging */
     size_t pin_size = 0;
     Arena pin_alloc;
-    int64_t *dev = nullptr;    /* device: claim head, tail seen, stop seen, workers alive */
+    int64_t *dev = nullptr;    /* device: the kernel's words (launch_serve) */
     uint32_t *slabs = nullptr;                     /* device */
     int grid = 0;
     size_t lds_bytes = 0;
@@ -162,7 +163,11 @@ struct Resident {
     std::vector<int> free_slots;
     std::vector<Request *> live; /* requests with tasks on the device */
     hipStream_t kstream = nullptr;
+    hipStream_t dstream = nullptr; /* results back to the host */
     hipEvent_t kdone = nullptr;
+    uint8_t *stage[2] = {nullptr, nullptr}; /* pinned staging of caller bytes */
+    hipEvent_t staged[2] = {nullptr, nullptr};
+    int stage_next = 0;
     bool launched = false;
     int64_t idle_ticks = 0;
     std::thread monitor;
@@ -172,6 +177,7 @@ constexpr int RES_SLOTS = 4096;
 constexpr int64_t RES_RING = (int64_t)1 << 20;
 constexpr size_t RES_PINNED = (size_t)512 << 20;
 constexpr int RES_WAVES_PER_CU = 8; /* two per SIMD: room for other kernels' waves */
+constexpr size_t RES_STAGE = (size_t)2 << 20; /* bytes per staging buffer */
 
 } // namespace
 
@@ -479,6 +485,26 @@ bool res_running(const Resident *R)
     return R->launched && hipEventQuery(R->kdone) == hipErrorNotReady;
 }
 
+/* Caller bytes to the device by way of the two pinned staging buffers: the
+ * host copies a chunk while the DMA engine moves the one before.  (A copy
+ * straight from pageable memory is staged by the runtime, which measured
+ * ~4.5 GB/s next to the running kernel.)  The copies are queued on st. */
+hipError_t res_stage_copy(Resident *R, hipStream_t st, uint8_t *dst, const uint8_t *src, size_t n)
+{
+    for (size_t o = 0; o < n; o += RES_STAGE) {
+        const int k = R->stage_next;
+        R->stage_next ^= 1;
+        hipError_t e = hipEventSynchronize(R->staged[k]); /* its last DMA is done */
+        if (e != hipSuccess) return e;
+        const size_t c = std::min(RES_STAGE, n - o);
+        std::memcpy(R->stage[k], src + o, c);
+        if ((e = hipMemcpyAsync(dst + o, R->stage[k], c, hipMemcpyHostToDevice, st)) != hipSuccess ||
+            (e = hipEventRecord(R->staged[k], st)) != hipSuccess)
+            return e;
+    }
+    return hipSuccess;
+}
+
 /* Launches the persistent kernel unless it is running (s->mu held). */
 int res_ensure_kernel(tal_bsw_server *s)
 {
@@ -562,7 +588,22 @@ void res_finish(tal_bsw_server *s, Request *r, int status)
 {
     Resident *R = s->res;
     if (status == TAL_OK) {
-        const tal_seqpair *p16 = R->reqs[r->slot].pairs16, *p8 = R->reqs[r->slot].pairs8;
+        /* the results come back with the pair blocks (the kernel's stores
+         * were complete when it raised the done flags) */
+        const tal_seqpair *p16 = r->hp16, *p8 = r->hp8;
+        hipError_t e = hipSuccess;
+        if (r->n16p)
+            e = hipMemcpyAsync(r->hp16, s->arena + r->p16_off, sizeof(tal_seqpair) * r->n16p,
+                               hipMemcpyDeviceToHost, R->dstream);
+        if (e == hipSuccess && r->n8p)
+            e = hipMemcpyAsync(r->hp8, s->arena + r->p8_off, sizeof(tal_seqpair) * r->n8p,
+                               hipMemcpyDeviceToHost, R->dstream);
+        const hipError_t es = hipStreamSynchronize(R->dstream);
+        if (e == hipSuccess) e = es;
+        if (e != hipSuccess) status = TAL_EDEVICE;
+    }
+    if (status == TAL_OK) {
+        const tal_seqpair *p16 = r->hp16, *p8 = r->hp8;
         auto put = [&](const std::vector<int32_t> &idx, const tal_seqpair *blk) {
             for (size_t t = 0; t < idx.size(); t++) {
                 const tal_seqpair &q = blk[t];
@@ -641,9 +682,12 @@ void res_admit_loop(tal_bsw_server *s)
                     nc = std::max(nc, r->pairs[r->idx16[32 * g + l]].len2);
                 fcols += (size_t)nc;
             }
-            /* device footprint: sequences plus the 16-bit bookkeeping */
+            /* device footprint: sequences, pair blocks and the 16-bit
+             * bookkeeping */
             r->bytes = align_up(std::max<size_t>(r->ref_len, 1)) +
                        align_up(std::max<size_t>(r->qer_len, 1)) +
+                       align_up(sizeof(tal_seqpair) * (size_t)std::max(r->n16p, 1)) +
+                       align_up(sizeof(tal_seqpair) * (size_t)std::max(r->n8p, 1)) +
                        align_up(sizeof(uint32_t) * (size_t)std::max(r->G16, 1)) +
                        align_up((size_t)std::max(r->G16, 1)) +
                        align_up(sizeof(uint32_t) * std::max<size_t>(fcols, 1));
@@ -677,28 +721,40 @@ void res_admit_loop(tal_bsw_server *s)
             r->slot = R->free_slots.back();
             R->free_slots.pop_back();
         }
-        /* device arena: ref, qer, dep, wrote, fmask */
+        /* device arena: ref, qer, pair blocks, dep, wrote, fmask */
         r->ref_off = r->off;
         r->qer_off = r->ref_off + align_up(std::max<size_t>(r->ref_len, 1));
-        const size_t dep_off = r->qer_off + align_up(std::max<size_t>(r->qer_len, 1));
+        r->p16_off = r->qer_off + align_up(std::max<size_t>(r->qer_len, 1));
+        r->p8_off = r->p16_off + align_up(sizeof(tal_seqpair) * (size_t)std::max(r->n16p, 1));
+        const size_t dep_off = r->p8_off + align_up(sizeof(tal_seqpair) * (size_t)std::max(r->n8p, 1));
         const size_t wrote_off = dep_off + align_up(sizeof(uint32_t) * (size_t)std::max(r->G16, 1));
         const size_t fmask_off = wrote_off + align_up((size_t)std::max(r->G16, 1));
         uint8_t *pin = R->pin + r->pin_off;
         tal_seqpair *p16 = (tal_seqpair *)(pin + L.p16), *p8 = (tal_seqpair *)(pin + L.p8);
+        r->hp16 = p16;
+        r->hp8 = p8;
         int32_t *ncol = (int32_t *)(pin + L.ncol);
         int64_t *foff = (int64_t *)(pin + L.foff);
         uint32_t *stale = (uint32_t *)(pin + L.stale);
         int32_t *reruns = (int32_t *)(pin + L.reruns);
-        /* the sequences go by DMA while the host builds the rest */
+        /* the sequences go by DMA (the last chunk while the host builds
+         * the rest) */
         hipError_t e = hipSuccess;
         if (r->ref_len)
-            e = hipMemcpyAsync(s->arena + r->ref_off, r->ref, r->ref_len, hipMemcpyHostToDevice,
-                               s->copy_stream);
+            e = res_stage_copy(R, s->copy_stream, s->arena + r->ref_off, r->ref, r->ref_len);
         if (e == hipSuccess && r->qer_len)
-            e = hipMemcpyAsync(s->arena + r->qer_off, r->qer, r->qer_len, hipMemcpyHostToDevice,
-                               s->copy_stream);
+            e = res_stage_copy(R, s->copy_stream, s->arena + r->qer_off, r->qer, r->qer_len);
         if (r->n16p) build_block_at(r, r->idx16, r->n16p, false, p16);
         if (r->n8p) build_block_at(r, r->idx8, r->n8p, true, p8);
+        /* the pair blocks live in device memory while the kernel works on
+         * them: read and written there, they put no small transfers on the
+         * bus, which the sequences' DMA shares */
+        if (e == hipSuccess && r->n16p)
+            e = hipMemcpyAsync(s->arena + r->p16_off, p16, sizeof(tal_seqpair) * r->n16p,
+                               hipMemcpyHostToDevice, s->copy_stream);
+        if (e == hipSuccess && r->n8p)
+            e = hipMemcpyAsync(s->arena + r->p8_off, p8, sizeof(tal_seqpair) * r->n8p,
+                               hipMemcpyHostToDevice, s->copy_stream);
         std::vector<ServeTask> tasks = res_group_tasks(r, p16, p8, ncol);
         int64_t fo = 0;
         for (int32_t g = 0; g < r->G16; g++) {
@@ -712,8 +768,8 @@ void res_admit_loop(tal_bsw_server *s)
         const hipError_t es = hipStreamSynchronize(s->copy_stream);
         if (e == hipSuccess) e = es;
         ServeReq q{};
-        q.pairs16 = p16;
-        q.pairs8 = p8;
+        q.pairs16 = (tal_seqpair *)(s->arena + r->p16_off);
+        q.pairs8 = (tal_seqpair *)(s->arena + r->p8_off);
         q.seq = s->arena;
         q.stale = stale;
         q.dep = (uint32_t *)(s->arena + dep_off);
@@ -818,9 +874,13 @@ void res_release(tal_bsw_server *s)
     if (R->launched) (void)hipEventSynchronize(R->kdone);
     if (R->kdone) (void)hipEventDestroy(R->kdone);
     if (R->kstream) (void)hipStreamDestroy(R->kstream);
+    if (R->dstream) (void)hipStreamDestroy(R->dstream);
     for (void *p : {(void *)R->dev, (void *)R->slabs})
         if (p) (void)hipFree(p);
-    for (void *p : {(void *)R->ring, (void *)R->done, (void *)R->reqs, (void *)R->ctl, (void *)R->pin})
+    for (hipEvent_t ev : R->staged)
+        if (ev) (void)hipEventDestroy(ev);
+    for (void *p : {(void *)R->ring, (void *)R->done, (void *)R->reqs, (void *)R->ctl, (void *)R->pin,
+                    (void *)R->stage[0], (void *)R->stage[1]})
         if (p) (void)hipHostFree(p);
     delete R;
     s->res = nullptr;
@@ -857,7 +917,9 @@ int res_create(tal_bsw_server *s, int max_qlen)
         hipHostMalloc((void **)&R->done, sizeof(int32_t) * (size_t)R->cap, fl) != hipSuccess ||
         hipHostMalloc((void **)&R->reqs, sizeof(ServeReq) * RES_SLOTS, fl) != hipSuccess ||
         hipHostMalloc((void **)&R->ctl, sizeof(ServeCtl), fl) != hipSuccess ||
-        hipHostMalloc((void **)&R->pin, R->pin_size, fl) != hipSuccess)
+        hipHostMalloc((void **)&R->pin, R->pin_size, fl) != hipSuccess ||
+        hipHostMalloc((void **)&R->stage[0], RES_STAGE, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&R->stage[1], RES_STAGE, hipHostMallocDefault) != hipSuccess)
         return TAL_ENOMEM;
     std::memset(R->done, 0, sizeof(int32_t) * (size_t)R->cap);
     std::memset(R->reqs, 0, sizeof(ServeReq) * RES_SLOTS);
@@ -865,12 +927,18 @@ int res_create(tal_bsw_server *s, int max_qlen)
     R->pin_alloc.reset(R->pin_size);
     for (int k = RES_SLOTS - 1; k >= 0; k--) R->free_slots.push_back(k);
     const size_t slab_bytes = (size_t)z.slab_cols * 64 * sizeof(uint32_t);
-    if (hipMalloc((void **)&R->dev, 4 * sizeof(int64_t)) != hipSuccess) return TAL_ENOMEM;
-    if (hipMemset(R->dev, 0, 4 * sizeof(int64_t)) != hipSuccess) return TAL_EDEVICE;
+    int64_t words[SERVE_DEV_WORDS];
+    serve_dev_init(words);
+    if (hipMalloc((void **)&R->dev, SERVE_DEV_BYTES) != hipSuccess) return TAL_ENOMEM;
+    if (hipMemcpy(R->dev, words, sizeof(words), hipMemcpyHostToDevice) != hipSuccess)
+        return TAL_EDEVICE;
     if (hipMalloc((void **)&R->slabs, slab_bytes * (size_t)R->grid) != hipSuccess) return TAL_ENOMEM;
     if (set_serve_lds_limit(prop.sharedMemPerBlock) != hipSuccess ||
         hipStreamCreateWithFlags(&R->kstream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&R->kdone, hipEventDisableTiming) != hipSuccess)
+        hipStreamCreateWithFlags(&R->dstream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&R->kdone, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&R->staged[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&R->staged[1], hipEventDisableTiming) != hipSuccess)
         return TAL_EDEVICE;
     return TAL_OK;
 }
