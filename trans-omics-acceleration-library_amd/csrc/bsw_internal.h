@@ -89,16 +89,31 @@ struct GroupShape {
 /* One task of request slot `req`: bits 16, two 16-bit groups (one per
  * half-wave, g1 = -1 for none); bits 8, one 8-bit group g0; bits 0, the
  * stale-column resolution of the request's 16-bit groups. */
+/* A task of the persistent serving kernel for request slot `req`: bits 16, a
+ * pair of 16-bit groups g0, g1 (g1 = -1: one); bits 8, the 8-bit group g0;
+ * bits 0, the stale-column resolution of the request's 16-bit groups; bits
+ * SERVE_COPY_IN / SERVE_COPY_OUT, chunk g0 (SERVE_COPY_CHUNK bytes) of the
+ * request's copy in or out. */
 struct ServeTask {
     int32_t req, bits, g0, g1;
 };
+constexpr int32_t SERVE_COPY_IN = 1, SERVE_COPY_OUT = 2;
+constexpr int64_t SERVE_COPY_CHUNK = (int64_t)256 << 10;
 
 /* A request slot.  Group g of the 16-bit (8-bit) block holds pairs
  * [32g, 32g+32) ([64g, 64g+64)) of pairs16 (pairs8); idr/idq of every record
- * are offsets into `seq`.  Pinned host memory: the pair blocks (outputs
- * written in place), fmask_off, ncol16 and reruns; device memory: seq and
- * the 16-bit stale-column bookkeeping. */
+ * are offsets into `seq`.  Device memory: the sequences, the pair blocks
+ * (outputs written in place) and the 16-bit stale-column bookkeeping; pinned
+ * host memory: fmask_off, ncol16, stale and reruns, and a mirror of the
+ * device layout's first cin_bytes (sequences and pair blocks), which the
+ * copy tasks move in (cin_*) and, for the pair blocks, back out (cout_*).
+ * Every copy address and size is a multiple of 256 bytes. */
 struct ServeReq {
+    const uint8_t *cin_src;
+    uint8_t *cin_dst;
+    const uint8_t *cout_src;
+    uint8_t *cout_dst;
+    int64_t cin_bytes, cout_bytes;
     tal_seqpair *pairs16, *pairs8;
     const uint8_t *seq;
     uint32_t *stale;   /* per 16-bit group: lanes whose F[ncol] starts nonzero */

@@ -1336,7 +1336,35 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bs
         const int g0 = hload(&tk->g0), g1 = hload(&tk->g1);
         const ServeReq *R = reqs + rq;
         const int32_t n16 = hload(&R->n16), n8 = hload(&R->n8), w = hload(&R->w);
-        if (bits == 0) {
+        if (bits == SERVE_COPY_IN || bits == SERVE_COPY_OUT) {
+            /* One chunk of a request's copy, in 16-byte pieces, 8 KiB of
+             * loads in flight per turn; system-scope on both sides (the
+             * pinned source was written by the host, device stores are read
+             * by other waves' system-scope loads).  Sizes and addresses are
+             * multiples of 256 bytes, so no piece crosses the range. */
+            const bool in = bits == SERVE_COPY_IN;
+            const uint8_t *src = hload_ptr(in ? &R->cin_src : &R->cout_src);
+            uint8_t *dstp = hload_ptr(in ? &R->cin_dst : &R->cout_dst);
+            const int64_t total = hload(in ? &R->cin_bytes : &R->cout_bytes);
+            const int64_t lo = (int64_t)g0 * SERVE_COPY_CHUNK;
+            const int n = (int)min(total - lo, SERVE_COPY_CHUNK);
+            if (n > 0) {
+                const rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void *)(src + lo), 0, n,
+                                                                    RSRC_WORD3);
+                const rsrc_t rd = __builtin_amdgcn_make_buffer_rsrc(dstp + lo, 0, n, RSRC_WORD3);
+                for (int base = 0; base < n; base += 8192) {
+                    u32x4 v[8];
+#pragma unroll
+                    for (int k = 0; k < 8; k++)
+                        v[k] = __builtin_amdgcn_raw_buffer_load_b128(
+                            rs, base + k * 1024 + lane * 16, 0, 0x11);
+#pragma unroll
+                    for (int k = 0; k < 8; k++)
+                        __builtin_amdgcn_raw_buffer_store_b128(v[k], rd, base + k * 1024 + lane * 16,
+                                                               0, 0x11);
+                }
+            }
+        } else if (bits == 0) {
             /* Stale-column resolution of the request's 16-bit groups from the
              * zero image of a fresh object (bsw_resolve with one segment):
              * lanes take groups, the groups to rerun go to the rerun list. */

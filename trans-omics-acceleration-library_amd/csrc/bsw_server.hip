@@ -123,7 +123,8 @@ struct Request {
     /* resident mode */
     int slot = -1;                  /* ServeReq slot */
     size_t pin_off = 0, pin_bytes = 0; /* pinned-arena slot */
-    tal_seqpair *hp16 = nullptr, *hp8 = nullptr; /* pinned pair blocks (device copies at p16_off/p8_off) */
+    tal_seqpair *hp16 = nullptr, *hp8 = nullptr; /* pinned mirrors of the pair blocks */
+    std::vector<ServeTask> gtasks;  /* group tasks, published once the copy-in is done */
     int32_t G16 = 0, G8 = 0;        /* groups of each width */
     int64_t t_first = 0, t_end = 0; /* the tasks being waited for: [t_first, t_end) */
     int64_t cursor = 0;             /* first of them not seen done */
@@ -163,11 +164,7 @@ struct Resident {
     std::vector<int> free_slots;
     std::vector<Request *> live; /* requests with tasks on the device */
     hipStream_t kstream = nullptr;
-    hipStream_t dstream = nullptr; /* results back to the host */
     hipEvent_t kdone = nullptr;
-    uint8_t *stage[2] = {nullptr, nullptr}; /* pinned staging of caller bytes */
-    hipEvent_t staged[2] = {nullptr, nullptr};
-    int stage_next = 0;
     bool launched = false;
     int64_t idle_ticks = 0;
     std::thread monitor;
@@ -175,9 +172,8 @@ struct Resident {
 
 constexpr int RES_SLOTS = 4096;
 constexpr int64_t RES_RING = (int64_t)1 << 20;
-constexpr size_t RES_PINNED = (size_t)512 << 20;
+constexpr size_t RES_PINNED = (size_t)1 << 30;
 constexpr int RES_WAVES_PER_CU = 8; /* two per SIMD: room for other kernels' waves */
-constexpr size_t RES_STAGE = (size_t)2 << 20; /* bytes per staging buffer */
 
 } // namespace
 
@@ -485,26 +481,6 @@ bool res_running(const Resident *R)
     return R->launched && hipEventQuery(R->kdone) == hipErrorNotReady;
 }
 
-/* Caller bytes to the device by way of the two pinned staging buffers: the
- * host copies a chunk while the DMA engine moves the one before.  (A copy
- * straight from pageable memory is staged by the runtime, which measured
- * ~4.5 GB/s next to the running kernel.)  The copies are queued on st. */
-hipError_t res_stage_copy(Resident *R, hipStream_t st, uint8_t *dst, const uint8_t *src, size_t n)
-{
-    for (size_t o = 0; o < n; o += RES_STAGE) {
-        const int k = R->stage_next;
-        R->stage_next ^= 1;
-        hipError_t e = hipEventSynchronize(R->staged[k]); /* its last DMA is done */
-        if (e != hipSuccess) return e;
-        const size_t c = std::min(RES_STAGE, n - o);
-        std::memcpy(R->stage[k], src + o, c);
-        if ((e = hipMemcpyAsync(dst + o, R->stage[k], c, hipMemcpyHostToDevice, st)) != hipSuccess ||
-            (e = hipEventRecord(R->staged[k], st)) != hipSuccess)
-            return e;
-    }
-    return hipSuccess;
-}
-
 /* Launches the persistent kernel unless it is running (s->mu held). */
 int res_ensure_kernel(tal_bsw_server *s)
 {
@@ -588,21 +564,7 @@ void res_finish(tal_bsw_server *s, Request *r, int status)
 {
     Resident *R = s->res;
     if (status == TAL_OK) {
-        /* the results come back with the pair blocks (the kernel's stores
-         * were complete when it raised the done flags) */
-        const tal_seqpair *p16 = r->hp16, *p8 = r->hp8;
-        hipError_t e = hipSuccess;
-        if (r->n16p)
-            e = hipMemcpyAsync(r->hp16, s->arena + r->p16_off, sizeof(tal_seqpair) * r->n16p,
-                               hipMemcpyDeviceToHost, R->dstream);
-        if (e == hipSuccess && r->n8p)
-            e = hipMemcpyAsync(r->hp8, s->arena + r->p8_off, sizeof(tal_seqpair) * r->n8p,
-                               hipMemcpyDeviceToHost, R->dstream);
-        const hipError_t es = hipStreamSynchronize(R->dstream);
-        if (e == hipSuccess) e = es;
-        if (e != hipSuccess) status = TAL_EDEVICE;
-    }
-    if (status == TAL_OK) {
+        /* (the copy-out tasks brought the pair blocks back to the mirror) */
         const tal_seqpair *p16 = r->hp16, *p8 = r->hp8;
         auto put = [&](const std::vector<int32_t> &idx, const tal_seqpair *blk) {
             for (size_t t = 0; t < idx.size(); t++) {
@@ -630,21 +592,47 @@ void res_finish(tal_bsw_server *s, Request *r, int status)
     s->cv.notify_all();
 }
 
-/* Pinned-arena layout of a request: pair blocks and per-group records. */
-struct PinLayout {
-    size_t p16, p8, ncol, foff, stale, reruns, bytes;
+/* Device-arena layout of a request (offsets from its slot): the sequences,
+ * the pair blocks, then the 16-bit bookkeeping; `mirror` bytes from the
+ * start (sequences and pair blocks) are also laid out in pinned memory. */
+struct DevLayout {
+    size_t ref, qer, p16, p8, dep, wrote, fmask, bytes, mirror;
 };
-PinLayout pin_layout(const Request *r)
+DevLayout dev_layout(const Request *r, size_t fcols)
 {
-    PinLayout L{};
+    DevLayout D{};
     size_t o = 0;
     auto take = [&](size_t b) {
         const size_t at = o;
         o += align_up(std::max<size_t>(b, 1));
         return at;
     };
-    L.p16 = take(sizeof(tal_seqpair) * (size_t)r->n16p);
-    L.p8 = take(sizeof(tal_seqpair) * (size_t)r->n8p);
+    D.ref = take(r->ref_len);
+    D.qer = take(r->qer_len);
+    D.p16 = take(sizeof(tal_seqpair) * (size_t)r->n16p);
+    D.p8 = take(sizeof(tal_seqpair) * (size_t)r->n8p);
+    D.mirror = o;
+    D.dep = take(sizeof(uint32_t) * (size_t)r->G16);
+    D.wrote = take((size_t)r->G16);
+    D.fmask = take(sizeof(uint32_t) * fcols);
+    D.bytes = o;
+    return D;
+}
+
+/* Pinned-arena layout of a request: the mirror of the device layout's
+ * sequences and pair blocks, then the per-group records. */
+struct PinLayout {
+    size_t ncol, foff, stale, reruns, bytes;
+};
+PinLayout pin_layout(const Request *r, const DevLayout &D)
+{
+    PinLayout L{};
+    size_t o = D.mirror;
+    auto take = [&](size_t b) {
+        const size_t at = o;
+        o += align_up(std::max<size_t>(b, 1));
+        return at;
+    };
     L.ncol = take(sizeof(int32_t) * (size_t)r->G16);
     L.foff = take(sizeof(int64_t) * (size_t)r->G16);
     L.stale = take(sizeof(uint32_t) * (size_t)r->G16);
@@ -673,24 +661,20 @@ void res_admit_loop(tal_bsw_server *s)
         r->G16 = r->n16p / 32;
         r->G8 = r->n8p / 64;
         PinLayout L{};
-        size_t fcols = 0; /* F-mask words of the 16-bit groups */
+        DevLayout D{};
+        int64_t nin = 0; /* copy-in chunks */
         if (rc == TAL_OK) {
-            L = pin_layout(r);
+            size_t fcols = 0; /* F-mask words of the 16-bit groups */
             for (int32_t g = 0; g < r->G16; g++) {
                 int nc = 0;
                 for (int l = 0; l < 32 && 32 * g + l < (int32_t)r->idx16.size(); l++)
                     nc = std::max(nc, r->pairs[r->idx16[32 * g + l]].len2);
                 fcols += (size_t)nc;
             }
-            /* device footprint: sequences, pair blocks and the 16-bit
-             * bookkeeping */
-            r->bytes = align_up(std::max<size_t>(r->ref_len, 1)) +
-                       align_up(std::max<size_t>(r->qer_len, 1)) +
-                       align_up(sizeof(tal_seqpair) * (size_t)std::max(r->n16p, 1)) +
-                       align_up(sizeof(tal_seqpair) * (size_t)std::max(r->n8p, 1)) +
-                       align_up(sizeof(uint32_t) * (size_t)std::max(r->G16, 1)) +
-                       align_up((size_t)std::max(r->G16, 1)) +
-                       align_up(sizeof(uint32_t) * std::max<size_t>(fcols, 1));
+            D = dev_layout(r, fcols);
+            L = pin_layout(r, D);
+            r->bytes = D.bytes; /* device footprint */
+            nin = ((int64_t)D.mirror + SERVE_COPY_CHUNK - 1) / SERVE_COPY_CHUNK;
             if (r->bytes > s->budget || L.bytes > R->pin_size) rc = TAL_ERANGE;
         }
         {
@@ -698,7 +682,8 @@ void res_admit_loop(tal_bsw_server *s)
             if (rc == TAL_OK) /* arrival order: wait for room for the head */
                 s->cv.wait(lk, [&] {
                     if (R->free_slots.empty()) return false;
-                    if (R->tail - res_oldest(R) + r->G16 + r->G8 + 2 > R->cap / 2) return false;
+                    if (R->tail - res_oldest(R) + r->G16 + r->G8 + 2 * nin + 2 > R->cap / 2)
+                        return false;
                     size_t po = 0, dof = 0;
                     if (!R->pin_alloc.take(L.bytes, &po)) return false;
                     if (!s->alloc.take(r->bytes, &dof)) {
@@ -722,40 +707,28 @@ void res_admit_loop(tal_bsw_server *s)
             R->free_slots.pop_back();
         }
         /* device arena: ref, qer, pair blocks, dep, wrote, fmask */
-        r->ref_off = r->off;
-        r->qer_off = r->ref_off + align_up(std::max<size_t>(r->ref_len, 1));
-        r->p16_off = r->qer_off + align_up(std::max<size_t>(r->qer_len, 1));
-        r->p8_off = r->p16_off + align_up(sizeof(tal_seqpair) * (size_t)std::max(r->n16p, 1));
-        const size_t dep_off = r->p8_off + align_up(sizeof(tal_seqpair) * (size_t)std::max(r->n8p, 1));
-        const size_t wrote_off = dep_off + align_up(sizeof(uint32_t) * (size_t)std::max(r->G16, 1));
-        const size_t fmask_off = wrote_off + align_up((size_t)std::max(r->G16, 1));
+        r->ref_off = r->off + D.ref;
+        r->qer_off = r->off + D.qer;
+        r->p16_off = r->off + D.p16;
+        r->p8_off = r->off + D.p8;
+        /* The pinned mirror: the host fills it, the kernel's copy-in tasks
+         * move it to the device and its copy-out tasks bring the pair blocks
+         * back.  (No DMA and no other stream: work queued on another stream
+         * can sit behind the persistent kernel when the two share a
+         * hardware queue.) */
         uint8_t *pin = R->pin + r->pin_off;
-        tal_seqpair *p16 = (tal_seqpair *)(pin + L.p16), *p8 = (tal_seqpair *)(pin + L.p8);
+        if (r->ref_len) std::memcpy(pin + D.ref, r->ref, r->ref_len);
+        if (r->qer_len) std::memcpy(pin + D.qer, r->qer, r->qer_len);
+        tal_seqpair *p16 = (tal_seqpair *)(pin + D.p16), *p8 = (tal_seqpair *)(pin + D.p8);
         r->hp16 = p16;
         r->hp8 = p8;
         int32_t *ncol = (int32_t *)(pin + L.ncol);
         int64_t *foff = (int64_t *)(pin + L.foff);
         uint32_t *stale = (uint32_t *)(pin + L.stale);
         int32_t *reruns = (int32_t *)(pin + L.reruns);
-        /* the sequences go by DMA (the last chunk while the host builds
-         * the rest) */
-        hipError_t e = hipSuccess;
-        if (r->ref_len)
-            e = res_stage_copy(R, s->copy_stream, s->arena + r->ref_off, r->ref, r->ref_len);
-        if (e == hipSuccess && r->qer_len)
-            e = res_stage_copy(R, s->copy_stream, s->arena + r->qer_off, r->qer, r->qer_len);
         if (r->n16p) build_block_at(r, r->idx16, r->n16p, false, p16);
         if (r->n8p) build_block_at(r, r->idx8, r->n8p, true, p8);
-        /* the pair blocks live in device memory while the kernel works on
-         * them: read and written there, they put no small transfers on the
-         * bus, which the sequences' DMA shares */
-        if (e == hipSuccess && r->n16p)
-            e = hipMemcpyAsync(s->arena + r->p16_off, p16, sizeof(tal_seqpair) * r->n16p,
-                               hipMemcpyHostToDevice, s->copy_stream);
-        if (e == hipSuccess && r->n8p)
-            e = hipMemcpyAsync(s->arena + r->p8_off, p8, sizeof(tal_seqpair) * r->n8p,
-                               hipMemcpyHostToDevice, s->copy_stream);
-        std::vector<ServeTask> tasks = res_group_tasks(r, p16, p8, ncol);
+        r->gtasks = res_group_tasks(r, p16, p8, ncol);
         int64_t fo = 0;
         for (int32_t g = 0; g < r->G16; g++) {
             foff[g] = fo;
@@ -763,44 +736,45 @@ void res_admit_loop(tal_bsw_server *s)
         }
         std::memset(stale, 0, sizeof(uint32_t) * (size_t)r->G16);
         reruns[0] = 0;
-        /* (also after a failed copy: one queued before it must end before
-         * the slot can be given back) */
-        const hipError_t es = hipStreamSynchronize(s->copy_stream);
-        if (e == hipSuccess) e = es;
         ServeReq q{};
+        q.cin_src = pin;
+        q.cin_dst = s->arena + r->off;
+        q.cin_bytes = (int64_t)D.mirror;
+        q.cout_src = s->arena + r->p16_off;
+        q.cout_dst = pin + D.p16;
+        q.cout_bytes = (int64_t)(D.mirror - D.p16);
         q.pairs16 = (tal_seqpair *)(s->arena + r->p16_off);
         q.pairs8 = (tal_seqpair *)(s->arena + r->p8_off);
         q.seq = s->arena;
         q.stale = stale;
-        q.dep = (uint32_t *)(s->arena + dep_off);
-        q.wrote = s->arena + wrote_off;
-        q.fmask = (uint32_t *)(s->arena + fmask_off);
+        q.dep = (uint32_t *)(s->arena + r->off + D.dep);
+        q.wrote = s->arena + r->off + D.wrote;
+        q.fmask = (uint32_t *)(s->arena + r->off + D.fmask);
         q.fmask_off = foff;
         q.ncol16 = ncol;
         q.reruns = reruns;
         q.n16 = r->n16p;
         q.n8 = r->n8p;
         q.w = r->w;
+        std::vector<ServeTask> cin;
+        for (int64_t k = 0; k < nin; k++) cin.push_back({r->slot, SERVE_COPY_IN, (int32_t)k, -1});
         std::unique_lock<std::mutex> lk(s->mu);
-        if (e != hipSuccess) {
-            res_finish(s, r, TAL_EDEVICE);
-        } else {
-            R->reqs[r->slot] = q;
-            r->t_admit = now_ms();
-            R->live.push_back(r);
-            r->in_flight = (int)R->live.size();
-            /* room was checked at admission; should the monitor's tasks have
-             * taken it meanwhile, wait for the device to drain some */
-            while (!res_publish(s, r, tasks)) {
-                lk.unlock();
-                std::this_thread::sleep_for(std::chrono::microseconds(50));
-                lk.lock();
-            }
-            const int krc = res_ensure_kernel(s);
-            if (krc != TAL_OK) {
-                r->keep = true;
-                res_finish(s, r, krc);
-            }
+        R->reqs[r->slot] = q;
+        r->t_admit = now_ms();
+        R->live.push_back(r);
+        r->in_flight = (int)R->live.size();
+        r->phase = 3;
+        /* room was checked at admission; should the monitor's tasks have
+         * taken it meanwhile, wait for the device to drain some */
+        while (!res_publish(s, r, cin)) {
+            lk.unlock();
+            std::this_thread::sleep_for(std::chrono::microseconds(50));
+            lk.lock();
+        }
+        const int krc = res_ensure_kernel(s);
+        if (krc != TAL_OK) {
+            r->keep = true;
+            res_finish(s, r, krc);
         }
         s->admitting = false;
         s->cv.notify_all();
@@ -823,12 +797,35 @@ void res_monitor_loop(tal_bsw_server *s)
                        (int32_t)(r->cursor + 1))
                 r->cursor++;
             if (r->cursor < r->t_end) continue;
+            /* the copy-out of the pair blocks: the request's last phase */
+            auto copy_out = [&]() {
+                const int64_t nb = R->reqs[r->slot].cout_bytes;
+                std::vector<ServeTask> t;
+                for (int64_t k = 0; k * SERVE_COPY_CHUNK < nb; k++)
+                    t.push_back({r->slot, SERVE_COPY_OUT, (int32_t)k, -1});
+                /* (no room in the ring yet: the next round tries again) */
+                if (res_publish(s, r, t)) {
+                    r->phase = 4;
+                    moved = true;
+                }
+            };
+            if (r->phase == 3) { /* copied in: the groups */
+                if (res_publish(s, r, r->gtasks)) {
+                    r->phase = 0;
+                    moved = true;
+                }
+                continue;
+            }
+            if (r->phase == 4) { /* copied out: done */
+                res_finish(s, r, TAL_OK);
+                moved = true;
+                continue;
+            }
             if (r->phase == 1) { /* a resolution finished: reruns, or done */
                 const int32_t *rr = R->reqs[r->slot].reruns;
                 const int32_t nrr = rr[0];
                 if (nrr == 0) {
-                    res_finish(s, r, TAL_OK);
-                    moved = true;
+                    copy_out();
                     continue;
                 }
                 std::vector<ServeTask> t;
@@ -840,14 +837,13 @@ void res_monitor_loop(tal_bsw_server *s)
                     s->st.reruns += nrr;
                     moved = true;
                 }
-            } else if (r->G16 > 0) { /* groups or reruns finished: resolve */
+            } else if (r->G16 > 0) { /* groups or reruns finished: resolve (again) */
                 if (res_publish(s, r, {ServeTask{r->slot, 0, 0, -1}})) {
                     r->phase = 1;
                     moved = true;
                 }
-            } else {
-                res_finish(s, r, TAL_OK);
-                moved = true;
+            } else { /* no 16-bit groups */
+                copy_out();
             }
         }
         if (!R->live.empty() && !res_running(R)) {
@@ -874,13 +870,9 @@ void res_release(tal_bsw_server *s)
     if (R->launched) (void)hipEventSynchronize(R->kdone);
     if (R->kdone) (void)hipEventDestroy(R->kdone);
     if (R->kstream) (void)hipStreamDestroy(R->kstream);
-    if (R->dstream) (void)hipStreamDestroy(R->dstream);
     for (void *p : {(void *)R->dev, (void *)R->slabs})
         if (p) (void)hipFree(p);
-    for (hipEvent_t ev : R->staged)
-        if (ev) (void)hipEventDestroy(ev);
-    for (void *p : {(void *)R->ring, (void *)R->done, (void *)R->reqs, (void *)R->ctl, (void *)R->pin,
-                    (void *)R->stage[0], (void *)R->stage[1]})
+    for (void *p : {(void *)R->ring, (void *)R->done, (void *)R->reqs, (void *)R->ctl, (void *)R->pin})
         if (p) (void)hipHostFree(p);
     delete R;
     s->res = nullptr;
@@ -917,9 +909,7 @@ int res_create(tal_bsw_server *s, int max_qlen)
         hipHostMalloc((void **)&R->done, sizeof(int32_t) * (size_t)R->cap, fl) != hipSuccess ||
         hipHostMalloc((void **)&R->reqs, sizeof(ServeReq) * RES_SLOTS, fl) != hipSuccess ||
         hipHostMalloc((void **)&R->ctl, sizeof(ServeCtl), fl) != hipSuccess ||
-        hipHostMalloc((void **)&R->pin, R->pin_size, fl) != hipSuccess ||
-        hipHostMalloc((void **)&R->stage[0], RES_STAGE, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&R->stage[1], RES_STAGE, hipHostMallocDefault) != hipSuccess)
+        hipHostMalloc((void **)&R->pin, R->pin_size, fl) != hipSuccess)
         return TAL_ENOMEM;
     std::memset(R->done, 0, sizeof(int32_t) * (size_t)R->cap);
     std::memset(R->reqs, 0, sizeof(ServeReq) * RES_SLOTS);
@@ -935,10 +925,7 @@ int res_create(tal_bsw_server *s, int max_qlen)
     if (hipMalloc((void **)&R->slabs, slab_bytes * (size_t)R->grid) != hipSuccess) return TAL_ENOMEM;
     if (set_serve_lds_limit(prop.sharedMemPerBlock) != hipSuccess ||
         hipStreamCreateWithFlags(&R->kstream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&R->dstream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&R->kdone, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&R->staged[0], hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&R->staged[1], hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&R->kdone, hipEventDisableTiming) != hipSuccess)
         return TAL_EDEVICE;
     return TAL_OK;
 }
