@@ -142,6 +142,78 @@ struct Request {
 
 namespace {
 
+/* Host copies split over a few helper threads (the admission thread takes
+ * the first slice): a request's sequences go into its pinned mirror at the
+ * rate of several cores. */
+struct CopyPool {
+    struct Slice {
+        uint8_t *dst = nullptr;
+        const uint8_t *src = nullptr;
+        size_t n = 0;
+    };
+    std::mutex mu;
+    std::condition_variable go, done;
+    std::vector<Slice> slices; /* one per helper */
+    std::vector<std::thread> helpers;
+    uint64_t round = 0;
+    int pending = 0;
+    bool stop = false;
+
+    void start(int k)
+    {
+        slices.resize((size_t)k);
+        for (int i = 0; i < k; i++) helpers.emplace_back([this, i] { run(i); });
+    }
+    void run(int i)
+    {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> lk(mu);
+        for (;;) {
+            go.wait(lk, [&] { return stop || round != seen; });
+            if (stop) return;
+            seen = round;
+            const Slice sl = slices[(size_t)i];
+            lk.unlock();
+            if (sl.n) std::memcpy(sl.dst, sl.src, sl.n);
+            lk.lock();
+            if (--pending == 0) done.notify_all();
+        }
+    }
+    void copy(uint8_t *dst, const uint8_t *src, size_t n)
+    {
+        const size_t parts = std::min(helpers.size() + 1, std::max<size_t>(1, n >> 20));
+        if (parts <= 1) {
+            std::memcpy(dst, src, n);
+            return;
+        }
+        const size_t per = (n / parts + 255) & ~(size_t)255;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            for (size_t i = 0; i < helpers.size(); i++) {
+                const size_t lo = std::min(n, per * (i + 1)), hi = std::min(n, per * (i + 2));
+                slices[i] = Slice{dst + lo, src + lo, hi - lo};
+            }
+            pending = (int)helpers.size();
+            round++;
+        }
+        go.notify_all();
+        std::memcpy(dst, src, std::min(per, n));
+        std::unique_lock<std::mutex> lk(mu);
+        done.wait(lk, [&] { return pending == 0; });
+    }
+    void shutdown()
+    {
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            stop = true;
+        }
+        go.notify_all();
+        for (std::thread &t : helpers)
+            if (t.joinable()) t.join();
+        helpers.clear();
+    }
+};
+
 /* Resident-mode state: the task ring and request slots in pinned host
  * memory, the claim counters and state slabs of the persistent kernel in
  * device memory. */
@@ -168,6 +240,7 @@ struct Resident {
     bool launched = false;
     int64_t idle_ticks = 0;
     std::thread monitor;
+    CopyPool copies;
 };
 
 constexpr int RES_SLOTS = 4096;
@@ -717,8 +790,8 @@ void res_admit_loop(tal_bsw_server *s)
          * can sit behind the persistent kernel when the two share a
          * hardware queue.) */
         uint8_t *pin = R->pin + r->pin_off;
-        if (r->ref_len) std::memcpy(pin + D.ref, r->ref, r->ref_len);
-        if (r->qer_len) std::memcpy(pin + D.qer, r->qer, r->qer_len);
+        if (r->ref_len) R->copies.copy(pin + D.ref, r->ref, r->ref_len);
+        if (r->qer_len) R->copies.copy(pin + D.qer, r->qer, r->qer_len);
         tal_seqpair *p16 = (tal_seqpair *)(pin + D.p16), *p8 = (tal_seqpair *)(pin + D.p8);
         r->hp16 = p16;
         r->hp8 = p8;
@@ -866,6 +939,7 @@ void res_release(tal_bsw_server *s)
 {
     Resident *R = s->res;
     if (!R) return;
+    R->copies.shutdown();
     if (R->ctl) __atomic_store_n(&R->ctl->stop, 1, __ATOMIC_RELEASE);
     if (R->launched) (void)hipEventSynchronize(R->kdone);
     if (R->kdone) (void)hipEventDestroy(R->kdone);
@@ -915,6 +989,7 @@ int res_create(tal_bsw_server *s, int max_qlen)
     std::memset(R->reqs, 0, sizeof(ServeReq) * RES_SLOTS);
     std::memset(R->ctl, 0, sizeof(ServeCtl));
     R->pin_alloc.reset(R->pin_size);
+    R->copies.start(3);
     for (int k = RES_SLOTS - 1; k >= 0; k--) R->free_slots.push_back(k);
     const size_t slab_bytes = (size_t)z.slab_cols * 64 * sizeof(uint32_t);
     int64_t words[SERVE_DEV_WORDS];
