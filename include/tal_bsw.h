@@ -273,10 +273,13 @@ int tal_bsw_set_tuning(tal_bsw *h, int32_t waves_per_cu, int32_t order);
  * With resident = 1 no batch is launched: one persistent kernel stays on the
  * GPU and takes tasks (a pair of 16-bit groups, an 8-bit group, or a
  * request's stale-column resolution) from a ring in pinned host memory.
- * Admission copies a request's sequences into device memory under the same
- * budget (its device footprint is the sequences plus its 16-bit F masks)
- * and publishes its tasks at once, so a request admitted while others run is
- * taken up by waves as they free up.  Queries longer than max_query_len are
+ * Admission lays a request's sequences and padded pair blocks out in pinned
+ * memory and publishes its tasks at once: the kernel copies them to device
+ * memory under the same budget (its device footprint is those plus its
+ * 16-bit F masks), scores them and copies the results back, so a request
+ * admitted while others run is taken up by waves as they free up.  The
+ * server queues no work on other streams; a stream of the caller's that
+ * shares a hardware queue with the kernel waits while it runs.  Queries longer than max_query_len are
  * TAL_ERANGE in this mode.  The kernel leaves after 50 ms without work and
  * is launched again when work arrives; the same independence holds.
  */
