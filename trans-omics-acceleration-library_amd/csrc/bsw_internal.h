@@ -39,6 +39,11 @@ struct KParams {
     int16_t e_del_s, oe_del_s, e_ins_s, oe_ins_s, zdrop_s;
 };
 
+/* Per-group shape: rows (max len1) and columns (max len2) of the group. */
+struct GroupShape {
+    int32_t nrow, ncol;
+};
+
 struct KBuffers {
     tal_seqpair *pairs;
     const uint8_t *ref;
@@ -48,7 +53,7 @@ struct KBuffers {
     int32_t *task_counter;      /* persistent-grid work counter */
     uint32_t *slabs;            /* per-wave slabs: query words, state past LDS */
     /* 16-bit stale-column bookkeeping (all null for the 8-bit path) */
-    const uint32_t *stale_in; /* per group: lanes whose F[ncol] is nonzero */
+    uint32_t *stale_in;       /* per group: lanes whose F[ncol] is nonzero */
     uint32_t *dep_out;        /* per group: lanes whose tail depended on it */
     uint8_t *wrote_out;       /* per group: F[ncol] stored during the group */
     uint32_t *fmask;          /* per group final F nonzero masks, c < ncol */
@@ -58,6 +63,11 @@ struct KBuffers {
     const int32_t *ntasks_dev;    /* tasks of this launch */
     const int32_t *last_len2_dev; /* 8-bit padding length */
     const int32_t *abort_flag;    /* nonzero: the batch left its plan; do nothing */
+    /* 16-bit groups settled inside the launch (null gstate: none are) */
+    const GroupShape *shapes;     /* per group rows and columns */
+    const uint32_t *image;        /* F[ncol] masks before the batch (per column) */
+    int32_t *gstate;              /* per group: 1 = final (zeroed before the launch) */
+    int32_t *early;               /* groups run again inside the launch (counter) */
 };
 
 /* Parameters of the per-pair (scalar) kernel: the object's gaps and the
@@ -75,10 +85,6 @@ hipError_t launch_scalar(int grid, const ScalarParams &P, tal_seqpair *pairs, co
                          const uint8_t *qer, const int32_t *order, int32_t ntasks,
                          int32_t *counter, int2 *slabs, int32_t *err, hipStream_t s);
 
-/* Per-group shape: rows (max len1) and columns (max len2) of the group. */
-struct GroupShape {
-    int32_t nrow, ncol;
-};
 
 /* ---- device-resident serving (bsw_server.hip, resident mode) ----
  * Everything below lives in pinned host memory, written by the host and read

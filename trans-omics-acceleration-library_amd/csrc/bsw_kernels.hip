@@ -332,7 +332,8 @@ template <bool SYS, class T> __device__ __forceinline__ void dst(T *p, T v)
  * (words [word][64 lanes], at the head of the slab).
  */
 template <int BITS, bool SYS = false>
-__device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRef &S, uint32_t qb)
+__device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRef &S, uint32_t qb,
+                          int64_t stale_word = -1)
 {
     typedef Width<BITS> Wd;
     constexpr int GL = Wd::GL;
@@ -425,7 +426,11 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
 
     S.reset();
     /* ---- H_h row -1 and F (:2867-2879, :3034) ---- */
-    const bool stale = BITS == 16 && B.stale_in && gvalid && ((dld<SYS>(B.stale_in + g) >> orig) & 1u);
+    /* (stale_word >= 0: the group's mask, given by the caller) */
+    const bool stale =
+        BITS == 16 && gvalid &&
+        (stale_word >= 0 ? (((uint32_t)stale_word >> orig) & 1u) != 0
+                         : B.stale_in && ((dld<SYS>(B.stale_in + g) >> orig) & 1u));
     /* nzhi: the highest state column of this lane that may hold a nonzero
      * value; blocks above it and above the band need no store */
     int nzhi = -1;
@@ -1090,9 +1095,11 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
     if (BITS == 16 && gvalid && B.dep_out) {
         const uint32_t d = (uint32_t)(dep >> (32 * half));
         const bool wr = fwritten;
+        /* (system-scope, written through: waves of the same launch may
+         * settle later groups on them) */
         if (gl == 0) {
-            dst<SYS>(B.dep_out + g, d);
-            dst<SYS>(B.wrote_out + g, (uint8_t)(wr ? 1 : 0));
+            dst<true>(B.dep_out + g, d);
+            dst<true>(B.wrote_out + g, (uint8_t)(wr ? 1 : 0));
         }
         uint32_t *fm = B.fmask + B.fmask_off[g];
         for (int b = 0; b < ncol; b += 8) {
@@ -1102,10 +1109,79 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
             for (int k = 0; k < 8; k++) {
                 const int nz_o = __builtin_amdgcn_ds_bpermute(inv * 4, (v[k] >> 16) != 0u ? 1 : 0);
                 const uint64_t nz = ballot(nz_o != 0);
-                if (gl == 0 && b + k < ncol) dst<SYS>(fm + b + k, (uint32_t)(nz >> (32 * half)));
+                if (gl == 0 && b + k < ncol) dst<true>(fm + b + k, (uint32_t)(nz >> (32 * half)));
             }
         }
     }
+}
+
+/* Groups scanned back at most when a finished group is settled in the
+ * launch; past it the group is left to the resolution after the launch. */
+constexpr int SETTLE_SCAN = 64;
+
+/*
+ * Settles the wave's two 16-bit groups right after they ran, inside the
+ * launch, when that is already possible -- the stale-column resolution of
+ * bsw_resolve for one group, done early.  A group whose run did not depend
+ * on its F[ncol] (dep == 0) is final.  Otherwise its true mask comes from
+ * the last earlier group that stored column c = ncol: if that group and the
+ * groups on the way with ncol == c (whose `wrote` decides) are final, the
+ * truth is known; equal to the assumed mask on the dep lanes, the group is
+ * final, else it runs again at once with the true mask and is then final.
+ * A group that cannot be settled yet stays pending, for the resolution and
+ * rerun rounds after the launch, which read the same records (a settled
+ * group's stale mask holds its truth, so they leave it alone).  Final
+ * groups are flagged in gstate after all their system-scope stores; the
+ * flags and the records they guard are read with system-scope loads.
+ * Returns the half-wave's verdict: 0 final, 1 run again with `truth`,
+ * 2 pending.
+ */
+__device__ int settle16(const KBuffers &B, int g, uint32_t &truth)
+{
+    const int lane = threadIdx.x, half = lane >> 5, gl = lane & 31;
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    int verdict = 0; /* 0 final, 1 run again with `truth`, 2 pending */
+    uint32_t tv = 0;
+    if (gl == 0 && g >= 0) {
+        const uint32_t d = dld<true>(B.dep_out + g);
+        if (d != 0u) {
+            verdict = 2;
+            const int c = B.shapes[g].ncol;
+            uint32_t tr = B.image[c];
+            bool known = true;
+            for (int h = g - 1, steps = 0; h >= 0; h--, steps++) {
+                if (steps == SETTLE_SCAN) {
+                    known = false;
+                    break;
+                }
+                const int nh = B.shapes[h].ncol;
+                if (nh < c) continue;
+                if (dld<true>(B.gstate + h) == 0) {
+                    known = false;
+                    break;
+                }
+                if (nh > c) {
+                    tr = dld<true>(B.fmask + B.fmask_off[h] + c);
+                    break;
+                }
+                if (dld<true>(B.wrote_out + h) != 0) {
+                    tr = 0u;
+                    break;
+                }
+            }
+            if (known) {
+                if ((tr & d) != (dld<true>(B.stale_in + g) & d)) {
+                    verdict = 1;
+                    tv = tr;
+                } else {
+                    verdict = 0;
+                }
+            }
+        }
+    }
+    verdict = __shfl(verdict, half << 5, 64);
+    truth = (uint32_t)__shfl((int)tv, half << 5, 64);
+    return verdict;
 }
 
 /* Persistent grid: each 64-thread block is one wave that pulls tasks. */
@@ -1142,7 +1218,30 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bs
             g = t * Wd::GPW + half;
             if (g >= K.ngroups) g = -1;
         }
-        run_group<BITS>(K, B, g, S, qb);
+        /* one call site of run_group (a second would be a second inlined
+         * copy): a group settled as "run again" comes round once more */
+        int gg = g;
+        int64_t sw = -1;
+        for (bool again = false;; again = true) {
+            run_group<BITS>(K, B, gg, S, qb, sw);
+            if (BITS != 16 || !B.gstate) break;
+            const int gl = lane & 31;
+            if (again) { /* the reruns, with their true masks, are final */
+                asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+                if (gl == 0 && gg >= 0) dst<true>(B.gstate + gg, 1);
+                break;
+            }
+            uint32_t truth;
+            const int verdict = settle16(B, g, truth);
+            if (gl == 0 && g >= 0 && verdict == 0) dst<true>(B.gstate + g, 1);
+            if (ballot(verdict == 1) == 0) break;
+            if (gl == 0 && verdict == 1) {
+                dst<true>(B.stale_in + g, truth);
+                if (B.early) atomicAdd(B.early, 1);
+            }
+            gg = verdict == 1 ? g : -1;
+            sw = verdict == 1 ? (int64_t)truth : -1;
+        }
     }
 }
 

@@ -128,7 +128,7 @@ struct Arena {
 /* Per-width scratch of one call. */
 struct PathBufs {
     DevBuf shapes, ncp1, foff, dep, wrote, stale, fmask, rerun, tasks, counter, slabs, last,
-        cubtmp, summary, ckey, ckey2, gids, order;
+        cubtmp, summary, ckey, ckey2, gids, order, gstate;
 };
 
 } // namespace
@@ -487,6 +487,12 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
         HIPCHK(h, hipMemsetAsync(stale, 0, sizeof(uint32_t) * G, s), "clear stale");
         B.stale_in = stale;
         B.fmask_off = foff;
+        if (!seg_first) { /* one call: groups are settled inside the launch too */
+            TAKE(h, pb.gstate, sizeof(int32_t) * G, "alloc group states", &B.gstate);
+            HIPCHK(h, hipMemsetAsync(B.gstate, 0, sizeof(int32_t) * G, s), "clear group states");
+            B.shapes = shapes;
+            B.image = fresh ? h->zero_image.as<uint32_t>() : h->image.as<uint32_t>();
+        }
     }
     /* The state slabs come last: a no-allocation call runs as many blocks as
      * the rest of the static scratch holds slabs for (the grid is persistent,
@@ -499,12 +505,13 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
         grid = (int)std::min<size_t>((size_t)grid, std::max<size_t>(fit, 1));
     }
     TAKE(h, pb.slabs, (size_t)grid * slab_bytes, "alloc state slabs", &B.slabs);
-    HIPCHK(h, hipMemsetAsync(B.task_counter, 0, sizeof(int32_t), s), "clear counter");
+    if (B.gstate) B.early = B.task_counter + 2;
+    HIPCHK(h, hipMemsetAsync(B.task_counter, 0, sizeof(int32_t) * 3, s), "clear counters");
     hipEvent_t ev_beg = h->ev[bits == 16 ? 0 : 2], ev_end = h->ev[bits == 16 ? 1 : 3];
     HIPCHK(h, hipEventRecord(ev_beg, s), "event");
     HIPCHK(h, launch_main(bits, lds, grid, lds ? lds_bytes : 0, K, B, s), "main kernel");
     HIPCHK(h, hipEventRecord(ev_end, s), "event");
-    int32_t reruns = 0;
+    int32_t reruns = 0, early = 0;
     float ms_rerun = 0.f;
 
     if (bits == 16) {
@@ -515,9 +522,11 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
                                      fresh ? h->zero_image.as<uint32_t>() : h->image.as<uint32_t>(),
                                      seg_first, stale, list, cnt, s),
                    "resolve");
-            int32_t c = 0;
-            HIPCHK(h, hipMemcpyAsync(&c, cnt, sizeof c, hipMemcpyDeviceToHost, s), "read reruns");
+            int32_t cc[2] = {0, 0}; /* reruns listed; groups run again inside launches */
+            HIPCHK(h, hipMemcpyAsync(cc, cnt, sizeof cc, hipMemcpyDeviceToHost, s), "read reruns");
             HIPCHK(h, hipStreamSynchronize(s), "sync reruns");
+            const int32_t c = cc[0];
+            early = cc[1];
             if (c == 0) break;
             reruns += c;
             if (iter > G + 1) return fail(h, TAL_EDEVICE, "stale-column resolution did not converge");
@@ -544,7 +553,7 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
         h->ev16 = true;
         h->st.ms_rerun16 = ms_rerun;
         h->st.groups16 = G;
-        h->st.reruns16 = reruns;
+        h->st.reruns16 = reruns + (B.gstate ? early : 0);
         h->st.lds_state16 = lds_cols >= ((state_cols + 7) & ~7) ? 1 : 0;
         if (!fresh) {
             HIPCHK(h, hipMemsetAsync(last, 0xFF, sizeof(int32_t) * IMAGE_COLS, s), "clear last");
@@ -687,6 +696,7 @@ int plan_width(tal_bsw *h, tal_bsw_plan *P, int bits, hipStream_t s)
         HIPCHK(h, pb.rerun.ensure(sizeof(int32_t) * (G + 1)), "alloc rerun");
         HIPCHK(h, pb.tasks.ensure(sizeof(int32_t) * (size_t)(G + 2) * 2), "alloc tasks");
         HIPCHK(h, pb.last.ensure(sizeof(int32_t) * IMAGE_COLS), "alloc last");
+        HIPCHK(h, pb.gstate.ensure(sizeof(int32_t) * G), "alloc group states");
         hipLaunchKernelGGL(stride_offsets, dim3((G + 255) / 256), dim3(256), 0, s,
                            pb.foff.as<int64_t>(), G, stride);
         HIPCHK(h, hipGetLastError(), "offsets");
@@ -695,6 +705,10 @@ int plan_width(tal_bsw *h, tal_bsw_plan *P, int bits, hipStream_t s)
         B.wrote_out = pb.wrote.as<uint8_t>();
         B.fmask = pb.fmask.as<uint32_t>();
         B.fmask_off = pb.foff.as<int64_t>();
+        B.shapes = pb.shapes.as<GroupShape>();
+        B.image = h->image.as<uint32_t>();
+        B.gstate = pb.gstate.as<int32_t>();
+        B.early = P->ctl.as<int32_t>() + 6;
     }
     return TAL_OK;
 }
@@ -730,6 +744,7 @@ int plan_enqueue(tal_bsw *h, tal_bsw_plan *P, hipStream_t s)
         HIPCHK(h, launch_plan_check16(shapes, G, W.max_ncol, ctl, s), "check16");
         hipLaunchKernelGGL(fill_i32, dim3((G + 255) / 256), dim3(256), 0, s,
                            (int32_t *)W.pb.stale.p, G, 0);
+        hipLaunchKernelGGL(fill_i32, dim3((G + 255) / 256), dim3(256), 0, s, W.B.gstate, G, 0);
         hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, s, W.B.task_counter, 2, 0);
         HIPCHK(h, launch_main(16, true, W.grid, W.z.lds_bytes, W.K, W.B, s), "main16");
     }
@@ -1316,7 +1331,7 @@ int tal_bsw_plan_launch(tal_bsw_plan *P, void *stream)
     h->st.pairs16 = P->n16;
     h->st.groups8 = P->p8.G;
     h->st.groups16 = P->p16.G;
-    h->st.reruns16 = ctl[5];
+    h->st.reruns16 = ctl[5] + ctl[6];
     if (ctl[0] != 0)
         return fail(h, TAL_ERANGE, "the batch no longer fits its plan (dispatcher split or "
                                    "group columns changed); build a new plan");
