@@ -190,6 +190,9 @@ typedef struct tal_bsw_stats {
     int32_t pairs16;    /* pairs sent to the 16-bit path */
     float ms_replay;    /* whole graph replay of the last plan launch (the
                            per-kernel fields stay 0 for plan launches) */
+    int32_t settled16;  /* of reruns16: groups run again inside the 16-bit
+                           main launch (their predecessors were final when
+                           they ended), the rest ran after it */
 } tal_bsw_stats;
 
 int tal_bsw_get_stats(tal_bsw *h, tal_bsw_stats *out);

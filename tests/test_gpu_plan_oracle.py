@@ -118,6 +118,24 @@ def test_stale_batch_eager():
     eng.close()
 
 
+def test_stale_batch_settled_in_launch():
+    """In input order each unit's two groups share a wave: the long group,
+    whose run does not read F[ncol], is final when it ends, so the group that
+    takes its true mask from it is settled and run again inside the main
+    launch (not by the resolution after it); outputs as the oracle's."""
+    _say("[gpu] stale-column batch, groups settled inside the launch")
+    pairs, ref, qer = stale_batch.make(32, 29)
+    eng = T.BandedPairWiseSW()
+    eng.set_tuning(8, 0)
+    o = O.Oracle()
+    got = eng.get_scores16(pairs.copy(), ref, qer, 100)
+    st = eng.stats()
+    eng.close()
+    assert st["settled16"] > 0, "no group was settled inside the launch"
+    assert st["settled16"] <= st["reruns16"]
+    assert_same_outputs(got, o.scores16(pairs.copy(), ref, qer, 100), "settled in the launch")
+
+
 @pytest.mark.parametrize("rounds", [2, 0])
 def test_stale_batch_plan(rounds):
     _say(f"[gpu] stale-column batch, plan with {rounds} rerun rounds in the graph")

@@ -68,6 +68,10 @@ struct KBuffers {
     const uint32_t *image;        /* F[ncol] masks before the batch (per column) */
     int32_t *gstate;              /* per group: 1 = final (zeroed before the launch) */
     int32_t *early;               /* groups run again inside the launch (counter) */
+    int32_t *pending;             /* [0] groups listed, [1] taken, then one slot per
+                                     group for those left pending (null: no
+                                     end-of-launch pass; before the launch [0] and
+                                     [1] are zeroed and the slots set to -1) */
 };
 
 /* Parameters of the per-pair (scalar) kernel: the object's gaps and the

@@ -1142,8 +1142,12 @@ __device__ int settle16(const KBuffers &B, int g, uint32_t &truth)
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     int verdict = 0; /* 0 final, 1 run again with `truth`, 2 pending */
     uint32_t tv = 0;
+    const uint32_t d = gl == 0 && g >= 0 ? dld<true>(B.dep_out + g) : 0u;
+    /* first the groups that did not depend on F[ncol]: final now, and
+     * flagged before the other half looks (it may be its predecessor) */
+    if (gl == 0 && g >= 0 && d == 0u) dst<true>(B.gstate + g, 1);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     if (gl == 0 && g >= 0) {
-        const uint32_t d = dld<true>(B.dep_out + g);
         if (d != 0u) {
             verdict = 2;
             const int c = B.shapes[g].ncol;
@@ -1210,23 +1214,42 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bs
         int t = 0;
         if (lane == 0) t = atomicAdd(B.task_counter, 1);
         t = __shfl(t, 0, 64);
-        if (t >= ntasks) break;
         int g;
-        if (B.task_groups)
-            g = B.task_groups[t * Wd::GPW + half];
-        else {
-            g = t * Wd::GPW + half;
-            if (g >= K.ngroups) g = -1;
+        bool run = true; /* false: a pending group, settled without running first */
+        if (t < ntasks) {
+            if (B.task_groups)
+                g = B.task_groups[t * Wd::GPW + half];
+            else {
+                g = t * Wd::GPW + half;
+                if (g >= K.ngroups) g = -1;
+            }
+        } else {
+            /* Out of tasks: take a group left pending by some wave, in the
+             * first half-wave, and try again now that more groups are final
+             * (one pass; what stays pending goes to the resolution after the
+             * launch, as do groups listed after this wave has left). */
+            if (BITS != 16 || !B.pending) break;
+            int k = 0;
+            if (lane == 0) k = atomicAdd(B.pending + 1, 1);
+            k = __shfl(k, 0, 64);
+            if (k >= dld<true>(B.pending)) break;
+            /* (a slot whose group is not visible yet still reads -1) */
+            const int pg = dld<true>(B.pending + 2 + k);
+            if (pg < 0 || pg >= K.ngroups) continue;
+            g = half == 0 ? pg : -1;
+            run = false;
         }
-        /* one call site of run_group (a second would be a second inlined
-         * copy): a group settled as "run again" comes round once more */
+        /* One call site of run_group (a second would be a second inlined
+         * copy): a group settled as "run again" comes round once more.  The
+         * pass number and `run` are the same in every lane. */
         int gg = g;
         int64_t sw = -1;
-        for (bool again = false;; again = true) {
-            run_group<BITS>(K, B, gg, S, qb, sw);
+        bool do_run = run;
+        for (int pass = 0;; pass++) {
+            if (do_run) run_group<BITS>(K, B, gg, S, qb, sw);
             if (BITS != 16 || !B.gstate) break;
             const int gl = lane & 31;
-            if (again) { /* the reruns, with their true masks, are final */
+            if (pass == 1) { /* the reruns, with their true masks, are final */
                 asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
                 if (gl == 0 && gg >= 0) dst<true>(B.gstate + gg, 1);
                 break;
@@ -1234,6 +1257,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bs
             uint32_t truth;
             const int verdict = settle16(B, g, truth);
             if (gl == 0 && g >= 0 && verdict == 0) dst<true>(B.gstate + g, 1);
+            if (run && B.pending && gl == 0 && g >= 0 && verdict == 2)
+                dst<true>(B.pending + 2 + atomicAdd(B.pending, 1), g);
             if (ballot(verdict == 1) == 0) break;
             if (gl == 0 && verdict == 1) {
                 dst<true>(B.stale_in + g, truth);
@@ -1241,6 +1266,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void bs
             }
             gg = verdict == 1 ? g : -1;
             sw = verdict == 1 ? (int64_t)truth : -1;
+            do_run = true;
         }
     }
 }

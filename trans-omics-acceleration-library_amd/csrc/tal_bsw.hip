@@ -128,7 +128,7 @@ struct Arena {
 /* Per-width scratch of one call. */
 struct PathBufs {
     DevBuf shapes, ncp1, foff, dep, wrote, stale, fmask, rerun, tasks, counter, slabs, last,
-        cubtmp, summary, ckey, ckey2, gids, order, gstate;
+        cubtmp, summary, ckey, ckey2, gids, order, gstate, pending;
 };
 
 } // namespace
@@ -490,6 +490,10 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
         if (!seg_first) { /* one call: groups are settled inside the launch too */
             TAKE(h, pb.gstate, sizeof(int32_t) * G, "alloc group states", &B.gstate);
             HIPCHK(h, hipMemsetAsync(B.gstate, 0, sizeof(int32_t) * G, s), "clear group states");
+            TAKE(h, pb.pending, sizeof(int32_t) * (G + 2), "alloc pending list", &B.pending);
+            HIPCHK(h, hipMemsetAsync(B.pending, 0, sizeof(int32_t) * 2, s), "clear pending");
+            HIPCHK(h, hipMemsetAsync(B.pending + 2, 0xFF, sizeof(int32_t) * G, s),
+                   "clear pending slots");
             B.shapes = shapes;
             B.image = fresh ? h->zero_image.as<uint32_t>() : h->image.as<uint32_t>();
         }
@@ -539,6 +543,7 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
             K2.ntasks = rt;
             KBuffers B2 = B;
             B2.task_groups = tasks;
+            B2.pending = nullptr;
             HIPCHK(h, hipMemsetAsync(B.task_counter, 0, sizeof(int32_t), s), "clear counter");
             const int g2 = grid_for(h, lds, lds_bytes, rt);
             HIPCHK(h, hipEventRecord(h->ev[4], s), "event");
@@ -554,6 +559,7 @@ int run_path(tal_bsw *h, int bits, tal_seqpair *pairs, const uint8_t *ref, const
         h->st.ms_rerun16 = ms_rerun;
         h->st.groups16 = G;
         h->st.reruns16 = reruns + (B.gstate ? early : 0);
+        h->st.settled16 = B.gstate ? early : 0;
         h->st.lds_state16 = lds_cols >= ((state_cols + 7) & ~7) ? 1 : 0;
         if (!fresh) {
             HIPCHK(h, hipMemsetAsync(last, 0xFF, sizeof(int32_t) * IMAGE_COLS, s), "clear last");
@@ -697,6 +703,7 @@ int plan_width(tal_bsw *h, tal_bsw_plan *P, int bits, hipStream_t s)
         HIPCHK(h, pb.tasks.ensure(sizeof(int32_t) * (size_t)(G + 2) * 2), "alloc tasks");
         HIPCHK(h, pb.last.ensure(sizeof(int32_t) * IMAGE_COLS), "alloc last");
         HIPCHK(h, pb.gstate.ensure(sizeof(int32_t) * G), "alloc group states");
+        HIPCHK(h, pb.pending.ensure(sizeof(int32_t) * (G + 2)), "alloc pending list");
         hipLaunchKernelGGL(stride_offsets, dim3((G + 255) / 256), dim3(256), 0, s,
                            pb.foff.as<int64_t>(), G, stride);
         HIPCHK(h, hipGetLastError(), "offsets");
@@ -709,6 +716,7 @@ int plan_width(tal_bsw *h, tal_bsw_plan *P, int bits, hipStream_t s)
         B.image = h->image.as<uint32_t>();
         B.gstate = pb.gstate.as<int32_t>();
         B.early = P->ctl.as<int32_t>() + 6;
+        B.pending = pb.pending.as<int32_t>();
     }
     return TAL_OK;
 }
@@ -745,6 +753,9 @@ int plan_enqueue(tal_bsw *h, tal_bsw_plan *P, hipStream_t s)
         hipLaunchKernelGGL(fill_i32, dim3((G + 255) / 256), dim3(256), 0, s,
                            (int32_t *)W.pb.stale.p, G, 0);
         hipLaunchKernelGGL(fill_i32, dim3((G + 255) / 256), dim3(256), 0, s, W.B.gstate, G, 0);
+        hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, s, W.B.pending, 2, 0);
+        hipLaunchKernelGGL(fill_i32, dim3((G + 255) / 256), dim3(256), 0, s, W.B.pending + 2, G,
+                           -1);
         hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, s, W.B.task_counter, 2, 0);
         HIPCHK(h, launch_main(16, true, W.grid, W.z.lds_bytes, W.K, W.B, s), "main16");
     }
@@ -779,6 +790,7 @@ int plan_enqueue(tal_bsw *h, tal_bsw_plan *P, hipStream_t s)
             KBuffers B2 = W.B;
             B2.task_groups = W.pb.tasks.as<int32_t>();
             B2.ntasks_dev = ctl + 2;
+            B2.pending = nullptr;
             HIPCHK(h, launch_main(16, true, W.rgrid, W.z.lds_bytes, W.K, B2, s), "rerun16");
             HIPCHK(h, launch_resolve(G, shapes, W.B.dep_out, W.B.wrote_out, W.B.fmask,
                                      W.B.fmask_off, h->image.as<uint32_t>(), nullptr,
@@ -821,6 +833,7 @@ int plan_finish(tal_bsw *h, tal_bsw_plan *P, hipStream_t s, int32_t pending)
         K2.ntasks = rt;
         KBuffers B2 = W.B;
         B2.task_groups = W.pb.tasks.as<int32_t>();
+        B2.pending = nullptr;
         HIPCHK(h, hipMemsetAsync(W.B.task_counter, 0, sizeof(int32_t), s), "clear counter");
         HIPCHK(h, launch_main(16, true, std::min(W.grid, grid_for(h, true, W.z.lds_bytes, rt)),
                               W.z.lds_bytes, K2, B2, s),
@@ -1332,6 +1345,7 @@ int tal_bsw_plan_launch(tal_bsw_plan *P, void *stream)
     h->st.groups8 = P->p8.G;
     h->st.groups16 = P->p16.G;
     h->st.reruns16 = ctl[5] + ctl[6];
+    h->st.settled16 = ctl[6];
     if (ctl[0] != 0)
         return fail(h, TAL_ERANGE, "the batch no longer fits its plan (dispatcher split or "
                                    "group columns changed); build a new plan");

@@ -48,7 +48,8 @@ class Stats(ctypes.Structure):
     _fields_ = [("ms_main16", ctypes.c_float), ("ms_main8", ctypes.c_float),
                 ("ms_rerun16", ctypes.c_float)] + [
         (n, ctypes.c_int32) for n in ("groups16", "groups8", "reruns16", "lds_state16",
-                                      "pairs8", "pairs16")] + [("ms_replay", ctypes.c_float)]
+                                      "pairs8", "pairs16")] + [("ms_replay", ctypes.c_float),
+                                                               ("settled16", ctypes.c_int32)]
 
 
 class Tuning(ctypes.Structure):
