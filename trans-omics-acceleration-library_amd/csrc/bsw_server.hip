@@ -116,6 +116,7 @@ struct Request {
     int32_t n = 0, w = 0;
     /* prepared */
     std::vector<int32_t> idx8, idx16; /* caller positions of the 8/16-bit pairs */
+    std::vector<tal_seqpair> hb8, hb16; /* host-batched mode: the padded blocks, built at admission */
     int32_t n8p = 0, n16p = 0;        /* padded counts */
     size_t off = 0, bytes = 0;        /* arena slot */
     size_t ref_off = 0, qer_off = 0, p8_off = 0, p16_off = 0; /* absolute arena offsets */
@@ -260,7 +261,6 @@ struct tal_bsw_server {
     int32_t max_batch = 0;
     uint8_t *arena = nullptr;
     Arena alloc;
-    hipStream_t copy_stream = nullptr;
     std::vector<tal_bsw *> engines;
     std::mutex mu;
     std::condition_variable cv; /* every state change */
@@ -294,10 +294,9 @@ int prepare(tal_bsw_server *s, Request *r)
     const int32_t n8 = (int32_t)r->idx8.size(), n16 = (int32_t)r->idx16.size();
     r->n8p = (n8 + 63) / 64 * 64;
     r->n16p = (n16 + 31) / 32 * 32;
-    const size_t seq = align_up(std::max<size_t>(r->ref_len, 1)) +
-                       align_up(std::max<size_t>(r->qer_len, 1));
-    r->bytes = seq + align_up(sizeof(tal_seqpair) * (size_t)r->n8p) +
-               align_up(sizeof(tal_seqpair) * (size_t)r->n16p);
+    /* host-batched footprint: the sequences (a batch's blocks live in its
+     * worker's buffers; resident mode sizes its own layout) */
+    r->bytes = align_up(std::max<size_t>(r->ref_len, 1)) + align_up(std::max<size_t>(r->qer_len, 1));
     return TAL_OK;
 }
 
@@ -340,10 +339,14 @@ void finish(tal_bsw_server *s, Request *r, int status)
     }
 }
 
+/* Admission of the host-batched mode: arrival order, room in the arena, the
+ * padded blocks built on the host.  The copies to the device are left to the
+ * worker that runs the request, on its own stream (a copy stream of its own
+ * here could share a hardware queue with a worker's stream, and each
+ * admission would then wait for a whole batch). */
 void admit_loop(tal_bsw_server *s)
 {
     (void)hipSetDevice(s->device);
-    std::vector<tal_seqpair> b8, b16;
     for (;;) {
         Request *r = nullptr;
         {
@@ -370,30 +373,11 @@ void admit_loop(tal_bsw_server *s)
         }
         r->ref_off = r->off;
         r->qer_off = r->ref_off + align_up(std::max<size_t>(r->ref_len, 1));
-        r->p8_off = r->qer_off + align_up(std::max<size_t>(r->qer_len, 1));
-        r->p16_off = r->p8_off + align_up(sizeof(tal_seqpair) * (size_t)r->n8p);
-        if (r->n8p) build_block(r, r->idx8, r->n8p, true, b8);
-        if (r->n16p) build_block(r, r->idx16, r->n16p, false, b16);
-        hipError_t e = hipSuccess;
-        hipStream_t cs = s->copy_stream;
-        if (r->ref_len)
-            e = hipMemcpyAsync(s->arena + r->ref_off, r->ref, r->ref_len, hipMemcpyHostToDevice, cs);
-        if (e == hipSuccess && r->qer_len)
-            e = hipMemcpyAsync(s->arena + r->qer_off, r->qer, r->qer_len, hipMemcpyHostToDevice, cs);
-        if (e == hipSuccess && r->n8p)
-            e = hipMemcpyAsync(s->arena + r->p8_off, b8.data(), sizeof(tal_seqpair) * r->n8p,
-                               hipMemcpyHostToDevice, cs);
-        if (e == hipSuccess && r->n16p)
-            e = hipMemcpyAsync(s->arena + r->p16_off, b16.data(), sizeof(tal_seqpair) * r->n16p,
-                               hipMemcpyHostToDevice, cs);
-        if (e == hipSuccess) e = hipStreamSynchronize(cs);
+        if (r->n8p) build_block(r, r->idx8, r->n8p, true, r->hb8);
+        if (r->n16p) build_block(r, r->idx16, r->n16p, false, r->hb16);
         std::unique_lock<std::mutex> lk(s->mu);
-        if (e != hipSuccess) {
-            finish(s, r, TAL_EDEVICE);
-        } else {
-            r->t_admit = now_ms();
-            s->ready.push_back(r);
-        }
+        r->t_admit = now_ms();
+        s->ready.push_back(r);
         s->admitting = false;
         s->cv.notify_all();
     }
@@ -445,14 +429,22 @@ int run_one(tal_bsw_server *s, tal_bsw *eng, WorkerBufs &wb, std::vector<Request
         return TAL_ENOMEM;
     wb.hseg.clear();
     int64_t o8 = 0, o16 = 0;
+    /* each request's sequences into its arena slot and its blocks into the
+     * batch, on this worker's stream, ahead of its kernels */
     for (Request *r : batch) {
+        if (r->ref_len && hipMemcpyAsync(s->arena + r->ref_off, r->ref, r->ref_len,
+                                         hipMemcpyHostToDevice, st) != hipSuccess)
+            return TAL_EDEVICE;
+        if (r->qer_len && hipMemcpyAsync(s->arena + r->qer_off, r->qer, r->qer_len,
+                                         hipMemcpyHostToDevice, st) != hipSuccess)
+            return TAL_EDEVICE;
         if (r->n8p &&
-            hipMemcpyAsync(wb.p8 + o8, s->arena + r->p8_off, sizeof(tal_seqpair) * r->n8p,
-                           hipMemcpyDeviceToDevice, st) != hipSuccess)
+            hipMemcpyAsync(wb.p8 + o8, r->hb8.data(), sizeof(tal_seqpair) * r->n8p,
+                           hipMemcpyHostToDevice, st) != hipSuccess)
             return TAL_EDEVICE;
         if (r->n16p &&
-            hipMemcpyAsync(wb.p16 + o16, s->arena + r->p16_off, sizeof(tal_seqpair) * r->n16p,
-                           hipMemcpyDeviceToDevice, st) != hipSuccess)
+            hipMemcpyAsync(wb.p16 + o16, r->hb16.data(), sizeof(tal_seqpair) * r->n16p,
+                           hipMemcpyHostToDevice, st) != hipSuccess)
             return TAL_EDEVICE;
         const int32_t first = (int32_t)(o16 / 32);
         for (int32_t g = 0; g < r->n16p / 32; g++) wb.hseg.push_back(first);
@@ -533,6 +525,10 @@ void work_loop(tal_bsw_server *s, int k)
         const double t0 = now_ms();
         const int rc = run_one(s, eng, wb, batch);
         const double t1 = now_ms();
+        for (Request *r : batch) { /* (the blocks are on the device or no longer needed) */
+            std::vector<tal_seqpair>().swap(r->hb8);
+            std::vector<tal_seqpair>().swap(r->hb16);
+        }
         std::unique_lock<std::mutex> lk(s->mu);
         s->active_batches--;
         s->st.batches++;
@@ -1043,12 +1039,9 @@ int tal_bsw_server_create(const tal_bsw_params *params, int device,
         s->arena = nullptr;
         rc = TAL_ENOMEM;
     }
-    if (rc == TAL_OK && hipStreamCreateWithFlags(&s->copy_stream, hipStreamNonBlocking) != hipSuccess)
-        rc = TAL_EDEVICE;
     if (rc != TAL_OK) {
         for (tal_bsw *e : s->engines) tal_bsw_destroy(e);
         res_release(s);
-        if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
         if (s->arena) (void)hipFree(s->arena);
         delete s;
         return rc;
@@ -1150,7 +1143,6 @@ void tal_bsw_server_destroy(tal_bsw_server *s)
     for (auto &kv : s->all) delete kv.second;
     for (tal_bsw *e : s->engines) tal_bsw_destroy(e);
     (void)hipSetDevice(s->device);
-    if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
     if (s->arena) (void)hipFree(s->arena);
     delete s;
 }
