@@ -566,14 +566,23 @@ def _release(W, torch):
 
 
 def _tune(W, sample):
-    t = time.time()
+    """Autotunes each band class on its own pairs (a band class is its own
+    getScores call, and the best waves per CU depend on the band width);
+    returns the largest class's choice, with every class's under per_class."""
+    per_class = {}
     big = max(W["classes"], key=lambda c: len(c["pairs"]))
-    tuning = big["eng"].autotune(*big["ptrs"], len(big["pairs"]), big["w"], sample=sample)
+    tuning = None
     for c in W["classes"]:
-        c["eng"].set_tuning(tuning["waves_per_cu"], tuning["order"])
-    log(f"[bench] autotune on {min(len(big['pairs']), sample)} pairs in {time.time() - t:.1f}s: "
-        f"waves/CU {tuning['waves_per_cu']}, order {tuning['order']}; "
-        + ", ".join(f"{x['waves_per_cu']}/{x['order']}:{x['ms']:.2f}" for x in tuning["trials"]))
+        t = time.time()
+        tc = c["eng"].autotune(*c["ptrs"], len(c["pairs"]), c["w"], sample=sample)
+        log(f"[bench] autotune w={c['w']} on {min(len(c['pairs']), sample)} pairs in "
+            f"{time.time() - t:.1f}s: waves/CU {tc['waves_per_cu']}, order {tc['order']}; "
+            + ", ".join(f"{x['waves_per_cu']}/{x['order']}:{x['ms']:.2f}" for x in tc["trials"]))
+        per_class[str(c["w"])] = {"waves_per_cu": tc["waves_per_cu"], "order": tc["order"]}
+        if c is big:
+            tuning = tc
+    if len(per_class) > 1:
+        tuning = {**tuning, "per_class": per_class}
     return tuning
 
 
@@ -643,7 +652,8 @@ def _workload_summary(cfg, args, torch, dev, stream):
            "gcups_rect": round(W["total_rect"] / (ms / 1e3) / 1e9, 2),
            "pairs_per_s": round(W["n"] / (ms / 1e3), 1),
            "pairs_8bit": int(W["is8"].sum()),
-           "tuning": {k: tuning[k] for k in ("waves_per_cu", "order")} if tuning else None}
+           "tuning": {k: tuning[k] for k in ("waves_per_cu", "order", "per_class") if k in tuning}
+           if tuning else None}
     if not args.no_naive:
         naive = _naive_all(W, stream, torch, args.bucket)
         out["naive_gcups"] = naive["gcups"]
@@ -799,7 +809,8 @@ def _batch(args):
         "parity": check,
         "pipeline": "eager dispatcher" if args.eager else "hipGraph plan replay",
         "eager_ms_per_step": round(eager_ms, 3),
-        "tuning": {k: tuning[k] for k in ("waves_per_cu", "order", "ms")} if tuning else None,
+        "tuning": {k: tuning[k] for k in ("waves_per_cu", "order", "ms", "per_class") if k in tuning}
+        if tuning else None,
         "naive_gpu": naive,
         "plain_gpu": plain,
         "end_to_end": e2e,
