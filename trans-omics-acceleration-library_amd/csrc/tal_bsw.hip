@@ -739,7 +739,7 @@ int plan_enqueue(tal_bsw *h, tal_bsw_plan *P, hipStream_t s)
         HIPCHK(h, launch_gather(P->pairs, P->sel16.as<int32_t>(), P->n16,
                                 P->pairs16.as<tal_seqpair>(), s),
                "gather16");
-    /* 16-bit main launch first; the 8-bit branch then runs beside the
+    /* the 16-bit main launch, with the 8-bit branch beside it and the
      * 16-bit resolution and rerun rounds (they share no data) */
     WidthPlan &W = P->p16;
     const int32_t G = W.G;
@@ -757,14 +757,18 @@ int plan_enqueue(tal_bsw *h, tal_bsw_plan *P, hipStream_t s)
         hipLaunchKernelGGL(fill_i32, dim3((G + 255) / 256), dim3(256), 0, s, W.B.pending + 2, G,
                            -1);
         hipLaunchKernelGGL(fill_i32, dim3(1), dim3(64), 0, s, W.B.task_counter, 2, 0);
-        HIPCHK(h, launch_main(16, true, W.grid, W.z.lds_bytes, W.K, W.B, s), "main16");
     }
+    /* The 8-bit branch forks just before the 16-bit launch: its blocks take
+     * the slots the 16-bit launch's waves free as its last tasks end (most
+     * stale-column groups are settled inside that launch, so the rounds
+     * after it are short and would leave the branch exposed). */
     hipStream_t s8 = s;
     if (P->n8 && P->n16 && P->s8) {
         HIPCHK(h, hipEventRecord(P->fork, s), "fork");
         HIPCHK(h, hipStreamWaitEvent(P->s8, P->fork, 0), "fork wait");
         s8 = P->s8;
     }
+    if (P->n16) HIPCHK(h, launch_main(16, true, W.grid, W.z.lds_bytes, W.K, W.B, s), "main16");
     if (P->n8) {
         WidthPlan &W8 = P->p8;
         HIPCHK(h, launch_plan_pad8(P->pairs8.as<tal_seqpair>(), P->n8, ctl + 1, ctl, s8), "pad8");
