@@ -670,12 +670,17 @@ def _workload_summary(cfg, args, torch, dev, stream):
 
 def _end_to_end(W, stream, torch, reps=3):
     """The whole batch from pinned host buffers: records and sequences in,
-    the plan's replay, records out; every step moves all of them over PCIe."""
+    the plan's replay, records out; every step moves all of them over PCIe.
+    Measured two ways: one step after another, and pipelined (two device
+    buffer sets with a plan each; step k+1's inputs go up and step k-1's
+    outputs come down while step k's replay runs; the replays stay in order
+    on one stream, so the handle's state evolves as in the serial form)."""
     cs = W["classes"]
     h_pairs = [torch.from_numpy(c["pairs"].view(np.uint8)).pin_memory() for c in cs]
     h_ref = torch.from_numpy(W["ref"]).pin_memory()
     h_qer = torch.from_numpy(W["qer"]).pin_memory()
-    h_out = [torch.empty(h.shape, dtype=h.dtype, pin_memory=True) for h in h_pairs]
+    h_out = [[torch.empty(h.shape, dtype=h.dtype, pin_memory=True) for h in h_pairs]
+             for _ in range(2)]
 
     # one explicit stream for the copies and the replays, so that each replay
     # runs after its inputs arrive and before its outputs leave (torch's
@@ -687,7 +692,7 @@ def _end_to_end(W, stream, torch, reps=3):
         with torch.cuda.stream(s):
             W["d_ref"].copy_(h_ref, non_blocking=True)
             W["d_qer"].copy_(h_qer, non_blocking=True)
-            for c, hp, ho in zip(cs, h_pairs, h_out):
+            for c, hp, ho in zip(cs, h_pairs, h_out[0]):
                 c["d_pairs"].copy_(hp, non_blocking=True)
                 c["plan"].launch(s.cuda_stream)
                 ho.copy_(c["d_pairs"], non_blocking=True)
@@ -700,9 +705,67 @@ def _end_to_end(W, stream, torch, reps=3):
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / reps
     moved = sum(h.numel() for h in h_pairs) * 2 + h_ref.numel() + h_qer.numel()
+
+    # pipelined: a second device buffer set with its own plans (a plan sizes
+    # itself from its batch, so the set holds the batch first)
+    d_ref2, d_qer2 = W["d_ref"].clone(), W["d_qer"].clone()
+    d_pairs2 = [c["d_pairs"].clone() for c in cs]
+    torch.cuda.synchronize()
+    plans2 = [c["eng"].plan(dp.data_ptr(), d_ref2.data_ptr(), d_qer2.data_ptr(), len(c["pairs"]),
+                            c["w"]) for c, dp in zip(cs, d_pairs2)]
+    sets = [(W["d_ref"], W["d_qer"], [c["d_pairs"] for c in cs], [c["plan"] for c in cs]),
+            (d_ref2, d_qer2, d_pairs2, plans2)]
+    up, run, down = torch.cuda.Stream(), torch.cuda.Stream(), torch.cuda.Stream()
+    freed = [None, None]  # a set's last download (its buffers may be refilled after it)
+    arrived = [None, None]
+
+    def upload(k):
+        ref, qer, dps, _ = sets[k % 2]
+        with torch.cuda.stream(up):
+            if freed[k % 2] is not None:
+                up.wait_event(freed[k % 2])
+            ref.copy_(h_ref, non_blocking=True)
+            qer.copy_(h_qer, non_blocking=True)
+            for dp, hp in zip(dps, h_pairs):
+                dp.copy_(hp, non_blocking=True)
+            arrived[k % 2] = up.record_event()
+
+    def replay_and_download(k):
+        _, _, dps, pls = sets[k % 2]
+        run.wait_event(arrived[k % 2])
+        for pl in pls:  # (each launch waits for its replay: the next upload is queued first)
+            pl.launch(run.cuda_stream)
+        done = run.record_event()
+        with torch.cuda.stream(down):
+            down.wait_event(done)
+            for ho, dp in zip(h_out[k % 2], dps):
+                ho.copy_(dp, non_blocking=True)
+            freed[k % 2] = down.record_event()
+
+    def pipeline(n_steps):
+        upload(0)
+        for k in range(n_steps):
+            if k + 1 < n_steps:
+                upload(k + 1)
+            replay_and_download(k)
+
+    pipeline(2)
+    torch.cuda.synchronize()
+    n_steps = 2 * reps
+    t0 = time.perf_counter()
+    pipeline(n_steps)
+    torch.cuda.synchronize()
+    dtp = (time.perf_counter() - t0) / n_steps
+    for pl in plans2:
+        pl.close()
     return {"form": "pinned host records and sequences in, plan replay, records out, per step",
             "ms_per_step": round(dt * 1e3, 2), "gcups": round(W["total_cells"] / dt / 1e9, 2),
-            "pairs_per_s": round(W["n"] / dt, 1), "pcie_bytes_per_step": int(moved)}
+            "pairs_per_s": round(W["n"] / dt, 1), "pcie_bytes_per_step": int(moved),
+            "pipelined": {"form": "the same steps, step k+1's inputs and step k-1's outputs moving "
+                                  "while step k's replay runs (two device buffer sets)",
+                          "ms_per_step": round(dtp * 1e3, 2),
+                          "gcups": round(W["total_cells"] / dtp / 1e9, 2),
+                          "pairs_per_s": round(W["n"] / dtp, 1)}}
 
 
 def _batch(args):
