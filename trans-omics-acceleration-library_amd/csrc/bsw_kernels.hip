@@ -646,6 +646,9 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
             const int sthi = max(tl, nzhi);
             /* the row's blocks: those in the LDS window, then the global ones */
             uint32_t pr[8];
+            /* the stores of the last edge block swept (zeros outside [beg,
+             * end)); with pr, the row's last block stays in registers */
+            uint32_t ev[8];
             int pb = -1;
             bool pneed = false;
             auto sweep = [&](auto gtag, int from, int to) {
@@ -740,7 +743,8 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                          * once.  A lane that skipped loading a global block
                          * has it outside [lo, tl] and stores masked zeros. */
 #pragma unroll
-                        for (int k = 0; k < 8; k++)
+                        for (int k = 0; k < 8; k++) {
+                            ev[k] = 0u;
                             if (b + k >= beg && b + k < end) {
                                 const uint32_t st =
                                     cell(sv[k], score((k & 1) ? sco : sce, k >> 1));
@@ -749,9 +753,11 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                                     bst32(S.slab, S.goff(b) + 4u * k, r);
                                 else
                                     lp[k] = r;
+                                ev[k] = r;
                                 any |= r;
                                 if (b + k > beg) kb = max(kb, (r << 16) | (uint32_t)k);
                             }
+                        }
                     }
                     mkey = max(mkey, kb + (uint32_t)b);
                     if (any != 0u) {
@@ -866,7 +872,16 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
             if (bfirst != NONE_COL) {
                 uint32_t vf[8], vl[8];
                 S.ld8any(bfirst, vf);
-                S.ld8any(blast, vl);
+                /* the row's last block is still in registers: pr if it was
+                 * a middle block, ev if an edge block */
+                const int bl = (end - 1) & ~7;
+                const bool lmid = bl > beg && bl + 8 <= end;
+                if (blast == bl) {
+#pragma unroll
+                    for (int k = 0; k < 8; k++) vl[k] = lmid ? pr[k] : ev[k];
+                } else {
+                    S.ld8any(blast, vl);
+                }
                 uint32_t nzf = 0, nzl = 0;
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
@@ -946,15 +961,20 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
         /* gscore (:3176-3195): the cell of the last query column is updated
          * when it was computed this row, the lane is live and tail == qlen;
          * then H_h[qlen] holds it (head <= qlen, or the lane would not be
-         * live, so the store at column qlen was not masked). */
-        if (gq) {
-            const int16_t hq = (int16_t)(vq & 0xFFFFu);
-            if (!(gscore > hq)) max_ie = i1;
-            gscore = smax16(gscore, hq);
-        }
+         * live, so the store at column qlen was not masked).  Nothing else
+         * in the row reads gscore, so it is folded in at the row's end, after
+         * the load has had the reductions to arrive. */
+        auto fold_gscore = [&]() {
+            if (gq) {
+                const int16_t hq = (int16_t)(vq & 0xFFFFu);
+                if (!(gscore > hq)) max_ie = i1;
+                gscore = smax16(gscore, hq);
+            }
+        };
 
         /* all lanes zero in this row: the group stops (:3205-3209) */
         if ((ballot(maxRS == 0) & gmask) == gmask) {
+            fold_gscore();
             gdone = true;
             continue;
         }
@@ -1081,6 +1101,7 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
             const int16_t idx = (live && es >= bs && lzs < es) ? Wd::v(lzs) : tail;
             tail = smin16(add16(idx, TWO), qlen);
         }
+        fold_gscore();
     }
 
     if (real) { /* outputs (:3327-3353) */
