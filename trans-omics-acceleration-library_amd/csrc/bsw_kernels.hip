@@ -716,17 +716,19 @@ __device__ void run_group(const KParams &Kref, const KBuffers &B, int g, StateRe
                             const uint16_t e0 = (uint16_t)e;
                             const uint16_t e1 = (uint16_t)smax16(am.x, (int16_t)__builtin_elementwise_sub_sat(e0, (uint16_t)e_ins_s));
                             const uint16_t e2 = (uint16_t)smax16(am.y, (int16_t)__builtin_elementwise_sub_sat(e1, (uint16_t)e_ins_s));
-                            const s2v E2 = as_s2((uint32_t)e0 | ((uint32_t)e1 << 16));
+                            const s2v E2 = as_s2(__builtin_amdgcn_perm((uint32_t)e1, (uint32_t)e, 0x05040100u));
                             const uint32_t h2 = as_u32(max2(max2(m, as_s2(u2_u32(F))), E2));
-                            const uint32_t st0 = __builtin_amdgcn_perm(as_u32(fn), hp, 0x05040100u);
+                            /* H of column k-1: the low half of hp entering the
+                             * block, then the high half of the previous h2 */
+                            const uint32_t st0 = __builtin_amdgcn_perm(as_u32(fn), hp, q == 0 ? 0x05040100u : 0x05040302u);
                             const uint32_t st1 = __builtin_amdgcn_perm(as_u32(fn), h2, 0x07060100u);
                             pr[k] = k <= tb ? st0 : 0u;
                             pr[k + 1] = (q < 3 ? k + 1 <= tb : rel0 + 7 < lim1) ? st1 : 0u;
-                            hp = h2 >> 16;
+                            hp = h2;
                             e = e2;
                         }
                         ez = e;
-                        hprev = hp;
+                        hprev = hp >> 16;
                         /* row-max key (store << 16 | column) and nonzero
                          * flag of the block, three operands at a time */
                         uint32_t kk[8];
