@@ -115,7 +115,7 @@ def test_autotune_configurations_agree():
     eng = T.BandedPairWiseSW()
     try:
         res = eng.autotune(da.ptr(0), da.ptr(1), da.ptr(2), len(pairs), 100, sample=8000)
-        assert len(res["trials"]) == 12 and res["ms"] > 0
+        assert len(res["trials"]) == 16 and res["ms"] > 0
         assert (res["waves_per_cu"], res["order"]) in {(t["waves_per_cu"], t["order"])
                                                         for t in res["trials"]}
         # the batch itself is untouched by the tuning runs

@@ -1410,7 +1410,10 @@ int tal_bsw_autotune(tal_bsw *h, const tal_seqpair *pairs, const uint8_t *ref,
                              hipMemcpyDeviceToDevice, t->stream),
            "copy batch");
     HIPCHK(h, hipStreamSynchronize(t->stream), "sync copy");
-    static const int waves[] = {4, 6, 8, 10, 12, 16};
+    /* every distinct LDS window from 160 to 40 columns but 128 (5 waves per
+     * CU, never the fastest): 9 and 10 waves give the same window, as do 12
+     * and 13, 14 and 16 */
+    static const int waves[] = {4, 6, 7, 8, 10, 11, 12, 16};
     tal_bsw_tuning res{};
     res.ms = -1.f;
     for (int order = 0; order < 2; order++)
