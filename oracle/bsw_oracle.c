@@ -28,6 +28,17 @@
 
 #define W16 32
 #define W8 64
+
+/* Observation points of the 16-bit group loop, for the row-pair model
+ * (rowpair_model.c defines them before including this file); empty here. */
+#ifndef OBS_GROUP
+#define OBS_GROUP()
+#define OBS_ROW(i)
+#define OBS_STORE(i, j, l, hst, fst, h, e)
+#define OBS_COLDONE(i, j)
+#define OBS_STOP(i)
+#define OBS_ROWEND(i)
+#endif
 #define MAXLEN16 32768 /* MAX_SEQ_LEN16, bandedSWA.h:83 */
 #define MAXLEN8 128    /* MAX_SEQ_LEN8,  bandedSWA.h:82 */
 #define CODE_N 4       /* AMBIG, bandedSWA.cpp:44 */
@@ -233,6 +244,7 @@ static void dp_group16(bsw_oracle *o, uint16_t nrow, uint16_t ncol,
     int fcol_written = 0; /* has this group stored F[ncol] yet? */
 
     int beg = 0, end = ncol, nbeg = 0, nend = ncol;
+    OBS_GROUP();
     for (int16_t i = 0; i < (int)nrow; i++) {
         /* column range of this row, :3059-3063 */
         beg = nbeg;
@@ -274,6 +286,7 @@ static void dp_group16(bsw_oracle *o, uint16_t nrow, uint16_t ncol,
         for (int l = 0; l < W16; l++)
             if (i1 > mlen[l] || tail[l] == head[l] || head[l] > tail[l])
                 exit0[l] = 0;
+        OBS_ROW(i);
 
         /* the column loop, :3134-3196 */
         int16_t jj = w16(beg); /* j512: column index + 1 after the bump */
@@ -319,6 +332,7 @@ static void dp_group16(bsw_oracle *o, uint16_t nrow, uint16_t ncol,
                 }
                 F[cj + l] = f21;
                 Hh[cj + l] = hst;
+                OBS_STORE(i, j, l, hst, f21, h11, e11[l]);
                 h10[l] = h11;
                 /* global score at the last query column, :3176-3195 */
                 if (j >= minq) {
@@ -331,6 +345,7 @@ static void dp_group16(bsw_oracle *o, uint16_t nrow, uint16_t ncol,
                     }
                 }
             }
+            OBS_COLDONE(i, j);
         }
         /* boundary store after the loop, at j (== end, or beg if the loop
          * did not run), :3197-3203 */
@@ -338,6 +353,7 @@ static void dp_group16(bsw_oracle *o, uint16_t nrow, uint16_t ncol,
             const int out1 = (jj > tail[l]) || (head[l] > jj);
             Hh[(size_t)j * W16 + l] = out1 ? 0 : h10[l];
             F[(size_t)j * W16 + l] = 0;
+            OBS_STORE(i, j, l, Hh[(size_t)j * W16 + l], 0, 0, 0);
         }
         if (j == ncol) fcol_written = 1;
 
@@ -345,7 +361,10 @@ static void dp_group16(bsw_oracle *o, uint16_t nrow, uint16_t ncol,
         uint32_t zrow = 0;
         for (int l = 0; l < W16; l++)
             if (maxRS[l] == 0) zrow |= 1u << l;
-        if (zrow == ALL16) break;
+        if (zrow == ALL16) {
+            OBS_STOP(i);
+            break;
+        }
         for (int l = 0; l < W16; l++) {
             if ((zrow >> l) & 1) exit0[l] = 0;
             const int16_t before = maxScore[l];
@@ -362,6 +381,7 @@ static void dp_group16(bsw_oracle *o, uint16_t nrow, uint16_t ncol,
             const int16_t drop = w16(w16(maxScore[l] - maxRS[l]) - off);
             if (drop > zdrop) exit0[l] = 0;
         }
+        OBS_ROWEND(i);
 
         /* narrow the group's column range, :3239-3260 */
         int c;
