@@ -40,13 +40,13 @@ struct bsw_oracle;
 static void rp_group(int nrow, int ncol, int w, const int16_t *band, const int16_t *qlen);
 static void rp_row(int i, int beg, int end, int nbeg, const int16_t *head, const int16_t *tail,
                    const int16_t *exit0, const int16_t *maxScore, int16_t zdrop,
-                   const int16_t *F, const int16_t *Hh);
+                   const int16_t *F, const int16_t *Hh, const int16_t *x, const int16_t *y);
 static void rp_store(int j, int l, int16_t hst, int16_t fst, int16_t h, int16_t e);
 static void rp_coldone(int i, int j, const int16_t *F, const int16_t *Hh);
 static void rp_stop(void);
 
 #define OBS_GROUP() rp_group(nrow, ncol, w, (const int16_t *)band, qlen)
-#define OBS_ROW(i) rp_row(i, beg, end, nbeg, head, tail, exit0, maxScore, zdrop, F, Hh)
+#define OBS_ROW(i) rp_row(i, beg, end, nbeg, head, tail, exit0, maxScore, zdrop, F, Hh, x, y)
 #define OBS_STORE(i, j, l, hst, fst, h, e) rp_store(j, l, hst, fst, h, e)
 #define OBS_COLDONE(i, j) rp_coldone(i, j, F, Hh)
 #define OBS_STOP(i) rp_stop()
@@ -61,12 +61,13 @@ static int g_nrow, g_ncol, g_w;
 static int16_t g_band[W16], g_qlen[W16];
 /* row i as it started */
 static int r_i = -1, r_beg, r_end, r_nbeg;
-static int16_t r_head[W16], r_tail[W16], r_exit[W16], r_maxs[W16], r_zdrop;
+static int16_t r_head[W16], r_tail[W16], r_exit[W16], r_maxs[W16], r_x[W16], r_y[W16], r_zdrop;
 static const int16_t *r_F, *r_Hh;
 /* what row i has stored so far */
 static int s_c1, s_lmax, s_anyh;
 static int s_fz[W16], s_lz[W16];
 static int16_t s_mrs[W16];
+static int s_y1[W16]; /* latest store column holding s_mrs: the row maximum's column + 1 */
 static int16_t s_h[W16], s_e[W16]; /* H of the last column swept, and E entering the next */
 /* blocks of row i+1 decided early */
 static int p_row = -1;
@@ -99,7 +100,7 @@ static int kept16(int c, int beg, int end, int16_t head, int16_t tail)
 
 static void rp_row(int i, int beg, int end, int nbeg, const int16_t *head, const int16_t *tail,
                    const int16_t *exit0, const int16_t *maxScore, int16_t zdrop,
-                   const int16_t *F, const int16_t *Hh)
+                   const int16_t *F, const int16_t *Hh, const int16_t *x, const int16_t *y)
 {
     /* check the blocks of this row decided during the last one */
     if (p_row == i) {
@@ -131,6 +132,8 @@ static void rp_row(int i, int beg, int end, int nbeg, const int16_t *head, const
     memcpy(r_tail, tail, sizeof r_tail);
     memcpy(r_exit, exit0, sizeof r_exit);
     memcpy(r_maxs, maxScore, sizeof r_maxs);
+    memcpy(r_x, x, sizeof r_x);
+    memcpy(r_y, y, sizeof r_y);
     r_zdrop = zdrop;
     r_F = F;
     r_Hh = Hh;
@@ -141,6 +144,7 @@ static void rp_row(int i, int beg, int end, int nbeg, const int16_t *head, const
         s_fz[l] = RP_NONE;
         s_lz[l] = -1;
         s_mrs[l] = 0;
+        s_y1[l] = 0;
         s_h[l] = 0;
         s_e[l] = 0;
     }
@@ -162,7 +166,10 @@ static void rp_store(int j, int l, int16_t hst, int16_t fst, int16_t h, int16_t 
     if (j > s_lz[l]) s_lz[l] = j;
     if (hst > 0) {
         s_anyh = 1;
-        if (hst > s_mrs[l]) s_mrs[l] = hst;
+        if (hst >= s_mrs[l]) {
+            s_mrs[l] = hst;
+            s_y1[l] = j;
+        }
     }
 }
 
@@ -279,9 +286,29 @@ static void rp_coldone(int i, int j, const int16_t *F, const int16_t *Hh)
                 okL = 0;
             }
         }
+        /* no later store (and no later counted H) of this lane can be nonzero */
+        const int settled = (int)r_tail[l] < P || (s_h[l] == 0 && s_e[l] == 0 && quiet_after(l, P));
         int live; /* 0 not live, 1 live, 2 open */
         if (r_exit[l] == 0)
             live = 0;
+        else if (s_mrs[l] == 0 && settled)
+            live = 0; /* its row maximum stays 0 (:3205-3211) */
+        else if (settled) {
+            /* the row maximum and its column are final: z-drop as at the
+             * row's end (:3212-3234, ZSCORE16 :1868-1881) */
+            const int16_t i1 = w16(i + 1);
+            int16_t ms = r_maxs[l], xx = r_x[l], yy = r_y[l];
+            const int16_t y1 = w16(s_y1[l]);
+            if (s_mrs[l] > ms) {
+                ms = s_mrs[l];
+                xx = i1;
+                yy = y1;
+            }
+            const int16_t di = w16(i1 - xx), dj = w16(y1 - yy);
+            const int16_t off = di > dj ? w16(di - dj) : w16(dj - di);
+            const int16_t drop = w16(w16(ms - s_mrs[l]) - off);
+            live = drop > r_zdrop ? 0 : 1;
+        }
         else if (s_mrs[l] > 0 && (int)r_maxs[l] - (int)s_mrs[l] <= (int)r_zdrop)
             live = 1;
         else
@@ -303,9 +330,15 @@ static void rp_coldone(int i, int j, const int16_t *F, const int16_t *Hh)
             all_unchanged = 0; /* the post-scan tail is not known exactly */
         } else {
             /* both outcomes must give the same masks, heads and tails */
-            if (f == RP_NONE || !okL || mL != mNL || hL != hNL) { t_why[2]++; return; }
+            /* the heads may differ if both lie at or below the block's start:
+             * then neither masks, trims or bounds the trim inside it */
+            if (f == RP_NONE || !okL || mL != mNL ||
+                (hL != hNL && (hL > w16(c0) || hNL > w16(c0)))) {
+                t_why[2]++;
+                return;
+            }
             m = mL;
-            hx[l] = hL;
+            hx[l] = smax16(hL, hNL);
             if (tkL == 0 && tL == tNL) {
                 tx[l] = tL;
                 tkind[l] = 0;
