@@ -76,6 +76,11 @@ static uint32_t p_kept[RP_MAXB][8];
 static int16_t p_F[RP_MAXB][8][W16], p_H[RP_MAXB][8][W16];
 /* totals */
 static uint64_t t_blocks, t_early, t_mismatch, t_rows;
+/* the same two counts over the blocks past an LDS window of rp_window blocks
+ * from the block holding the row's start (the kernel keeps those in global
+ * memory); 0 = no window */
+static int rp_window = 0;
+static uint64_t t_gblocks, t_gearly;
 /* why a block was not decided early: 0 the group may stop, 1 it may hold the
  * row's end, 2 a lane's live/not-live outcomes differ, 3 a lane's tail is open,
  * 4 a trim may reach it */
@@ -106,9 +111,12 @@ static void rp_row(int i, int beg, int end, int nbeg, const int16_t *head, const
     if (p_row == i) {
         t_rows++;
         for (int k = beg >> 3; k * 8 < end && k < RP_MAXB; k++) {
+            const int global = rp_window > 0 && k >= (beg >> 3) + rp_window;
             t_blocks++;
+            t_gblocks += (uint64_t)global;
             if (!p_valid[k]) continue;
             t_early++;
+            t_gearly += (uint64_t)global;
             int bad = 0;
             for (int t = 0; t < 8; t++) {
                 const int c = 8 * k + t;
@@ -411,6 +419,13 @@ void rp_stats(uint64_t out[4])
     out[3] = t_rows;
 }
 
+void rp_global(int window, uint64_t out[2])
+{
+    rp_window = window;
+    out[0] = t_gblocks;
+    out[1] = t_gearly;
+}
+
 void rp_why(uint64_t out[5])
 {
     for (int k = 0; k < 5; k++) out[k] = t_why[k];
@@ -419,5 +434,6 @@ void rp_why(uint64_t out[5])
 void rp_reset(void)
 {
     t_blocks = t_early = t_mismatch = t_rows = 0;
+    t_gblocks = t_gearly = 0;
     for (int k = 0; k < 5; k++) t_why[k] = 0;
 }
