@@ -81,6 +81,9 @@ static uint64_t t_blocks, t_early, t_mismatch, t_rows;
  * memory); 0 = no window */
 static int rp_window = 0;
 static uint64_t t_gblocks, t_gearly;
+/* early blocks in an unbroken run from the row's first block: the next row's
+ * E and H pass from block to block, so only those can run early */
+static uint64_t t_prefix, t_gprefix;
 /* why a block was not decided early: 0 the group may stop, 1 it may hold the
  * row's end, 2 a lane's live/not-live outcomes differ, 3 a lane's tail is open,
  * 4 a trim may reach it */
@@ -110,13 +113,21 @@ static void rp_row(int i, int beg, int end, int nbeg, const int16_t *head, const
     /* check the blocks of this row decided during the last one */
     if (p_row == i) {
         t_rows++;
+        int run = 1;
         for (int k = beg >> 3; k * 8 < end && k < RP_MAXB; k++) {
             const int global = rp_window > 0 && k >= (beg >> 3) + rp_window;
             t_blocks++;
             t_gblocks += (uint64_t)global;
-            if (!p_valid[k]) continue;
+            if (!p_valid[k]) {
+                run = 0;
+                continue;
+            }
             t_early++;
             t_gearly += (uint64_t)global;
+            if (run) {
+                t_prefix++;
+                t_gprefix += (uint64_t)global;
+            }
             int bad = 0;
             for (int t = 0; t < 8; t++) {
                 const int c = 8 * k + t;
@@ -419,11 +430,13 @@ void rp_stats(uint64_t out[4])
     out[3] = t_rows;
 }
 
-void rp_global(int window, uint64_t out[2])
+void rp_global(int window, uint64_t out[4])
 {
     rp_window = window;
     out[0] = t_gblocks;
     out[1] = t_gearly;
+    out[2] = t_prefix;
+    out[3] = t_gprefix;
 }
 
 void rp_why(uint64_t out[5])
@@ -434,6 +447,6 @@ void rp_why(uint64_t out[5])
 void rp_reset(void)
 {
     t_blocks = t_early = t_mismatch = t_rows = 0;
-    t_gblocks = t_gearly = 0;
+    t_gblocks = t_gearly = t_prefix = t_gprefix = 0;
     for (int k = 0; k < 5; k++) t_why[k] = 0;
 }
