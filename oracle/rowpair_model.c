@@ -13,7 +13,8 @@
  * and tail scans (:3269-3316) and from z-drop (:1868-1881), and the row runs
  * at all only if some lane of row i scored (:3205-3209).  While row i sweeps,
  * after each of its 8-column blocks (columns below P done), the model decides
- * whether block k = P/8 - 2 of row i+1 could run already, with sufficient
+ * whether block k = P/8 - 1 - lag of row i+1 could run already (lag 1 by
+ * default, rp_set_lag), with sufficient
  * rules over what row i has stored so far:
  *   - the group goes on: some lane has stored H > 0;
  *   - the range: the first column with any nonzero store is known, which
@@ -80,6 +81,8 @@ static uint64_t t_blocks, t_early, t_mismatch, t_rows;
  * from the block holding the row's start (the kernel keeps those in global
  * memory); 0 = no window */
 static int rp_window = 0;
+/* blocks row i+1 runs behind row i (1: block k once row i has done k+1) */
+static int rp_lag = 1;
 static uint64_t t_gblocks, t_gearly;
 /* early blocks in an unbroken run from the row's first block: the next row's
  * E and H pass from block to block, so only those can run early */
@@ -231,7 +234,7 @@ static void rp_coldone(int i, int j, const int16_t *F, const int16_t *Hh)
     (void)Hh;
     const int P = j + 1;
     if (P & 7) return;
-    const int k = P / 8 - 2, c0 = 8 * k;
+    const int k = P / 8 - 1 - rp_lag, c0 = 8 * k;
     if (k < 0 || k >= RP_MAXB || i + 1 >= g_nrow || p_row != i + 1) return;
     if (!s_anyh || s_c1 == RP_NONE) { t_why[0]++; return; } /* the group may stop */
     /* row i+1's range */
@@ -429,6 +432,8 @@ void rp_stats(uint64_t out[4])
     out[2] = t_mismatch;
     out[3] = t_rows;
 }
+
+void rp_set_lag(int lag) { rp_lag = lag < 1 ? 1 : lag; }
 
 void rp_global(int window, uint64_t out[4])
 {
